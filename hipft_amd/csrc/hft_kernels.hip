@@ -1,0 +1,822 @@
+// hft_kernels.hip -- HIP kernels of the flux-transport step for gfx950 (FP64).
+//
+// Every kernel works on whole batches of realizations.  Work is tiled as
+// (realization, theta row, block of phi columns); the 1-D launch index is
+// remapped so that each of the 8 XCDs receives a contiguous range of tiles
+// (neighbouring rows of one realization), which keeps the stencil's row
+// neighbours in that XCD's L2.
+//
+// Floating-point expressions follow the reference's operation order and the
+// library is built with -ffp-contract=off, so results match the CPU
+// restatement of src/hipft.f90 bit for bit (polar sums use the fixed order
+// of wave_pole_sum).
+#include "hft_device.h"
+
+namespace hft {
+
+static constexpr int kBlock = 256;
+
+// Tile decomposition -------------------------------------------------------
+struct Tiles {
+  int ctiles;  // column tiles per row
+  int ntm;
+  int nreal;   // realizations in the launch
+};
+
+__device__ inline int xcd_remap(int L, int T) {
+  // Blocks are dispatched round-robin over 8 XCDs (block L -> XCD L%8);
+  // give XCD x the contiguous tile range [start_x, start_x + size_x).
+  int per = T >> 3, rem = T & 7;
+  int x = L & 7, slot = L >> 3;
+  return x * per + (x < rem ? x : rem) + slot;
+}
+
+__device__ inline void tile_of(int t, const Tiles &g, int &z, int &j, int &ct) {
+  ct = t % g.ctiles;
+  int rj = t / g.ctiles;
+  j = rj % g.ntm;
+  z = rj / g.ntm;
+}
+
+__device__ inline int wrapc(int c, int nu) {
+  return c < 0 ? c + nu : (c >= nu ? c - nu : c);
+}
+
+// Sum over v(c), c = 0..nu-1, in the fixed order: lane l of a wavefront
+// accumulates c = l, l+64, ... in increasing c, then a butterfly combines
+// partials (l, l+32), (l, l+16), ... (bitwise the halving tree of the CPU
+// restatement).  Every lane returns the same value.
+template <typename F>
+__device__ inline double wave_pole_sum(int nu, F v) {
+  int lane = threadIdx.x & 63;
+  double s = 0.0;
+  for (int c = lane; c < nu; c += 64) s = s + v(c);
+  for (int h = 32; h >= 1; h >>= 1) s = s + __shfl_xor(s, h, 64);
+  return s;
+}
+
+// ---------------------------------------------------------------------------
+// Diffusion: 5-point surface Laplacian with polar closure
+// (diffusion_operator_cd, src/hipft.f90:7927-7990) for a constant
+// per-realization diffusivity D_r (coef from load_diffusion_matrix,
+// src/hipft.f90:4960-4981, recomputed in registers in reference order).
+// ---------------------------------------------------------------------------
+__device__ inline double lap_interior(const Mesh &m, double D, const double *x, int j, int c,
+                                      int nu, int pitch) {
+  // reference indices: jj = j+1, k = c+2
+  const double *row = x + (size_t)j * pitch;
+  double xc = row[c];
+  double xw = row[wrapc(c - 1, nu)];
+  double xe = row[wrapc(c + 1, nu)];
+  double xn = row[c - pitch];
+  double xs = row[c + pitch];
+  double sti = m.st_i[j];
+  double dph_lo = m.dph_i[c + 1], dph_hi = m.dph_i[c + 2], dpi = m.dp_i[c + 1];
+  double c1 = D * dph_lo;
+  c1 = c1 * dpi;
+  c1 = c1 * sti;
+  c1 = c1 * sti;
+  double c2 = D * m.dth_i[j];
+  c2 = c2 * m.dt_i[j];
+  c2 = c2 * sti;
+  c2 = c2 * m.sth[j];
+  double c4 = D * m.dth_i[j + 1];
+  c4 = c4 * m.dt_i[j];
+  c4 = c4 * sti;
+  c4 = c4 * m.sth[j + 1];
+  double c5 = D * dph_hi;
+  c5 = c5 * dpi;
+  c5 = c5 * sti;
+  c5 = c5 * sti;
+  double c3 = -(((c1 + c2) + c4) + c5);
+  double y = c1 * xw;
+  y = y + c2 * xn;
+  y = y + c3 * xc;
+  y = y + c4 * xs;
+  y = y + c5 * xe;
+  return y;
+}
+
+__device__ inline double lap_pole(const Mesh &m, double D, const double *x, bool north,
+                                  int pitch, int ntm, int nu) {
+  double D2 = D + D;
+  if (north) {
+    const double *r0 = x, *r1 = x + pitch;
+    double fn = wave_pole_sum(nu, [&](int c) { return (D2 * (r1[c] - r0[c])) * m.dp[c + 1]; });
+    return fn * m.bc_diff_n;
+  }
+  const double *r0 = x + (size_t)(ntm - 1) * pitch, *r1 = x + (size_t)(ntm - 2) * pitch;
+  double fs = wave_pole_sum(nu, [&](int c) { return (D2 * (r0[c] - r1[c])) * m.dp[c + 1]; });
+  return -fs * m.bc_diff_s;
+}
+
+__device__ inline double laplacian(const Mesh &m, double D, const double *x, int j, int c,
+                                   int ntm, int nu, int pitch) {
+  if (j == 0) return lap_pole(m, D, x, true, pitch, ntm, nu);
+  if (j == ntm - 1) return lap_pole(m, D, x, false, pitch, ntm, nu);
+  return lap_interior(m, D, x, j, c, nu, pitch);
+}
+
+// RKG2 first stage (src/hipft.f90:5211-5218):
+//   dty0 = dt*L(f);  u1 = f + ubj(1)*dty0
+struct Stage1Args {
+  Mesh m;
+  Tiles g;
+  const double *f;
+  double *dty0;
+  double *u1;
+  const double *D;       // [nr] diffusivity in code units
+  const double *dt;      // [nr] cycle time step
+  const double *ub1;     // [nr] sts_ubj(1)
+  const int *order;      // launch index -> realization
+  const double *Af;      // optional precomputed L(f) (PTL pass), may be null
+  size_t rstride;
+};
+
+__global__ __launch_bounds__(kBlock) void k_rkg2_stage1(Stage1Args a) {
+  int t = xcd_remap(blockIdx.x, gridDim.x);
+  int z, j, ct;
+  tile_of(t, a.g, z, j, ct);
+  int r = a.order[z];
+  const Mesh &m = a.m;
+  int c = ct * kBlock + threadIdx.x;
+  size_t base = (size_t)r * a.rstride;
+  const double *x = a.f + base;
+  double L;
+  if (a.Af) {
+    L = (c < m.nu) ? a.Af[base + (size_t)j * m.pitch + c] : 0.0;
+  } else {
+    // pole rows: every lane takes part in the wave sum
+    L = (j == 0 || j == m.ntm - 1 || c < m.nu) ? laplacian(m, a.D[r], x, j, c < m.nu ? c : 0,
+                                                           m.ntm, m.nu, m.pitch)
+                                               : 0.0;
+  }
+  if (c >= m.nu) return;
+  size_t p = base + (size_t)j * m.pitch + c;
+  double d0 = a.dt[r] * L;
+  a.dty0[p] = d0;
+  a.u1[p] = x[(size_t)j * m.pitch + c] + a.ub1[r] * d0;
+}
+
+// RKG2 stage k >= 2 (src/hipft.f90:5222-5235).  u_{k-2} is overwritten in
+// place (only the point itself reads it); the last stage of each
+// realization writes straight into the state array (which also holds u0).
+struct StageArgs {
+  Mesh m;
+  Tiles g;
+  const double *x;      // u_{k-1}
+  const double *ukm2;   // u_{k-2}
+  double *out;          // u_k unless last stage
+  double *f;            // state (u0), receives u_s
+  const double *dty0;
+  const double *D;
+  const StageCoef *coef;  // coefficients of this stage, [nr]
+  const int64_t *s;       // [nr] stage counts
+  const int *order;
+  int k;
+  size_t rstride;
+};
+
+__global__ __launch_bounds__(kBlock) void k_rkg2_stage(StageArgs a) {
+  int t = xcd_remap(blockIdx.x, gridDim.x);
+  int z, j, ct;
+  tile_of(t, a.g, z, j, ct);
+  int r = a.order[z];
+  const Mesh &m = a.m;
+  int c = ct * kBlock + threadIdx.x;
+  size_t base = (size_t)r * a.rstride;
+  const double *x = a.x + base;
+  double L = (j == 0 || j == m.ntm - 1 || c < m.nu)
+                 ? laplacian(m, a.D[r], x, j, c < m.nu ? c : 0, m.ntm, m.nu, m.pitch)
+                 : 0.0;
+  if (c >= m.nu) return;
+  size_t off = (size_t)j * m.pitch + c;
+  size_t p = base + off;
+  const StageCoef cf = a.coef[r];
+  double v = cf.a * x[off];
+  v = v + cf.b * a.ukm2[p];
+  v = v + cf.c * a.f[p];
+  v = v + cf.d * L;
+  v = v + cf.g * a.dty0[p];
+  double *o = (a.k == a.s[r]) ? a.f : a.out;
+  o[p] = v;
+}
+
+// Plain operator application y = L x (tests, PTL pass).
+struct LapArgs {
+  Mesh m;
+  Tiles g;
+  const double *x;
+  double *y;
+  const double *D;
+  const int *order;
+  size_t rstride;
+  // PTL: per-realization max |Lx| over points with |x| > 1e-14
+  unsigned long long *absmax;  // may be null
+};
+
+__global__ __launch_bounds__(kBlock) void k_laplacian(LapArgs a) {
+  int t = xcd_remap(blockIdx.x, gridDim.x);
+  int z, j, ct;
+  tile_of(t, a.g, z, j, ct);
+  int r = a.order[z];
+  const Mesh &m = a.m;
+  int c = ct * kBlock + threadIdx.x;
+  size_t base = (size_t)r * a.rstride;
+  const double *x = a.x + base;
+  double L = (j == 0 || j == m.ntm - 1 || c < m.nu)
+                 ? laplacian(m, a.D[r], x, j, c < m.nu ? c : 0, m.ntm, m.nu, m.pitch)
+                 : 0.0;
+  double cand = 0.0;
+  if (c < m.nu) {
+    size_t off = (size_t)j * m.pitch + c;
+    a.y[base + off] = L;
+    if (fabs(x[off]) > 1e-14) cand = fabs(L);
+  }
+  if (a.absmax) {
+    // wave max, then one atomic per wave (|Af| >= 0: integer order of the
+    // bit patterns is the numeric order)
+    for (int h = 32; h >= 1; h >>= 1) cand = fmax(cand, __shfl_xor(cand, h, 64));
+    if ((threadIdx.x & 63) == 0 && cand > 0.0)
+      atomicMax(&a.absmax[r], (unsigned long long)__double_as_longlong(cand));
+  }
+}
+
+// PTL step limit at the points where |Af| equals the maximum
+// (get_dtime_diffusion_ptl, src/hipft.f90:5109-5136).
+struct PtlArgs {
+  Mesh m;
+  Tiles g;
+  const double *f, *Af;
+  const unsigned long long *absmax;  // [nr]
+  unsigned long long *dtmin;         // [nr] bits of min(-du/dF)
+  const int *order;
+  int coupled;      // target = max over the launch's realizations
+  int nlaunch;
+  size_t rstride;
+};
+
+__global__ __launch_bounds__(kBlock) void k_ptl_min(PtlArgs a) {
+  int t = xcd_remap(blockIdx.x, gridDim.x);
+  int z, j, ct;
+  tile_of(t, a.g, z, j, ct);
+  int r = a.order[z];
+  const Mesh &m = a.m;
+  int c = ct * kBlock + threadIdx.x;
+  double target;
+  if (a.coupled) {
+    unsigned long long mx = 0;
+    for (int q = 0; q < a.nlaunch; q++) {
+      unsigned long long v = a.absmax[a.order[q]];
+      mx = v > mx ? v : mx;
+    }
+    target = __longlong_as_double((long long)mx);
+  } else {
+    target = __longlong_as_double((long long)a.absmax[r]);
+  }
+  double best = 1.7976931348623157e308;
+  if (target > 0.0 && c < m.nu && j > 0 && j < m.ntm - 1) {
+    size_t base = (size_t)r * a.rstride;
+    const double *f = a.f + base, *A = a.Af + base;
+    size_t off = (size_t)j * m.pitch + c;
+    double fc = f[off], ac = A[off];
+    if (target == fabs(ac) && fabs(fc) > 1e-14) {
+      size_t nb[4] = {off - m.pitch, off + m.pitch, (size_t)j * m.pitch + wrapc(c - 1, m.nu),
+                      (size_t)j * m.pitch + wrapc(c + 1, m.nu)};
+      for (int q = 0; q < 4; q++) {
+        double du = fc - f[nb[q]];
+        double dF = ac - A[nb[q]];
+        if (du * dF < -1e-14) best = fmin(-du / dF, best);
+      }
+    }
+  }
+  for (int h = 32; h >= 1; h >>= 1) best = fmin(best, __shfl_xor(best, h, 64));
+  if ((threadIdx.x & 63) == 0 && best < 1.7976931348623157e308)
+    atomicMin(&a.dtmin[r], (unsigned long long)__double_as_longlong(best));
+}
+
+// ---------------------------------------------------------------------------
+// Advection: WENO3-CS(h) with local Lax-Friedrichs splitting on the
+// nonuniform grid (advection_operator_weno3, src/hipft.f90:7711-7925) fused
+// with one SSPRK(4,3) stage (src/hipft.f90:7551-7574).
+// ---------------------------------------------------------------------------
+struct WenoW {
+  double mcm, cpm, mc0, cp0, ptm, mctm, mt0, cpt0, em, e0;
+};
+
+__device__ inline WenoW weno_w(const double *CP, const double *MC, const double *MT,
+                               const double *CPT, const double *PT, const double *MCT,
+                               const double *eps, int im1, int i0) {
+  WenoW w;
+  w.mcm = MC[im1];
+  w.cpm = CP[im1];
+  w.mc0 = MC[i0];
+  w.cp0 = CP[i0];
+  w.ptm = PT[im1];
+  w.mctm = MCT[im1];
+  w.mt0 = MT[i0];
+  w.cpt0 = CPT[i0];
+  w.em = eps[im1];
+  w.e0 = eps[i0];
+  return w;
+}
+
+// One face flux from the split fluxes at the six surrounding cells
+// (src/hipft.f90:7775-7801, 7852-7878).
+__device__ inline double weno_face(double LNm2, double LNm1, double LN0, double LPm1, double LP0,
+                                   double LPp1, const WenoW &w) {
+  const double one = 1.0, four = 4.0;
+  double p0m = (one + w.mcm) * LNm1 - w.mcm * LNm2;
+  double p1m = w.mc0 * LNm1 + w.cpm * LN0;
+  double p0p = (one + w.cp0) * LP0 - w.cp0 * LPp1;
+  double p1p = w.cpm * LP0 + w.mc0 * LPm1;
+  double t0 = w.mcm * (LNm1 - LNm2), t1 = w.cpm * (LN0 - LNm1);
+  double t2 = w.cp0 * (LPp1 - LP0), t3 = w.mc0 * (LP0 - LPm1);
+  double B0m = four * (t0 * t0), B1m = four * (t1 * t1);
+  double B0p = four * (t2 * t2), B1p = four * (t3 * t3);
+  double e0 = w.em + B0m, e1 = w.em + B1m, e2 = w.e0 + B0p, e3 = w.e0 + B1p;
+  double w0m = w.ptm / (e0 * e0);
+  double w1m = w.mctm / (e1 * e1);
+  double w0p = w.mt0 / (e2 * e2);
+  double w1p = w.cpt0 / (e3 * e3);
+  double wm_sum = w0m + w1m, wp_sum = w0p + w1p;
+  double OM0m = w0m / wm_sum, OM1m = w1m / wm_sum;
+  double OM0p = w0p / wp_sum, OM1p = w1p / wp_sum;
+  double um = OM0m * p0m + OM1m * p1m;
+  double up = OM0p * p0p + OM1p * p1p;
+  return up + um;
+}
+
+// sign(1, v) of Fortran: -1 for negative values (and -0.0), else +1.
+__device__ inline double fsign1(double v) { return signbit(v) ? -1.0 : 1.0; }
+
+template <bool F2D>
+struct FlowAcc {
+  Flow1D f1;
+  Flow2D f2;
+  int nt, ntm, pitch, nu;
+  size_t rstride;
+  int r;
+  // vt at theta face jf (0-based ref jf-1), column c
+  __device__ double vt(int jf0, int c) const {
+    if (F2D) return f2.vt[(size_t)r * f2.vt_rstride + (size_t)jf0 * pitch + c];
+    return f1.vt[(size_t)r * nt + jf0];
+  }
+  __device__ double at(int j, int c) const {
+    if (F2D) return f2.at[(size_t)r * rstride + (size_t)j * pitch + c];
+    return f1.at[(size_t)r * ntm + j];
+  }
+  // vp at phi face q (q = reference k-2 of the face), row j
+  __device__ double vp(int j, int q) const {
+    if (F2D) return f2.vp[(size_t)r * rstride + (size_t)j * pitch + q];
+    return f1.vp[(size_t)r * ntm + j];
+  }
+  __device__ double ap(int j, int c) const {
+    if (F2D) return f2.ap[(size_t)r * rstride + (size_t)j * pitch + c];
+    return f1.ap[(size_t)r * ntm + j];
+  }
+};
+
+struct AdvArgs {
+  Mesh m;
+  Tiles g;
+  Flow1D f1;
+  Flow2D f2;
+  const double *x;   // operator input
+  const double *y;   // second input of stage 3 (may be null)
+  double *out;
+  double ca, cb, cdt;  // out = ca*y + cb*x - cdt*A(x)   (or x - cdt*A(x))
+  const int *order;
+  size_t rstride;
+  int write_aop;       // operator-only mode: out = A(x)
+};
+
+template <bool F2D>
+__device__ inline double theta_face_upwind(const FlowAcc<F2D> &fa, const double *x, int jf0,
+                                           int c, int pitch, int ntm) {
+  // jf0 = 1 (reference face 2) or nt-2 (reference face nt-1)
+  const double one = 1.0, half = 0.5;
+  double v = fa.vt(jf0, c);
+  double cct = fsign1(v);
+  double xa, xb;  // (one-cct) multiplies xa, (one+cct) multiplies xb
+  if (jf0 == 1) {
+    xa = x[(size_t)1 * pitch + c];
+    xb = x[c];
+  } else {
+    xa = x[(size_t)(ntm - 1) * pitch + c];
+    xb = x[(size_t)(ntm - 2) * pitch + c];
+  }
+  return v * half * ((one - cct) * xa + (one + cct) * xb);
+}
+
+template <bool F2D>
+__device__ inline double theta_face_weno(const Mesh &m, const FlowAcc<F2D> &fa, const double *x,
+                                         int jf, int c) {
+  // reference face jf (3..nt-2): cells jf-2..jf+1 (reference) = rows jf-3..jf
+  const double half = 0.5;
+  double LN[3], LP[3];
+  for (int q = 0; q < 3; q++) {
+    int jr = jf - 2 + q;  // reference cell for LN: jf-2, jf-1, jf
+    int j0 = jr - 1;
+    double xv = x[(size_t)j0 * m.pitch + c];
+    double a = fa.at(j0, c);
+    LN[q] = half * xv * (fa.vt(jr - 1, c) + a);
+  }
+  for (int q = 0; q < 3; q++) {
+    int jr = jf - 1 + q;  // reference cell for LP: jf-1, jf, jf+1
+    int j0 = jr - 1;
+    double xv = x[(size_t)j0 * m.pitch + c];
+    double a = fa.at(j0, c);
+    LP[q] = half * xv * (fa.vt(jr, c) - a);
+  }
+  WenoW w = weno_w(m.tCP, m.tMC, m.tMT, m.tCPT, m.tPT, m.tMCT, m.teps, jf - 2, jf - 1);
+  return weno_face(LN[0], LN[1], LN[2], LP[0], LP[1], LP[2], w);
+}
+
+template <bool F2D>
+__device__ inline double phi_face_weno(const Mesh &m, const FlowAcc<F2D> &fa, const double *xrow,
+                                       int j, int q) {
+  // face q <-> reference k = q+2; cells k-2..k+1 <-> columns q-2..q+1
+  const double half = 0.5;
+  int nu = m.nu;
+  double LN[3], LP[3];
+  for (int d = 0; d < 3; d++) {
+    int cc = wrapc(q - 2 + d, nu);
+    double xv = xrow[cc];
+    double a = fa.ap(j, cc);
+    LN[d] = half * xv * (fa.vp(j, cc) + a);  // vp at the cell's west face
+  }
+  for (int d = 0; d < 3; d++) {
+    int cc = wrapc(q - 1 + d, nu);
+    double xv = xrow[cc];
+    double a = fa.ap(j, cc);
+    LP[d] = half * xv * (fa.vp(j, wrapc(cc + 1, nu)) - a);  // east face
+  }
+  WenoW w = weno_w(m.pCP, m.pMC, m.pMT, m.pCPT, m.pPT, m.pMCT, m.peps, q, q + 1);
+  return weno_face(LN[0], LN[1], LN[2], LP[0], LP[1], LP[2], w);
+}
+
+template <bool F2D>
+__device__ inline double advection_op(const Mesh &m, const FlowAcc<F2D> &fa, const double *x,
+                                      int j, int c) {
+  int ntm = m.ntm, nt = m.nt, pitch = m.pitch, nu = m.nu;
+  if (j == 0 || j == ntm - 1) {
+    bool north = j == 0;
+    int jf0 = north ? 1 : nt - 2;
+    double s = wave_pole_sum(nu, [&](int cc) {
+      return theta_face_upwind(fa, x, jf0, cc, pitch, ntm) * m.dp[cc + 1];
+    });
+    return north ? s * m.bc_flow_n : -s * m.bc_flow_s;
+  }
+  int jr = j + 1;  // reference cell
+  double ft0 = (jr == 2) ? theta_face_upwind(fa, x, 1, c, pitch, ntm)
+                         : theta_face_weno(m, fa, x, jr, c);
+  double ft1 = (jr + 1 == nt - 1) ? theta_face_upwind(fa, x, nt - 2, c, pitch, ntm)
+                                  : theta_face_weno(m, fa, x, jr + 1, c);
+  const double *xrow = x + (size_t)j * pitch;
+  double fp0 = phi_face_weno(m, fa, xrow, j, c);
+  double fp1 = phi_face_weno(m, fa, xrow, j, wrapc(c + 1, nu));
+  double sti = m.st_i[j];
+  double at_ = (m.sth[j + 1] * ft1 - m.sth[j] * ft0) * sti;
+  at_ = at_ * m.dt_i[j];
+  double ap_ = (fp1 - fp0) * sti;
+  ap_ = ap_ * m.dp_i[c + 1];
+  return at_ + ap_;
+}
+
+template <bool F2D, bool HAS_Y>
+__global__ __launch_bounds__(kBlock) void k_advection_stage(AdvArgs a) {
+  int t = xcd_remap(blockIdx.x, gridDim.x);
+  int z, j, ct;
+  tile_of(t, a.g, z, j, ct);
+  int r = a.order[z];
+  const Mesh &m = a.m;
+  int c = ct * kBlock + threadIdx.x;
+  FlowAcc<F2D> fa;
+  fa.f1 = a.f1;
+  fa.f2 = a.f2;
+  fa.nt = m.nt;
+  fa.ntm = m.ntm;
+  fa.pitch = m.pitch;
+  fa.nu = m.nu;
+  fa.rstride = a.rstride;
+  fa.r = r;
+  size_t base = (size_t)r * a.rstride;
+  const double *x = a.x + base;
+  bool pole = (j == 0 || j == m.ntm - 1);
+  if (!pole && c >= m.nu) return;
+  double A = advection_op<F2D>(m, fa, x, j, c < m.nu ? c : 0);
+  if (c >= m.nu) return;
+  size_t off = (size_t)j * m.pitch + c;
+  double v;
+  if (a.write_aop) {
+    v = A;
+  } else if (HAS_Y) {
+    v = a.ca * a.y[base + off] + a.cb * x[off];
+    v = v - a.cdt * A;
+  } else {
+    v = x[off] - a.cdt * A;
+  }
+  a.out[base + off] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Analysis (analysis_step, src/hipft.f90:3145-3325): per-block partials then
+// a fixed-order final reduction.
+// ---------------------------------------------------------------------------
+static constexpr int kNPart = 16;
+struct AnaArgs {
+  const double *f;
+  const double *fval;        // validation field in device layout or null
+  const double *fval_scale;  // [nr] or null
+  int ntm, nu, pitch, nr;
+  size_t rstride;
+  // 1-D weights: theta [ntm]: da_t, sn_t, cs_t, t; phi [npm1] over the
+  // reference's i = 1..npm-1: da_p, cs_p, sn_p, column index
+  const double *da_t, *sn_t, *cs_t, *tt;
+  const double *da_p, *cs_p, *sn_p;
+  const int *pcol;
+  double lat_n, lat_s;   // pole_flux_lat_limit*d2r and pi - that
+  int rows_per_block;
+  double *part;          // [nr][nblk][kNPart]
+  int nblk;
+};
+
+__device__ inline void block_reduce(double *sh, double v, int op, double *out) {
+  // op 0 sum, 1 min, 2 max; fixed tree order
+  int tid = threadIdx.x;
+  sh[tid] = v;
+  __syncthreads();
+  for (int h = kBlock / 2; h >= 1; h >>= 1) {
+    if (tid < h) {
+      double a = sh[tid], b = sh[tid + h];
+      sh[tid] = op == 0 ? a + b : (op == 1 ? fmin(a, b) : fmax(a, b));
+    }
+    __syncthreads();
+  }
+  if (tid == 0) *out = sh[0];
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(kBlock) void k_analysis_part(AnaArgs a) {
+  __shared__ double sh[kBlock];
+  int r = blockIdx.y, b = blockIdx.x;
+  const double huge = 1.7976931348623157e308, tiny = 2.2250738585072014e-308;
+  double mn = huge, mx = tiny, mnabs = huge, verr = 0.0, sfv = 0.0;
+  double fp = 0, fm = 0, fppn = 0, fmpn = 0, fpps = 0, fmps = 0, apn = 0, aps = 0;
+  double e1 = 0, e2 = 0, ax = 0;
+  int j0 = b * a.rows_per_block, j1 = min(a.ntm, j0 + a.rows_per_block);
+  size_t base = (size_t)r * a.rstride;
+  double scale = a.fval_scale ? a.fval_scale[r] : 1.0;
+  for (int j = j0; j < j1; j++) {
+    const double *row = a.f + base + (size_t)j * a.pitch;
+    const double *vrow = a.fval ? a.fval + base + (size_t)j * a.pitch : nullptr;
+    for (int c = threadIdx.x; c < a.nu; c += kBlock) {
+      double f = row[c];
+      double fv = vrow ? (a.fval_scale ? vrow[c] * scale : vrow[c]) : f;
+      mn = fmin(f, mn);
+      mx = fmax(f, mx);
+      mnabs = fmin(fabs(f), mnabs);
+      double d = f - fv;
+      verr = verr + d * d;
+      sfv = sfv + fabs(f) * fabs(fv);
+    }
+    double dat = a.da_t[j], snt = a.sn_t[j], cst = a.cs_t[j], tj = a.tt[j];
+    for (int i = threadIdx.x; i < a.nu + 1; i += kBlock) {
+      double f = row[a.pcol[i]];
+      double dap = a.da_p[i];
+      double fv = f * dat;
+      fv = fv * dap;
+      if (fv > 0.0) fp = fp + fv; else fm = fm + fv;
+      double area = dat * dap;
+      if (tj <= a.lat_n) {
+        if (fv > 0.0) fppn = fppn + fv; else fmpn = fmpn + fv;
+        apn = apn + area;
+      }
+      if (tj >= a.lat_s) {
+        if (fv > 0.0) fpps = fpps + fv; else fmps = fmps + fv;
+        aps = aps + area;
+      }
+      double q = f * snt;
+      e1 = e1 + ((q * a.cs_p[i]) * dat) * dap;
+      e2 = e2 + ((q * a.sn_p[i]) * dat) * dap;
+      ax = ax + ((f * cst) * dat) * dap;
+    }
+  }
+  double *o = a.part + ((size_t)r * a.nblk + b) * kNPart;
+  double vals[kNPart] = {mn, mx, mnabs, verr, sfv, fp, fm, fppn, fmpn, fpps, fmps, apn, aps, e1, e2, ax};
+  int ops[kNPart] = {1, 2, 1, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int q = 0; q < kNPart; q++) block_reduce(sh, vals[q], ops[q], o + q);
+}
+
+__global__ void k_analysis_final(const double *part, int nblk, int nr, int ntm, int npm,
+                                 double *out) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nr) return;
+  const double huge = 1.7976931348623157e308, tiny = 2.2250738585072014e-308;
+  double acc[kNPart];
+  for (int q = 0; q < kNPart; q++) acc[q] = 0.0;
+  acc[0] = huge;
+  acc[1] = tiny;
+  acc[2] = huge;
+  for (int b = 0; b < nblk; b++) {
+    const double *p = part + ((size_t)r * nblk + b) * kNPart;
+    acc[0] = fmin(acc[0], p[0]);
+    acc[1] = fmax(acc[1], p[1]);
+    acc[2] = fmin(acc[2], p[2]);
+    for (int q = 3; q < kNPart; q++) acc[q] = acc[q] + p[q];
+  }
+  const double rsun_cm2 = 4.84416e21, three_quarter = 0.75, pi_i = 0.3183098861837907;
+  double *o = out + (size_t)r * 14;
+  o[0] = acc[0];
+  o[1] = acc[1];
+  o[2] = acc[2];
+  o[13] = (acc[4] == 0.0) ? sqrt(acc[3] / (double)(ntm * (npm - 1))) : sqrt(acc[3] / acc[4]);
+  for (int q = 0; q < 8; q++) o[3 + q] = rsun_cm2 * acc[5 + q];
+  o[12] = three_quarter * pi_i * acc[15];
+  double e1 = three_quarter * pi_i * acc[13];
+  double e2 = three_quarter * pi_i * acc[14];
+  o[11] = sqrt(e1 * e1 + e2 * e2);
+}
+
+// ---------------------------------------------------------------------------
+// Data assimilation blend (src/hipft.f90:4242-4271, 4115-4141).
+// ---------------------------------------------------------------------------
+struct AssimArgs {
+  Mesh m;
+  Tiles g;
+  double *f;
+  double *delta;            // optional: store delta instead of applying (balance)
+  const double *map;        // [(nr)][3][ntm][npm1]
+  size_t map_rstride;       // 0 when shared by all realizations
+  int npm1;
+  int custom;
+  const double *mu_power, *mu_limit;
+  const int *tidx0, *tidx1; // 1-based lat-limit indices, -1 = none; used if lat_on
+  const int *lat_on;
+  double mult;
+  int add_mode;
+  const int *order;
+  size_t rstride;
+};
+
+__global__ __launch_bounds__(kBlock) void k_assimilate(AssimArgs a) {
+  int t = xcd_remap(blockIdx.x, gridDim.x);
+  int z, j, ct;
+  tile_of(t, a.g, z, j, ct);
+  int r = a.order[z];
+  int c = ct * kBlock + threadIdx.x;
+  if (c >= a.m.nu) return;
+  // unique column c <-> reference k = c+2 <-> map column l = k (1-based)
+  const double *mp = a.map + (size_t)r * a.map_rstride;
+  size_t sl = (size_t)a.m.ntm * a.npm1;
+  size_t q = (size_t)j * a.npm1 + (c + 1);
+  double d = mp[q], w = mp[sl + q];
+  if (a.custom) {
+    double mu = mp[2 * sl + q];
+    w = (mu > a.mu_limit[r]) ? pow(mu, a.mu_power[r]) : 0.0;
+  }
+  if (a.lat_on[r]) {
+    int jr = j + 1;
+    if (jr <= a.tidx0[r] || (a.tidx1[r] >= 1 && jr >= a.tidx1[r])) w = 0.0;
+  }
+  d = a.mult * d;
+  size_t p = (size_t)r * a.rstride + (size_t)j * a.m.pitch + c;
+  double fo = a.f[p];
+  double dl = a.add_mode ? w * d : w * (d - fo);
+  if (a.delta)
+    a.delta[p] = dl;
+  else
+    a.f[p] = fo + dl;
+}
+
+// Flux of a field per realization (get_flux, src/hipft.f90:8672-8738) as
+// per-block partials [nr][nblk][2]; reuses the analysis weights.
+__global__ __launch_bounds__(kBlock) void k_flux_part(AnaArgs a) {
+  __shared__ double sh[kBlock];
+  int r = blockIdx.y, b = blockIdx.x;
+  double fp = 0, fm = 0;
+  int j0 = b * a.rows_per_block, j1 = min(a.ntm, j0 + a.rows_per_block);
+  for (int j = j0; j < j1; j++) {
+    const double *row = a.f + (size_t)r * a.rstride + (size_t)j * a.pitch;
+    double dat = a.da_t[j];
+    for (int i = threadIdx.x; i < a.nu + 1; i += kBlock) {
+      double fv = row[a.pcol[i]] * dat;
+      fv = fv * a.da_p[i];
+      if (fv > 0.0) fp = fp + fv; else fm = fm + fv;
+    }
+  }
+  double *o = a.part + ((size_t)r * a.nblk + b) * 2;
+  block_reduce(sh, fp, 0, o);
+  block_reduce(sh, fm, 0, o + 1);
+}
+
+// f += balanced delta (balance_flux, src/hipft.f90:8788-8801).
+__global__ __launch_bounds__(kBlock) void k_apply_balanced(double *f, const double *delta,
+                                                           const double *ratio, int ntm, int nu,
+                                                           int pitch, size_t rstride) {
+  int r = blockIdx.y;
+  size_t n = (size_t)ntm * pitch;
+  for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) {
+    int c = (int)(i % pitch);
+    if (c >= nu) continue;
+    size_t p = (size_t)r * rstride + i;
+    double br = delta[p];
+    double rt = ratio[r];
+    double d = (rt == 0.0) ? br : (br > 0 ? br / rt : br * rt);
+    f[p] = f[p] + d;
+  }
+}
+
+// Attenuated flows (update_flow, src/hipft.f90:3489-3507): from base 1-D
+// profiles and the state, build 2-D vt (nt rows) and vp (ntm rows, faces q).
+struct AttenArgs {
+  const double *f;
+  const double *vt1, *vp1;   // [nr][nt], [nr][ntm] unattenuated profiles
+  double *vt, *vp;           // 2-D outputs
+  const double *inv;         // [nr] 1/flow_attenuate_value
+  int ntm, nt, nu, pitch;
+  size_t rstride, vt_rstride;
+};
+
+__global__ __launch_bounds__(kBlock) void k_attenuate(AttenArgs a) {
+  int r = blockIdx.z, j = blockIdx.y;
+  int c = blockIdx.x * kBlock + threadIdx.x;
+  if (c >= a.nu) return;
+  const double one = 1.0;
+  const double *f = a.f + (size_t)r * a.rstride;
+  double inv = a.inv[r];
+  // vt(jf) for jf = 1..nt (0-based j = jf-1 < nt): attenuated for jf = 2..nt-1
+  if (j < a.nt) {
+    double v = a.vt1[(size_t)r * a.nt + j];
+    if (j >= 1 && j <= a.nt - 2) {
+      double b0 = fabs(f[(size_t)(j - 1) * a.pitch + c]), b1 = fabs(f[(size_t)j * a.pitch + c]);
+      double m = b0 > b1 ? b0 : b1;
+      v = v * (one - tanh(m * inv));
+    }
+    a.vt[(size_t)r * a.vt_rstride + (size_t)j * a.pitch + c] = v;
+  }
+  // vp at face q = c (reference k = c+2, between columns c-1 and c)
+  if (j < a.ntm) {
+    double v = a.vp1[(size_t)r * a.ntm + j];
+    double b0 = fabs(f[(size_t)j * a.pitch + wrapc(c - 1, a.nu)]), b1 = fabs(f[(size_t)j * a.pitch + c]);
+    double m = b0 > b1 ? b0 : b1;
+    v = v * (one - tanh(m * inv));
+    a.vp[(size_t)r * a.rstride + (size_t)j * a.pitch + c] = v;
+  }
+}
+
+// Lax-Friedrichs alphas for 2-D flows (set_lf_alpha, src/hipft.f90:6968-7046).
+__global__ __launch_bounds__(kBlock) void k_lf_alpha(const double *vt, const double *vp,
+                                                     double *at, double *ap, int ntm, int nt,
+                                                     int nu, int pitch, size_t rstride,
+                                                     size_t vt_rstride) {
+  int r = blockIdx.z, j = blockIdx.y;
+  int c = blockIdx.x * kBlock + threadIdx.x;
+  if (c >= nu) return;
+  const double *V = vt + (size_t)r * vt_rstride;
+  // alpha_t(jr) for jr = 1..nt-1 (= ntm): window of |vt| faces
+  int jr = j + 1;
+  int lo, hi;
+  if (jr == 1) { lo = 1; hi = 4; }
+  else if (jr == 2) { lo = 1; hi = 5; }
+  else if (jr == nt - 2) { lo = jr - 2; hi = jr + 2; }
+  else if (jr == nt - 1) { lo = jr - 2; hi = jr + 1; }
+  else { lo = jr - 2; hi = jr + 3; }
+  double m = fabs(V[(size_t)(lo - 1) * pitch + c]);
+  for (int q = lo + 1; q <= hi; q++) m = fmax(m, fabs(V[(size_t)(q - 1) * pitch + c]));
+  at[(size_t)r * rstride + (size_t)j * pitch + c] = m;
+  // alpha_p: rows 2..ntm-1, window k-2..k+3 over faces (periodic)
+  double mp = 0.0;
+  if (jr >= 2 && jr <= ntm - 1) {
+    const double *P = vp + (size_t)r * rstride + (size_t)j * pitch;
+    mp = fabs(P[wrapc(c - 2, nu)]);
+    for (int d = -1; d <= 3; d++) mp = fmax(mp, fabs(P[wrapc(c + d, nu)]));
+  }
+  ap[(size_t)r * rstride + (size_t)j * pitch + c] = mp;
+}
+
+// CFL limit of 2-D flows (get_flow_dtmax, src/hipft.f90:7321-7327):
+// per-realization min of 1/2/max(kdotv, tiny) as bits (positive doubles).
+__global__ __launch_bounds__(kBlock) void k_flow_dtmax(const double *vt, const double *vp,
+                                                       const double *dt_i, const double *st_i,
+                                                       const double *dp_i, int ntm, int nu,
+                                                       int pitch, size_t rstride,
+                                                       size_t vt_rstride,
+                                                       unsigned long long *out) {
+  int r = blockIdx.z, j = blockIdx.y + 1;  // rows 1..ntm-2
+  int c = blockIdx.x * kBlock + threadIdx.x;
+  double best = 1.7976931348623157e308;
+  if (c < nu && j <= ntm - 2) {
+    const double *V = vt + (size_t)r * vt_rstride;
+    const double *P = vp + (size_t)r * rstride + (size_t)j * pitch;
+    double a = fmax(fabs(V[(size_t)j * pitch + c]), fabs(V[(size_t)(j + 1) * pitch + c]));
+    double b = fmax(fabs(P[c]), fabs(P[wrapc(c + 1, nu)]));
+    double kdotv = a * dt_i[j] + b * st_i[j] * dp_i[c + 1];
+    best = 0.5 / fmax(kdotv, 2.2250738585072014e-308);
+  }
+  for (int h = 32; h >= 1; h >>= 1) best = fmin(best, __shfl_xor(best, h, 64));
+  if ((threadIdx.x & 63) == 0) atomicMin(&out[r], (unsigned long long)__double_as_longlong(best));
+}
+
+}  // namespace hft

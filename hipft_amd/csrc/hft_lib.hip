@@ -1,0 +1,1391 @@
+// hft_lib.hip -- libhft: host controller and C ABI (include/hft.h) of the
+// MI355X flux-transport core.  Kernels live in hft_kernels.hip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/hft.h"
+#include "hft_kernels.hip"
+
+using namespace hft;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPCHK(x)                                                                  \
+  do {                                                                             \
+    hipError_t e_ = (x);                                                           \
+    if (e_ != hipSuccess)                                                          \
+      return fail(HFT_E_HIP, "%s failed: %s (%s:%d)", #x, hipGetErrorString(e_),  \
+                  __FILE__, __LINE__);                                             \
+  } while (0)
+
+// Reference constants (src/hipft.f90:83-136).
+const double zero = 0.0, one = 1.0, two = 2.0, three = 3.0, four = 4.0, six = 6.0;
+const double ten = 10.0, fifteen = 15.0, half = 0.5, quarter = 0.25;
+const double twentyfour = 24.0, twentyfive = 25.0;
+const double sixth = 0.16666666666666666, two_third = 0.66666666666666666;
+const double third = 0.33333333333333333;
+const double pi = 3.1415926535897932, pi_i = 0.3183098861837907;
+const double twopi = 6.2831853071795864, twopi_i = 0.15915494309189535;
+const double d2r = 0.017453292519943295;
+const double km_s_to_rs_hr = 0.005172413793103448, m_s_to_rs_hr = 5.172413793103448e-06;
+const double safety = 0.95;
+const double kHuge = 1.7976931348623157e308, kTiny = 2.2250738585072014e-308;
+
+// x**n, integer n >= 0, by square-and-multiply (the expansion compilers emit
+// for an integer exponent).
+double ipow(double x, int n) {
+  double res = 1.0, sq = x;
+  bool have = false;
+  while (n) {
+    if (n & 1) {
+      res = have ? res * sq : sq;
+      have = true;
+    }
+    n >>= 1;
+    if (n) sq = sq * sq;
+  }
+  return res;
+}
+
+template <typename T>
+int dalloc(T **p, size_t n) {
+  if (n == 0) n = 1;
+  hipError_t e = hipMalloc((void **)p, n * sizeof(T));
+  if (e != hipSuccess) return fail(HFT_E_NOMEM, "hipMalloc(%zu bytes): %s", n * sizeof(T), hipGetErrorString(e));
+  return 0;
+}
+
+struct TimedPair {
+  hipEvent_t a, b;
+  int group;
+};
+
+const char *kGroups[] = {"rkg2", "weno", "ptl", "analysis", "assim", "flow"};
+constexpr int kNGroups = 6;
+
+}  // namespace
+
+struct hft_ctx {
+  int ntm, npm1, npm, nt, np, nu, pitch, nr, mode, device;
+  size_t rstride, nfield;
+  // host mesh, reference indexing (element 0 unused)
+  std::vector<double> t, p, th, ph, dt, dth, dp, dph, dt_i, dth_i, dp_i, dph_i, st, sth, ct,
+      cth, st_i, sth_i;
+  std::vector<double> wt[7], wp[7];  // WENO weights CP MC MT CPT PT MCT eps
+  hipStream_t stream = nullptr;
+  // device mesh storage
+  std::vector<double *> dmesh;
+  Mesh m;
+  // fields
+  double *f = nullptr, *buf0 = nullptr, *buf1 = nullptr, *dty0 = nullptr, *fval = nullptr;
+  bool have_fval = false;
+  // flows
+  bool flow_set = false, flow2d = false;
+  std::vector<double> prof_vt, prof_vp;  // base profiles (code units) or empty
+  std::vector<double> an_p0, an_p2, an_p4, an_p1, an_p3, an_p5;
+  double an_omega = 0, an_vtc = 0;
+  std::vector<double> vt1, vp1, at1, ap1;  // [nr][nt], [nr][ntm] x3
+  double *d_vt1 = nullptr, *d_vp1 = nullptr, *d_at1 = nullptr, *d_ap1 = nullptr;
+  std::vector<double> atten_inv;
+  bool atten = false;
+  double *d_atten_inv = nullptr, *d_vt2 = nullptr, *d_vp2 = nullptr, *d_at2 = nullptr,
+         *d_ap2 = nullptr;
+  size_t vt_rstride = 0;
+  // diffusion
+  bool diff_set = false;
+  std::vector<double> D, dt_euler_r;
+  double dt_euler = 0;
+  double *d_D = nullptr;
+  // RKG2 / PTL work arrays
+  StageCoef *d_coef = nullptr;
+  size_t coef_cap = 0;
+  StageCoef *h_coef = nullptr;  // pinned
+  size_t h_coef_cap = 0;
+  hipEvent_t coef_ev = nullptr;
+  int64_t *d_s = nullptr;
+  int *d_order = nullptr, *d_order_id = nullptr;
+  double *d_cdt = nullptr, *d_ub1 = nullptr;
+  unsigned long long *d_absmax = nullptr, *d_dtmin = nullptr, *d_target = nullptr;
+  int *h_small = nullptr;  // pinned scratch for small uploads
+  double *h_smalld = nullptr;
+  std::vector<int64_t> last_s, last_cycles;
+  // analysis
+  double *d_part = nullptr, *d_hist = nullptr;
+  double *d_da_t = nullptr, *d_sn_t = nullptr, *d_cs_t = nullptr, *d_tt = nullptr;
+  double *d_da_p = nullptr, *d_cs_p = nullptr, *d_sn_p = nullptr;
+  int *d_pcol = nullptr;
+  int ana_nblk = 0, ana_rows = 0;
+  // assimilation
+  double *d_map = nullptr;
+  size_t map_cap = 0;
+  double *d_apar = nullptr;
+  int *d_ipar = nullptr;
+  // timing
+  bool timing = false;
+  std::vector<TimedPair> pending;
+  std::vector<hipEvent_t> free_ev;
+  double t_ms[kNGroups] = {0};
+  int64_t t_n[kNGroups] = {0};
+};
+
+namespace {
+
+int ev_get(hft_ctx *c, hipEvent_t *e) {
+  if (!c->free_ev.empty()) {
+    *e = c->free_ev.back();
+    c->free_ev.pop_back();
+    return 0;
+  }
+  HIPCHK(hipEventCreate(e));
+  return 0;
+}
+
+int drain_timing(hft_ctx *c) {
+  if (c->pending.empty()) return 0;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (auto &tp : c->pending) {
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, tp.a, tp.b));
+    c->t_ms[tp.group] += ms;
+    c->t_n[tp.group] += 1;
+    c->free_ev.push_back(tp.a);
+    c->free_ev.push_back(tp.b);
+  }
+  c->pending.clear();
+  return 0;
+}
+
+// Launch helper: records events around the launch when timing is enabled.
+template <typename K, typename A>
+int launch(hft_ctx *c, int group, K kern, int blocks, const A &args) {
+  if (blocks <= 0) return 0;
+  hipEvent_t a = nullptr, b = nullptr;
+  if (c->timing) {
+    if (c->pending.size() > 4096) {
+      int rc = drain_timing(c);
+      if (rc) return rc;
+    }
+    int rc = ev_get(c, &a);
+    if (!rc) rc = ev_get(c, &b);
+    if (rc) return rc;
+    HIPCHK(hipEventRecord(a, c->stream));
+  }
+  hipLaunchKernelGGL(kern, dim3(blocks), dim3(kBlock), 0, c->stream, args);
+  HIPCHK(hipGetLastError());
+  if (c->timing) {
+    HIPCHK(hipEventRecord(b, c->stream));
+    c->pending.push_back({a, b, group});
+  }
+  return 0;
+}
+
+Tiles tiles_for(const hft_ctx *c, int nreal) {
+  Tiles g;
+  g.ctiles = (c->nu + kBlock - 1) / kBlock;
+  g.ntm = c->ntm;
+  g.nreal = nreal;
+  return g;
+}
+int nblocks(const Tiles &g) { return g.ctiles * g.ntm * g.nreal; }
+
+// ---- set_mesh: src/hipft.f90:6495-6653 ------------------------------------
+void build_mesh(hft_ctx *c, const double *tin, const double *pin) {
+  int ntm = c->ntm, npm1 = c->npm1, nt = c->nt, np = c->np, npm = c->npm;
+  auto V = [](int n) { return std::vector<double>(n + 1, 0.0); };
+  c->t = V(ntm);
+  c->p = V(np);
+  for (int i = 1; i <= ntm; i++) c->t[i] = tin[i - 1];
+  for (int i = 1; i <= npm1; i++) c->p[i] = pin[i - 1];
+  c->p[npm1 + 1] = c->p[2] + twopi;
+  c->dt = V(ntm); c->dt_i = V(ntm); c->st = V(ntm); c->st_i = V(ntm); c->ct = V(ntm);
+  c->dp = V(npm); c->dp_i = V(npm);
+  c->th = V(nt); c->ph = V(np); c->dth = V(nt); c->dth_i = V(nt); c->sth = V(nt);
+  c->sth_i = V(nt); c->cth = V(nt); c->dph = V(np); c->dph_i = V(np);
+  auto &t = c->t, &p = c->p;
+  for (int i = 2; i <= ntm; i++) {
+    c->th[i] = half * (t[i] + t[i - 1]);
+    c->dth[i] = t[i] - t[i - 1];
+  }
+  c->th[1] = c->th[2] - c->dth[2];
+  c->th[nt] = c->th[ntm] + c->dth[ntm];
+  c->dth[1] = c->dth[2];
+  c->dth[nt] = c->dth[ntm];
+  for (int i = 1; i <= nt; i++) c->dth_i[i] = one / c->dth[i];
+  for (int i = 1; i <= ntm; i++) {
+    c->dt[i] = c->th[i + 1] - c->th[i];
+    c->dt_i[i] = one / c->dt[i];
+  }
+  for (int i = 1; i <= ntm; i++) c->st[i] = std::sin(t[i]);
+  c->st[1] = 0.0;
+  c->st[ntm] = 0.0;
+  for (int i = 1; i <= ntm; i++) c->ct[i] = std::cos(t[i]);
+  c->ct[1] = one;
+  c->ct[ntm] = one;
+  for (int i = 2; i <= ntm - 1; i++) c->st_i[i] = one / c->st[i];
+  c->st_i[1] = 0.0;
+  c->st_i[ntm] = 0.0;
+  for (int i = 1; i <= nt; i++) c->sth[i] = std::sin(std::fabs(c->th[i]));
+  for (int i = 2; i <= ntm; i++) c->sth_i[i] = one / c->sth[i];
+  c->sth_i[1] = 0.0;
+  c->sth_i[nt] = 0.0;
+  for (int i = 1; i <= nt; i++) c->cth[i] = std::cos(c->th[i]);
+  for (int i = 2; i <= np; i++) {
+    c->ph[i] = half * (p[i] + p[i - 1]);
+    c->dph[i] = p[i] - p[i - 1];
+  }
+  c->ph[1] = c->ph[npm1] - twopi;
+  c->dph[1] = c->dph[npm1];
+  for (int i = 1; i <= np; i++) c->dph_i[i] = one / c->dph[i];
+  for (int i = 1; i <= npm - 1; i++) c->dp[i] = c->ph[i + 1] - c->ph[i];
+  c->dp[npm] = c->dp[2];
+  for (int i = 1; i <= npm; i++) c->dp_i[i] = one / c->dp[i];
+  // load_weno: src/hipft.f90:6895-6932 (weno_eps = 0: cell widths)
+  for (int q = 0; q < 7; q++) {
+    c->wt[q] = V(nt);
+    c->wp[q] = V(np);
+  }
+  for (int j = 1; j <= ntm; j++) c->wt[6][j] = c->dt[j];
+  for (int k = 1; k <= npm; k++) c->wp[6][k] = c->dp[k];
+  auto weights = [](std::vector<double> *w, const std::vector<double> &d, int lo, int hi) {
+    for (int j = lo; j <= hi; j++) {
+      double tot = d[j - 1] + d[j] + d[j + 1];
+      w[0][j] = d[j] / (d[j] + d[j + 1]);          // D_C_CP
+      w[1][j] = d[j] / (d[j - 1] + d[j]);          // D_C_MC
+      w[2][j] = d[j - 1] / tot;                    // D_M_T
+      w[3][j] = (d[j] + d[j + 1]) / tot;           // D_CP_T
+      w[4][j] = d[j + 1] / tot;                    // D_P_T
+      w[5][j] = (d[j - 1] + d[j]) / tot;           // D_MC_T
+    }
+  };
+  weights(c->wt, c->dt, 2, nt - 2);
+  weights(c->wp, c->dp, 2, np - 1);
+  for (int q = 0; q < 6; q++) {
+    c->wp[q][1] = c->wp[q][np - 1];
+    c->wp[q][np] = c->wp[q][2];
+  }
+}
+
+int upload_vec(hft_ctx *c, const std::vector<double> &v, const double **out) {
+  double *d;
+  int rc = dalloc(&d, v.size());
+  if (rc) return rc;
+  c->dmesh.push_back(d);
+  // reference arrays are 1-based: drop element 0
+  HIPCHK(hipMemcpy(d, v.data() + 1, (v.size() - 1) * sizeof(double), hipMemcpyHostToDevice));
+  *out = d;
+  return 0;
+}
+
+template <typename T>
+int upload_raw(hft_ctx *c, const std::vector<T> &v, T **out) {
+  int rc = dalloc(out, v.size());
+  if (rc) return rc;
+  HIPCHK(hipMemcpy(*out, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  return 0;
+}
+
+int setup_device(hft_ctx *c) {
+  Mesh &m = c->m;
+  m.ntm = c->ntm; m.nt = c->nt; m.npm = c->npm; m.np = c->np; m.nu = c->nu; m.pitch = c->pitch;
+  int rc = 0;
+  rc |= upload_vec(c, c->dt_i, &m.dt_i);
+  rc |= upload_vec(c, c->st_i, &m.st_i);
+  rc |= upload_vec(c, c->sth, &m.sth);
+  rc |= upload_vec(c, c->dth_i, &m.dth_i);
+  rc |= upload_vec(c, c->dth, &m.dth);
+  rc |= upload_vec(c, c->dp_i, &m.dp_i);
+  rc |= upload_vec(c, c->dph_i, &m.dph_i);
+  rc |= upload_vec(c, c->dp, &m.dp);
+  const double **tw[7] = {&m.tCP, &m.tMC, &m.tMT, &m.tCPT, &m.tPT, &m.tMCT, &m.teps};
+  const double **pw[7] = {&m.pCP, &m.pMC, &m.pMT, &m.pCPT, &m.pPT, &m.pMCT, &m.peps};
+  for (int q = 0; q < 7; q++) {
+    rc |= upload_vec(c, c->wt[q], tw[q]);
+    rc |= upload_vec(c, c->wp[q], pw[q]);
+  }
+  if (rc) return rc;
+  m.bc_diff_n = pi_i * c->dth_i[2] * c->dth_i[2];  // src/hipft.f90:6841-6842
+  m.bc_diff_s = pi_i * c->dth_i[c->nt - 1] * c->dth_i[c->nt - 1];
+  m.bc_flow_n = two * c->dth_i[2] * pi_i;  // src/hipft.f90:2568-2569
+  m.bc_flow_s = two * c->dth_i[c->nt - 1] * pi_i;
+  size_t nf = c->nfield;
+  if ((rc = dalloc(&c->f, nf)) || (rc = dalloc(&c->buf0, nf)) || (rc = dalloc(&c->buf1, nf)) ||
+      (rc = dalloc(&c->dty0, nf)))
+    return rc;
+  HIPCHK(hipMemset(c->f, 0, nf * sizeof(double)));
+  HIPCHK(hipMemset(c->buf0, 0, nf * sizeof(double)));
+  HIPCHK(hipMemset(c->buf1, 0, nf * sizeof(double)));
+  HIPCHK(hipMemset(c->dty0, 0, nf * sizeof(double)));
+  int nr = c->nr;
+  std::vector<int> id(nr);
+  for (int i = 0; i < nr; i++) id[i] = i;
+  if ((rc = upload_raw(c, id, &c->d_order_id))) return rc;
+  if ((rc = dalloc(&c->d_order, nr)) || (rc = dalloc(&c->d_s, nr)) ||
+      (rc = dalloc(&c->d_cdt, nr)) || (rc = dalloc(&c->d_ub1, nr)) ||
+      (rc = dalloc(&c->d_absmax, nr)) || (rc = dalloc(&c->d_dtmin, nr)) ||
+      (rc = dalloc(&c->d_target, 1)) || (rc = dalloc(&c->d_D, nr)))
+    return rc;
+  HIPCHK(hipHostMalloc((void **)&c->h_small, sizeof(int) * (nr * 8 + 64)));
+  HIPCHK(hipHostMalloc((void **)&c->h_smalld, sizeof(double) * (nr * 8 + 64)));
+  HIPCHK(hipEventCreateWithFlags(&c->coef_ev, hipEventDisableTiming));
+  // analysis weights (analysis_step, src/hipft.f90:3235-3262)
+  int ntm = c->ntm, npm = c->npm;
+  std::vector<double> da_t(ntm), sn_t(ntm), cs_t(ntm), tt(ntm);
+  for (int j = 1; j <= ntm; j++) {
+    double sn, cs, d_t, dat;
+    if (j == 1) {
+      double tav = half * (c->t[1] + c->t[2]);
+      sn = std::sin(tav); cs = std::cos(tav); d_t = c->dth[2]; dat = quarter * sn * d_t;
+    } else if (j == ntm) {
+      double tav = half * (c->t[ntm] + c->t[ntm - 1]);
+      sn = std::sin(tav); cs = std::cos(tav); d_t = c->dth[ntm]; dat = quarter * sn * d_t;
+    } else {
+      sn = std::sin(c->t[j]); cs = std::cos(c->t[j]); d_t = c->dt[j]; dat = sn * d_t;
+    }
+    da_t[j - 1] = dat; sn_t[j - 1] = sn; cs_t[j - 1] = cs; tt[j - 1] = c->t[j];
+  }
+  std::vector<double> da_p(npm - 1), cs_p(npm - 1), sn_p(npm - 1);
+  std::vector<int> pcol(npm - 1);
+  for (int i = 1; i <= npm - 1; i++) {
+    double dap = (i == 1) ? half * c->dph[1] : (i == npm - 1 ? half * c->dph[npm - 1] : c->dp[i]);
+    da_p[i - 1] = dap;
+    cs_p[i - 1] = std::cos(c->p[i]);
+    sn_p[i - 1] = std::sin(c->p[i]);
+    pcol[i - 1] = (i == 1) ? c->nu - 1 : i - 2;
+  }
+  if ((rc = upload_raw(c, da_t, &c->d_da_t)) || (rc = upload_raw(c, sn_t, &c->d_sn_t)) ||
+      (rc = upload_raw(c, cs_t, &c->d_cs_t)) || (rc = upload_raw(c, tt, &c->d_tt)) ||
+      (rc = upload_raw(c, da_p, &c->d_da_p)) || (rc = upload_raw(c, cs_p, &c->d_cs_p)) ||
+      (rc = upload_raw(c, sn_p, &c->d_sn_p)) || (rc = upload_raw(c, pcol, &c->d_pcol)))
+    return rc;
+  c->ana_rows = std::max(1, (ntm + 63) / 64);
+  c->ana_nblk = (ntm + c->ana_rows - 1) / c->ana_rows;
+  if ((rc = dalloc(&c->d_part, (size_t)nr * c->ana_nblk * kNPart)) ||
+      (rc = dalloc(&c->d_hist, (size_t)nr * 14)) || (rc = dalloc(&c->d_apar, (size_t)nr * 4)) ||
+      (rc = dalloc(&c->d_ipar, (size_t)nr * 4)))
+    return rc;
+  return 0;
+}
+
+// ---- field layout conversions ---------------------------------------------
+// reference f(ntm,npm,nr) <-> device [r][j][c] (c <-> column c+2)
+void tp_to_dev(const hft_ctx *c, const double *tp, std::vector<double> &dev) {
+  dev.assign(c->nfield, 0.0);
+  for (int r = 0; r < c->nr; r++)
+    for (int k = 2; k <= c->npm - 1; k++)
+      for (int j = 0; j < c->ntm; j++)
+        dev[fidx(r, j, k - 2, c->ntm, c->pitch)] =
+            tp[j + (size_t)c->ntm * ((k - 1) + (size_t)c->npm * r)];
+}
+
+void dev_to_tp(const hft_ctx *c, const std::vector<double> &dev, double *tp) {
+  for (int r = 0; r < c->nr; r++)
+    for (int k = 1; k <= c->npm; k++) {
+      int cc = (k == 1) ? c->nu - 1 : (k == c->npm ? 0 : k - 2);
+      for (int j = 0; j < c->ntm; j++)
+        tp[j + (size_t)c->ntm * ((k - 1) + (size_t)c->npm * r)] =
+            dev[fidx(r, j, cc, c->ntm, c->pitch)];
+    }
+}
+
+// PT (npm1,ntm,nr) with one-point overlap -> device layout, no ingest
+void pt_to_dev(const hft_ctx *c, const double *pt, std::vector<double> &dev) {
+  dev.assign(c->nfield, 0.0);
+  for (int r = 0; r < c->nr; r++)
+    for (int j = 0; j < c->ntm; j++)
+      for (int l = 2; l <= c->npm1; l++)
+        dev[fidx(r, j, l - 2, c->ntm, c->pitch)] =
+            pt[(l - 1) + (size_t)c->npm1 * (j + (size_t)c->ntm * r)];
+}
+
+int put_field(hft_ctx *c, double *d, const std::vector<double> &h) {
+  HIPCHK(hipMemcpyAsync(d, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int get_field(hft_ctx *c, const double *d, std::vector<double> &h) {
+  h.resize(c->nfield);
+  HIPCHK(hipMemcpyAsync(h.data(), d, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+// ---- flows ------------------------------------------------------------------
+// update_flow (src/hipft.f90:3411-3448, 7243-7283) and set_lf_alpha
+// (6968-7041) for phi-independent flows.
+int rebuild_flow(hft_ctx *c) {
+  int nr = c->nr, nt = c->nt, ntm = c->ntm;
+  c->vt1.assign((size_t)nr * nt, 0.0);
+  c->vp1.assign((size_t)nr * ntm, 0.0);
+  c->at1.assign((size_t)nr * ntm, 0.0);
+  c->ap1.assign((size_t)nr * ntm, 0.0);
+  for (int r = 0; r < nr; r++) {
+    double *vt = &c->vt1[(size_t)r * nt];
+    double *vp = &c->vp1[(size_t)r * ntm];
+    if (!c->prof_vt.empty())
+      for (int j = 0; j < nt; j++) vt[j] = c->prof_vt[(size_t)r * nt + j];
+    if (!c->prof_vp.empty())
+      for (int j = 0; j < ntm; j++) vp[j] = c->prof_vp[(size_t)r * ntm + j];
+    if (!c->an_p1.empty())
+      for (int jr = 1; jr <= nt; jr++) {
+        double cc = c->cth[jr];
+        vt[jr - 1] = vt[jr - 1] - m_s_to_rs_hr * c->sth[jr] *
+                                      (c->an_p1[r] * cc + c->an_p3[r] * ipow(cc, 3) +
+                                       c->an_p5[r] * ipow(cc, 5));
+      }
+    if (!c->an_p0.empty())
+      for (int jr = 1; jr <= ntm; jr++) {
+        double cc = c->ct[jr];
+        vp[jr - 1] = vp[jr - 1] + m_s_to_rs_hr * c->st[jr] *
+                                      (c->an_p0[r] + c->an_p2[r] * ipow(cc, 2) +
+                                       c->an_p4[r] * ipow(cc, 4));
+      }
+    if (c->an_omega != 0.0)
+      for (int jr = 1; jr <= ntm; jr++)
+        vp[jr - 1] = vp[jr - 1] + c->an_omega * km_s_to_rs_hr * c->st[jr];
+    if (c->an_vtc != 0.0)
+      for (int jr = 1; jr <= nt; jr++) vt[jr - 1] = vt[jr - 1] + c->an_vtc * km_s_to_rs_hr;
+    // alphas
+    double *at = &c->at1[(size_t)r * ntm];
+    double *ap = &c->ap1[(size_t)r * ntm];
+    auto av = [&](int jf) { return std::fabs(vt[jf - 1]); };
+    for (int jr = 1; jr <= nt - 1; jr++) {
+      int lo, hi;
+      if (jr == 1) { lo = 1; hi = 4; }
+      else if (jr == 2) { lo = 1; hi = 5; }
+      else if (jr == nt - 2) { lo = jr - 2; hi = jr + 2; }
+      else if (jr == nt - 1) { lo = jr - 2; hi = jr + 1; }
+      else { lo = jr - 2; hi = jr + 3; }
+      double mm = av(lo);
+      for (int q = lo + 1; q <= hi; q++) mm = std::max(mm, av(q));
+      at[jr - 1] = mm;
+    }
+    for (int jr = 2; jr <= ntm - 1; jr++) ap[jr - 1] = std::fabs(vp[jr - 1]);
+  }
+  auto up = [&](std::vector<double> &h, double **d) -> int {
+    if (!*d) {
+      int rc = dalloc(d, h.size());
+      if (rc) return rc;
+    }
+    HIPCHK(hipMemcpy(*d, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+    return 0;
+  };
+  int rc;
+  if ((rc = up(c->vt1, &c->d_vt1)) || (rc = up(c->vp1, &c->d_vp1)) || (rc = up(c->at1, &c->d_at1)) ||
+      (rc = up(c->ap1, &c->d_ap1)))
+    return rc;
+  c->flow_set = true;
+  return 0;
+}
+
+int ensure_flow2d(hft_ctx *c) {
+  if (c->d_vt2) return 0;
+  c->vt_rstride = (size_t)c->nt * c->pitch;
+  int rc;
+  if ((rc = dalloc(&c->d_vt2, c->vt_rstride * c->nr)) || (rc = dalloc(&c->d_vp2, c->nfield)) ||
+      (rc = dalloc(&c->d_at2, c->nfield)) || (rc = dalloc(&c->d_ap2, c->nfield)))
+    return rc;
+  HIPCHK(hipMemset(c->d_vt2, 0, c->vt_rstride * c->nr * sizeof(double)));
+  HIPCHK(hipMemset(c->d_vp2, 0, c->nfield * sizeof(double)));
+  HIPCHK(hipMemset(c->d_at2, 0, c->nfield * sizeof(double)));
+  HIPCHK(hipMemset(c->d_ap2, 0, c->nfield * sizeof(double)));
+  return 0;
+}
+
+int update_flow_2d(hft_ctx *c) {
+  int rc = ensure_flow2d(c);
+  if (rc) return rc;
+  AttenArgs a;
+  a.f = c->f; a.vt1 = c->d_vt1; a.vp1 = c->d_vp1; a.vt = c->d_vt2; a.vp = c->d_vp2;
+  a.inv = c->d_atten_inv; a.ntm = c->ntm; a.nt = c->nt; a.nu = c->nu; a.pitch = c->pitch;
+  a.rstride = c->rstride; a.vt_rstride = c->vt_rstride;
+  dim3 g((c->nu + kBlock - 1) / kBlock, c->nt, c->nr);
+  hipLaunchKernelGGL(k_attenuate, g, dim3(kBlock), 0, c->stream, a);
+  HIPCHK(hipGetLastError());
+  dim3 g2((c->nu + kBlock - 1) / kBlock, c->ntm, c->nr);
+  hipLaunchKernelGGL(k_lf_alpha, g2, dim3(kBlock), 0, c->stream, (const double *)c->d_vt2,
+                     (const double *)c->d_vp2, c->d_at2, c->d_ap2, c->ntm, c->nt, c->nu, c->pitch,
+                     c->rstride, c->vt_rstride);
+  HIPCHK(hipGetLastError());
+  c->flow2d = true;
+  return 0;
+}
+
+// ---- RKG2 coefficients: load_sts_rkg2, src/hipft.f90:5399-5461 ----------------
+int64_t sts_stages(double dtl, double dte) {
+  double s_real = half * (std::sqrt(twentyfive + twentyfour * (dtl / dte)) - three);
+  double sc = std::ceil(s_real);
+  int64_t s = sc > 9.0e15 ? (int64_t)9.0e15 : (int64_t)sc;
+  if (s < 3) s = 3;
+  if (s % 2 == 0) s = s + 1;
+  return s;
+}
+
+// uj, vj, ubj, gj for j = 1..s (vectors of size s+1, element 0 unused)
+void sts_coefs(int64_t s, std::vector<double> &uj, std::vector<double> &vj,
+               std::vector<double> &ubj, std::vector<double> &gj) {
+  uj.assign(s + 1, 0.0); vj.assign(s + 1, 0.0); ubj.assign(s + 1, 0.0); gj.assign(s + 1, 0.0);
+  std::vector<double> b(s + 1, 0.0);
+  double w = six / (double)((s + 4) * (s - 1));
+  b[1] = one / three;
+  b[2] = one / fifteen;
+  uj[1] = one; vj[1] = -9999.0; ubj[1] = w; gj[1] = -9999.0;
+  uj[2] = one / two; vj[2] = -one / ten; ubj[2] = uj[2] * w; gj[2] = zero;
+  for (int64_t j = 3; j <= s; j++) {
+    double dj = (double)j;
+    b[j] = (four * (double)(j - 1) * (double)(j + 4)) /
+           (three * dj * (double)(j + 1) * (double)(j + 2) * (double)(j + 3));
+    double bj_bjm1 = b[j] / b[j - 1];
+    double bj_bjm2 = b[j] / b[j - 2];
+    uj[j] = bj_bjm1 * (two + one / dj);
+    vj[j] = -bj_bjm2 * (one / dj + one);
+    ubj[j] = uj[j] * w;
+    gj[j] = (half * dj * (dj + one) * b[j - 1] - one) * ubj[j];
+  }
+}
+
+struct CoefCache {
+  int64_t s = -1;
+  std::vector<double> u, v, ub, g;
+};
+
+// One RKG2 super-step for the listed realizations, each with its own cycle
+// time step dt2[r] and stage count s[r] (diffusion_step_sts_cd,
+// src/hipft.f90:5163-5242).  use_af: dty0 already holds L(f) from the PTL
+// pass (same operator, so identical to recomputing it).
+int rkg2_superstep(hft_ctx *c, const std::vector<int> &reals, const std::vector<double> &dt2,
+                   const std::vector<int64_t> &s, bool use_af) {
+  int n = (int)reals.size();
+  if (n == 0) return 0;
+  int nr = c->nr;
+  std::vector<int> ord(reals);
+  std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return s[a] > s[b]; });
+  int64_t smax = s[ord[0]];
+  size_t need = (size_t)(smax + 1) * nr;
+  if (need > c->coef_cap) {
+    if (c->d_coef) hipFree(c->d_coef);
+    size_t cap = std::max(need, (size_t)1024);
+    int rc = dalloc(&c->d_coef, cap);
+    if (rc) return rc;
+    c->coef_cap = cap;
+  }
+  // the previous upload from the pinned buffers must be complete
+  HIPCHK(hipEventSynchronize(c->coef_ev));
+  if (need > c->h_coef_cap) {
+    if (c->h_coef) hipHostFree(c->h_coef);
+    size_t cap = std::max(need, (size_t)1024);
+    HIPCHK(hipHostMalloc((void **)&c->h_coef, cap * sizeof(StageCoef)));
+    c->h_coef_cap = cap;
+  }
+  CoefCache cache;
+  for (int z = 0; z < n; z++) {
+    int r = ord[z];
+    if (cache.s != s[r]) {
+      cache.s = s[r];
+      sts_coefs(s[r], cache.u, cache.v, cache.ub, cache.g);
+    }
+    double dtl = dt2[r];
+    for (int64_t k = 2; k <= s[r]; k++) {
+      StageCoef &cf = c->h_coef[(size_t)k * nr + r];
+      cf.a = cache.u[k];
+      cf.b = cache.v[k];
+      cf.c = one - cache.u[k] - cache.v[k];
+      cf.d = cache.ub[k] * dtl;
+      cf.g = cache.g[k];
+    }
+    c->h_smalld[r] = dtl;           // cycle dt
+    c->h_smalld[nr + r] = cache.ub[1];
+  }
+  int64_t *hs = (int64_t *)(c->h_small + 2 * nr + 16);
+  for (int z = 0; z < n; z++) {
+    c->h_small[z] = ord[z];
+    hs[ord[z]] = s[ord[z]];
+  }
+  HIPCHK(hipMemcpyAsync(c->d_coef, c->h_coef, need * sizeof(StageCoef), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->d_order, c->h_small, n * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->d_s, hs, nr * sizeof(int64_t), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->d_cdt, c->h_smalld, nr * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->d_ub1, c->h_smalld + nr, nr * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipEventRecord(c->coef_ev, c->stream));
+
+  Stage1Args a1;
+  a1.m = c->m; a1.g = tiles_for(c, n); a1.f = c->f; a1.dty0 = c->dty0; a1.u1 = c->buf1;
+  a1.D = c->d_D; a1.dt = c->d_cdt; a1.ub1 = c->d_ub1; a1.order = c->d_order;
+  a1.Af = use_af ? c->dty0 : nullptr; a1.rstride = c->rstride;
+  int rc = launch(c, 0, k_rkg2_stage1, nblocks(a1.g), a1);
+  if (rc) return rc;
+  StageArgs a;
+  a.m = c->m; a.f = c->f; a.dty0 = c->dty0; a.D = c->d_D; a.s = c->d_s; a.order = c->d_order;
+  a.rstride = c->rstride;
+  int active = n;
+  for (int64_t k = 2; k <= smax; k++) {
+    while (active > 0 && s[ord[active - 1]] < k) active--;
+    a.g = tiles_for(c, active);
+    a.k = (int)k;
+    a.x = (k & 1) ? c->buf0 : c->buf1;                   // u_{k-1}
+    a.ukm2 = (k == 2) ? c->f : ((k & 1) ? c->buf1 : c->buf0);
+    a.out = (k & 1) ? c->buf1 : c->buf0;
+    a.coef = c->d_coef + (size_t)k * nr;
+    rc = launch(c, 0, k_rkg2_stage, nblocks(a.g), a);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+// get_dtime_diffusion_ptl (src/hipft.f90:5077-5156) for a set of
+// realizations.  Leaves L(f) in dty0.  res[r] = PTL time step.
+int ptl_pass(hft_ctx *c, const std::vector<int> &reals, bool coupled,
+             const std::vector<double> &dt_in, std::vector<double> &res) {
+  int n = (int)reals.size();
+  int nr = c->nr;
+  for (int z = 0; z < n; z++) c->h_small[z] = reals[z];
+  unsigned long long *hmax = (unsigned long long *)(c->h_smalld);
+  unsigned long long *hmin = hmax + nr;
+  unsigned long long hugebits;
+  double hv = kHuge;
+  memcpy(&hugebits, &hv, 8);
+  for (int r = 0; r < nr; r++) {
+    hmax[r] = 0ull;
+    hmin[r] = hugebits;
+  }
+  HIPCHK(hipEventSynchronize(c->coef_ev));
+  HIPCHK(hipMemcpyAsync(c->d_order, c->h_small, n * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->d_absmax, hmax, nr * 8, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->d_dtmin, hmin, nr * 8, hipMemcpyHostToDevice, c->stream));
+  LapArgs la;
+  la.m = c->m; la.g = tiles_for(c, n); la.x = c->f; la.y = c->dty0; la.D = c->d_D;
+  la.order = c->d_order; la.rstride = c->rstride; la.absmax = c->d_absmax;
+  int rc = launch(c, 2, k_laplacian, nblocks(la.g), la);
+  if (rc) return rc;
+  PtlArgs pa;
+  pa.m = c->m; pa.g = la.g; pa.f = c->f; pa.Af = c->dty0; pa.absmax = c->d_absmax;
+  pa.dtmin = c->d_dtmin; pa.order = c->d_order; pa.coupled = coupled ? 1 : 0; pa.nlaunch = n;
+  pa.rstride = c->rstride;
+  rc = launch(c, 2, k_ptl_min, nblocks(pa.g), pa);
+  if (rc) return rc;
+  std::vector<unsigned long long> mx(nr), mn(nr);
+  HIPCHK(hipMemcpyAsync(mx.data(), c->d_absmax, nr * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(mn.data(), c->d_dtmin, nr * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  auto asd = [](unsigned long long b) {
+    double d;
+    memcpy(&d, &b, 8);
+    return d;
+  };
+  const double safe = 0.95;
+  res.assign(nr, kHuge);
+  if (coupled) {
+    double ax = 0.0, dmin = kHuge;
+    for (int r : reals) ax = std::max(ax, asd(mx[r]));
+    for (int r : reals) dmin = std::min(dmin, asd(mn[r]));
+    double dtp = kHuge;
+    if (ax > zero) {
+      dtp = safe * dmin;
+      if (dtp < dt_in[reals[0]]) dtp = dt_in[reals[0]];
+    }
+    for (int r : reals) res[r] = dtp;
+  } else {
+    for (int r : reals) {
+      double dtp = kHuge;
+      if (asd(mx[r]) > zero) {
+        dtp = safe * asd(mn[r]);
+        if (dtp < dt_in[r]) dtp = dt_in[r];
+      }
+      res[r] = dtp;
+    }
+  }
+  return 0;
+}
+
+}  // namespace
+
+// ============================================================================
+// C ABI
+// ============================================================================
+extern "C" {
+
+const char *hft_last_error(void) { return g_err.c_str(); }
+
+int hft_create(int ntm, int npm1, const double *t, const double *p, int nr, int mode,
+               hft_ctx **out) {
+  if (!out || !t || !p) return fail(HFT_E_ARG, "hft_create: null argument");
+  *out = nullptr;
+  if (ntm < 5 || npm1 < 6 || nr < 1)
+    return fail(HFT_E_ARG, "hft_create: grid too small (ntm=%d npm1=%d nr=%d)", ntm, npm1, nr);
+  if (mode != HFT_MODE_COUPLED && mode != HFT_MODE_INDEPENDENT)
+    return fail(HFT_E_ARG, "hft_create: bad mode %d", mode);
+  // src/hipft.f90:6542-6564: the mesh must cover the sphere
+  const double eps = 1.e-5;
+  if (std::fabs(t[0]) > eps || std::fabs(t[ntm - 1] - pi) > eps || std::fabs(p[0]) > eps ||
+      std::fabs(p[npm1 - 1] - twopi) > eps)
+    return fail(HFT_E_ARG, "hft_create: mesh does not cover the sphere (t %g..%g, p %g..%g)",
+                t[0], t[ntm - 1], p[0], p[npm1 - 1]);
+  hft_ctx *c = new hft_ctx();
+  c->ntm = ntm; c->npm1 = npm1; c->npm = npm1 + 1; c->np = npm1 + 1; c->nt = ntm + 1;
+  c->nu = npm1 - 1; c->nr = nr; c->mode = mode;
+  c->pitch = ((c->nu + 63) / 64) * 64;
+  c->rstride = (size_t)ntm * c->pitch;
+  c->nfield = c->rstride * nr;
+  hipError_t e = hipGetDevice(&c->device);
+  if (e != hipSuccess) {
+    delete c;
+    return fail(HFT_E_HIP, "hipGetDevice: %s", hipGetErrorString(e));
+  }
+  e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete c;
+    return fail(HFT_E_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
+  }
+  build_mesh(c, t, p);
+  int rc = setup_device(c);
+  if (rc) {
+    std::string msg = g_err;
+    hft_destroy(c);
+    g_err = msg;
+    return rc;
+  }
+  c->last_s.assign(nr, 0);
+  c->last_cycles.assign(nr, 0);
+  *out = c;
+  return 0;
+}
+
+int hft_destroy(hft_ctx *c) {
+  if (!c) return 0;
+  if (c->stream) hipStreamSynchronize(c->stream);
+  for (double *d : c->dmesh) hipFree(d);
+  void *ptrs[] = {c->f, c->buf0, c->buf1, c->dty0, c->fval, c->d_vt1, c->d_vp1, c->d_at1,
+                  c->d_ap1, c->d_atten_inv, c->d_vt2, c->d_vp2, c->d_at2, c->d_ap2, c->d_D,
+                  c->d_coef, c->d_s, c->d_order, c->d_order_id, c->d_cdt, c->d_ub1,
+                  c->d_absmax, c->d_dtmin, c->d_target, c->d_part, c->d_hist, c->d_da_t,
+                  c->d_sn_t, c->d_cs_t, c->d_tt, c->d_da_p, c->d_cs_p, c->d_sn_p, c->d_pcol,
+                  c->d_map, c->d_apar, c->d_ipar};
+  for (void *q : ptrs)
+    if (q) hipFree(q);
+  if (c->h_coef) hipHostFree(c->h_coef);
+  if (c->h_small) hipHostFree(c->h_small);
+  if (c->h_smalld) hipHostFree(c->h_smalld);
+  for (auto &tp : c->pending) {
+    hipEventDestroy(tp.a);
+    hipEventDestroy(tp.b);
+  }
+  for (auto e : c->free_ev) hipEventDestroy(e);
+  if (c->coef_ev) hipEventDestroy(c->coef_ev);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+  return 0;
+}
+
+int hft_dims(const hft_ctx *c, int *ntm, int *npm, int *nr) {
+  if (!c) return fail(HFT_E_ARG, "null context");
+  if (ntm) *ntm = c->ntm;
+  if (npm) *npm = c->npm;
+  if (nr) *nr = c->nr;
+  return 0;
+}
+
+int hft_set_state(hft_ctx *c, const double *f, int layout, double mult_fac) {
+  if (!c || !f) return fail(HFT_E_ARG, "hft_set_state: null argument");
+  std::vector<double> dev;
+  if (layout == HFT_LAYOUT_TP) {
+    tp_to_dev(c, f, dev);
+  } else if (layout == HFT_LAYOUT_PT) {
+    // load_initial_condition ingest, src/hipft.f90:2116-2157, 2171-2173
+    int ntm = c->ntm, npm1 = c->npm1, npm = c->npm;
+    std::vector<double> tp((size_t)ntm * npm * c->nr);
+    auto F = [&](int j, int k, int r) -> double & {
+      return tp[(j - 1) + (size_t)ntm * ((k - 1) + (size_t)npm * r)];
+    };
+    for (int r = 0; r < c->nr; r++) {
+      for (int k = 1; k <= npm1; k++)
+        for (int j = 1; j <= ntm; j++) F(j, k, r) = f[(k - 1) + (size_t)npm1 * ((j - 1) + (size_t)ntm * r)];
+      for (int j = 1; j <= ntm; j++) {
+        F(j, 1, r) = half * (F(j, 1, r) + F(j, npm1, r));
+        F(j, npm1, r) = F(j, 1, r);
+        F(j, npm1 + 1, r) = F(j, 2, r);
+      }
+      double fn1 = 0.0, fs1 = 0.0;
+      for (int k = 2; k <= npm - 1; k++) {
+        fn1 = fn1 + F(1, k, r) * c->dp[k];
+        fs1 = fs1 + F(ntm, k, r) * c->dp[k];
+      }
+      for (int k = 1; k <= npm; k++) {
+        F(1, k, r) = fn1 * twopi_i;
+        F(ntm, k, r) = fs1 * twopi_i;
+      }
+      for (int k = 1; k <= npm; k++)
+        for (int j = 1; j <= ntm; j++) F(j, k, r) = mult_fac * F(j, k, r);
+    }
+    tp_to_dev(c, tp.data(), dev);
+  } else {
+    return fail(HFT_E_ARG, "hft_set_state: bad layout %d", layout);
+  }
+  return put_field(c, c->f, dev);
+}
+
+int hft_get_state(hft_ctx *c, double *f, int layout) {
+  if (!c || !f) return fail(HFT_E_ARG, "hft_get_state: null argument");
+  std::vector<double> dev;
+  int rc = get_field(c, c->f, dev);
+  if (rc) return rc;
+  if (layout == HFT_LAYOUT_TP) {
+    dev_to_tp(c, dev, f);
+  } else if (layout == HFT_LAYOUT_PT) {
+    for (int r = 0; r < c->nr; r++)
+      for (int j = 0; j < c->ntm; j++)
+        for (int l = 1; l <= c->npm1; l++) {
+          int cc = (l == 1) ? c->nu - 1 : l - 2;
+          f[(l - 1) + (size_t)c->npm1 * (j + (size_t)c->ntm * r)] = dev[fidx(r, j, cc, c->ntm, c->pitch)];
+        }
+  } else {
+    return fail(HFT_E_ARG, "hft_get_state: bad layout %d", layout);
+  }
+  return 0;
+}
+
+int hft_set_flow_analytic(hft_ctx *c, const double *p0, const double *p2, const double *p4,
+                          const double *p1, const double *p3, const double *p5,
+                          double vp_rigid_omega, double vt_const) {
+  if (!c) return fail(HFT_E_ARG, "null context");
+  int nr = c->nr;
+  auto cp = [nr](const double *s, std::vector<double> &d) {
+    if (s) d.assign(s, s + nr); else d.clear();
+  };
+  if ((p0 == nullptr) != (p2 == nullptr) || (p0 == nullptr) != (p4 == nullptr))
+    return fail(HFT_E_ARG, "hft_set_flow_analytic: p0,p2,p4 must be all set or all NULL");
+  if ((p1 == nullptr) != (p3 == nullptr) || (p1 == nullptr) != (p5 == nullptr))
+    return fail(HFT_E_ARG, "hft_set_flow_analytic: p1,p3,p5 must be all set or all NULL");
+  cp(p0, c->an_p0); cp(p2, c->an_p2); cp(p4, c->an_p4);
+  cp(p1, c->an_p1); cp(p3, c->an_p3); cp(p5, c->an_p5);
+  c->an_omega = vp_rigid_omega;
+  c->an_vtc = vt_const;
+  int rc = rebuild_flow(c);
+  if (rc) return rc;
+  if (c->atten) return update_flow_2d(c);
+  return 0;
+}
+
+int hft_set_flow_profiles(hft_ctx *c, const double *vt, const double *vp) {
+  if (!c) return fail(HFT_E_ARG, "null context");
+  if (vt) c->prof_vt.assign(vt, vt + (size_t)c->nr * c->nt); else c->prof_vt.clear();
+  if (vp) c->prof_vp.assign(vp, vp + (size_t)c->nr * c->ntm); else c->prof_vp.clear();
+  int rc = rebuild_flow(c);
+  if (rc) return rc;
+  if (c->atten) return update_flow_2d(c);
+  return 0;
+}
+
+int hft_set_flow_attenuation(hft_ctx *c, const double *value) {
+  if (!c) return fail(HFT_E_ARG, "null context");
+  if (!value) {
+    c->atten = false;
+    c->flow2d = false;
+    return 0;
+  }
+  c->atten_inv.resize(c->nr);
+  for (int r = 0; r < c->nr; r++) c->atten_inv[r] = value[r] > zero ? one / value[r] : kHuge;
+  if (!c->d_atten_inv) {
+    int rc = dalloc(&c->d_atten_inv, c->nr);
+    if (rc) return rc;
+  }
+  HIPCHK(hipMemcpy(c->d_atten_inv, c->atten_inv.data(), c->nr * sizeof(double), hipMemcpyHostToDevice));
+  c->atten = true;
+  if (!c->flow_set) {
+    int rc = rebuild_flow(c);
+    if (rc) return rc;
+  }
+  return update_flow_2d(c);
+}
+
+int hft_update_flow(hft_ctx *c) {
+  if (!c) return fail(HFT_E_ARG, "null context");
+  if (!c->atten) return 0;
+  return update_flow_2d(c);
+}
+
+int hft_flow_dtmax(hft_ctx *c, int strang, double *dt, double *dt_r) {
+  if (!c || !dt) return fail(HFT_E_ARG, "hft_flow_dtmax: null argument");
+  if (!c->flow_set) return fail(HFT_E_STATE, "hft_flow_dtmax: no flow set");
+  int nr = c->nr, ntm = c->ntm, npm = c->npm;
+  std::vector<double> per(nr, kHuge);
+  if (c->flow2d) {
+    unsigned long long hb;
+    double hv = kHuge;
+    memcpy(&hb, &hv, 8);
+    std::vector<unsigned long long> init(nr, hb);
+    HIPCHK(hipMemcpyAsync(c->d_dtmin, init.data(), nr * 8, hipMemcpyHostToDevice, c->stream));
+    dim3 g((c->nu + kBlock - 1) / kBlock, ntm - 2, nr);
+    hipLaunchKernelGGL(k_flow_dtmax, g, dim3(kBlock), 0, c->stream, (const double *)c->d_vt2,
+                       (const double *)c->d_vp2, c->m.dt_i, c->m.st_i, c->m.dp_i, ntm, c->nu,
+                       c->pitch, c->rstride, c->vt_rstride, c->d_dtmin);
+    HIPCHK(hipGetLastError());
+    std::vector<unsigned long long> out(nr);
+    HIPCHK(hipMemcpyAsync(out.data(), c->d_dtmin, nr * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int r = 0; r < nr; r++) memcpy(&per[r], &out[r], 8);
+  } else {
+    // phi-independent flows: kdotv grows with dp_i(k), so the minimum over k
+    // is taken at the largest dp_i (rounding is monotone).
+    double dpmax = 0.0;
+    for (int k = 2; k <= npm - 1; k++) dpmax = std::max(dpmax, c->dp_i[k]);
+    for (int r = 0; r < nr; r++) {
+      const double *vt = &c->vt1[(size_t)r * c->nt];
+      const double *vp = &c->vp1[(size_t)r * ntm];
+      double dtmax = kHuge;
+      for (int j = 2; j <= ntm - 1; j++) {
+        double a = std::max(std::fabs(vt[j - 1]), std::fabs(vt[j]));
+        double b = std::max(std::fabs(vp[j - 1]), std::fabs(vp[j - 1]));
+        double kdotv = a * c->dt_i[j] + b * c->st_i[j] * dpmax;
+        double deltat = half / std::max(kdotv, kTiny);
+        dtmax = std::min(dtmax, deltat);
+      }
+      per[r] = dtmax;
+    }
+  }
+  double mn = kHuge;
+  for (int r = 0; r < nr; r++) {
+    double v = safety * per[r];
+    if (strang) v = two * v;
+    v = two * v;  // SSPRK(4,3), src/hipft.f90:7340-7342
+    per[r] = v;
+    mn = std::min(mn, v);
+  }
+  *dt = mn;
+  if (dt_r) memcpy(dt_r, per.data(), nr * sizeof(double));
+  return 0;
+}
+
+int hft_set_diffusion(hft_ctx *c, const double *eta, double factor) {
+  if (!c || !eta) return fail(HFT_E_ARG, "hft_set_diffusion: null argument");
+  int nr = c->nr, ntm = c->ntm, npm = c->npm;
+  c->D.resize(nr);
+  c->dt_euler_r.resize(nr);
+  for (int r = 0; r < nr; r++) {
+    if (!(eta[r] >= 0.0)) return fail(HFT_E_ARG, "hft_set_diffusion: eta[%d] = %g", r, eta[r]);
+    c->D[r] = factor * eta[r];
+  }
+  // Gershgorin bound of the constant-coefficient matrix, evaluated with the
+  // coefficient expressions of load_diffusion_matrix (4960-4981) and
+  // get_dtime_diffusion_euler (5017-5033).
+  double max_all = 0.0;
+  for (int r = 0; r < nr; r++) {
+    double D = c->D[r], max_eig = 0.0;
+    for (int k = 2; k <= npm - 1; k++)
+      for (int j = 2; j <= ntm - 1; j++) {
+        double c1 = half * (D + D) * c->dph_i[k] * c->dp_i[k] * c->st_i[j] * c->st_i[j];
+        double c2 = half * (D + D) * c->dth_i[j] * c->dt_i[j] * c->st_i[j] * c->sth[j];
+        double c4 = half * (D + D) * c->dth_i[j + 1] * c->dt_i[j] * c->st_i[j] * c->sth[j + 1];
+        double c5 = half * (D + D) * c->dph_i[k + 1] * c->dp_i[k] * c->st_i[j] * c->st_i[j];
+        double c3 = -(c1 + c2 + c4 + c5);
+        double g = zero;
+        g = g + std::fabs(c1);
+        g = g + std::fabs(c2);
+        g = g + std::fabs(c3);
+        g = g + std::fabs(c4);
+        g = g + std::fabs(c5);
+        max_eig = std::max(g, max_eig);
+      }
+    c->dt_euler_r[r] = safety * (two / max_eig);
+    max_all = std::max(max_all, max_eig);
+  }
+  c->dt_euler = safety * (two / max_all);
+  HIPCHK(hipMemcpy(c->d_D, c->D.data(), nr * sizeof(double), hipMemcpyHostToDevice));
+  c->diff_set = true;
+  return 0;
+}
+
+int hft_dt_euler(hft_ctx *c, double *dt, double *dt_r) {
+  if (!c) return fail(HFT_E_ARG, "null context");
+  if (!c->diff_set) return fail(HFT_E_STATE, "hft_dt_euler: diffusion not set");
+  if (dt) *dt = c->dt_euler;
+  if (dt_r) memcpy(dt_r, c->dt_euler_r.data(), c->nr * sizeof(double));
+  return 0;
+}
+
+static int advection_stage(hft_ctx *c, const double *x, const double *y, double *out, double ca,
+                           double cb, double cdt, bool aop_only) {
+  AdvArgs a;
+  a.m = c->m;
+  a.g = tiles_for(c, c->nr);
+  a.f1.vt = c->d_vt1; a.f1.at = c->d_at1; a.f1.vp = c->d_vp1; a.f1.ap = c->d_ap1;
+  a.f2.vt = c->d_vt2; a.f2.vp = c->d_vp2; a.f2.at = c->d_at2; a.f2.ap = c->d_ap2;
+  a.f2.vt_rstride = c->vt_rstride;
+  a.x = x; a.y = y; a.out = out; a.ca = ca; a.cb = cb; a.cdt = cdt;
+  a.order = c->d_order_id; a.rstride = c->rstride; a.write_aop = aop_only ? 1 : 0;
+  int nb = nblocks(a.g);
+  if (c->flow2d) {
+    if (y) return launch(c, 1, k_advection_stage<true, true>, nb, a);
+    return launch(c, 1, k_advection_stage<true, false>, nb, a);
+  }
+  if (y) return launch(c, 1, k_advection_stage<false, true>, nb, a);
+  return launch(c, 1, k_advection_stage<false, false>, nb, a);
+}
+
+int hft_advection_step(hft_ctx *c, double dtl) {
+  if (!c) return fail(HFT_E_ARG, "null context");
+  if (!c->flow_set) return fail(HFT_E_STATE, "hft_advection_step: no flow set");
+  // advection_step_ssprk43, src/hipft.f90:7551-7574
+  int rc;
+  if ((rc = advection_stage(c, c->f, nullptr, c->buf0, 0, 0, half * dtl, false))) return rc;
+  if ((rc = advection_stage(c, c->buf0, nullptr, c->buf1, 0, 0, half * dtl, false))) return rc;
+  if ((rc = advection_stage(c, c->buf1, c->f, c->buf0, two_third, third, sixth * dtl, false))) return rc;
+  if ((rc = advection_stage(c, c->buf0, nullptr, c->f, 0, 0, half * dtl, false))) return rc;
+  return 0;
+}
+
+int hft_diffusion_step(hft_ctx *c, double dtl, int subcycles, hft_cycle_fn cb, void *user) {
+  if (!c) return fail(HFT_E_ARG, "null context");
+  if (!c->diff_set) return fail(HFT_E_STATE, "hft_diffusion_step: diffusion not set");
+  if (subcycles < 0) return fail(HFT_E_ARG, "hft_diffusion_step: subcycles < 0");
+  // diffusion_step, src/hipft.f90:4589-4690.  One controller for the whole
+  // batch (coupled) or one per realization (independent); all controllers
+  // that still cycle advance together in one batched RKG2 super-step.
+  const int subcycles_max = 60;
+  int nr = c->nr;
+  bool coupled = c->mode == HFT_MODE_COUPLED;
+  bool auto_sc = subcycles == 0;
+  int ng = coupled ? 1 : nr;
+  std::vector<int> cyc(ng, 1);
+  std::vector<char> done(ng, 0);
+  std::vector<double> stepped(ng, 0.0), dt2(ng, auto_sc ? zero : dtl / (double)subcycles);
+  std::vector<int64_t> sdiag(ng, subcycles);
+  std::vector<hft_cycle> recs(nr);
+  std::vector<double> dt2r(nr), dtin(nr), ptl;
+  std::vector<int64_t> sr(nr, 0);
+  for (int round = 1;; round++) {
+    std::vector<int> gs, reals;
+    for (int g = 0; g < ng; g++)
+      if (!done[g]) gs.push_back(g);
+    if (gs.empty()) break;
+    for (int g : gs) {
+      if (coupled)
+        for (int r = 0; r < nr; r++) reals.push_back(r);
+      else
+        reals.push_back(g);
+    }
+    if (auto_sc) {
+      for (int g : gs) {
+        if (coupled)
+          for (int r = 0; r < nr; r++) dtin[r] = dt2[0];
+        else
+          dtin[g] = dt2[g];
+      }
+      int rc = ptl_pass(c, reals, coupled, dtin, ptl);
+      if (rc) return rc;
+      for (int g : gs) dt2[g] = ptl[coupled ? 0 : g];
+    }
+    for (int g : gs) {
+      if (auto_sc && stepped[g] + dt2[g] >= dtl) {
+        dt2[g] = dtl - stepped[g];
+        done[g] = 1;
+      }
+      if (auto_sc && cyc[g] == subcycles_max - 1) {
+        dt2[g] = dtl - stepped[g];
+        done[g] = 1;
+      }
+      if (auto_sc) {
+        double q = std::ceil((dtl - stepped[g]) / dt2[g]);
+        sdiag[g] = q > 2e9 ? 2000000000LL : (int64_t)q;
+      }
+      double dte = coupled ? c->dt_euler : c->dt_euler_r[g];
+      int64_t s = sts_stages(dt2[g], dte);
+      if (coupled)
+        for (int r = 0; r < nr; r++) { sr[r] = s; dt2r[r] = dt2[g]; }
+      else {
+        sr[g] = s;
+        dt2r[g] = dt2[g];
+      }
+    }
+    int rc = rkg2_superstep(c, reals, dt2r, sr, auto_sc);
+    if (rc) return rc;
+    for (auto &rr : recs) rr = hft_cycle{0, 0, 0, 0};
+    for (int g : gs) {
+      stepped[g] = stepped[g] + dt2[g];
+      auto put = [&](int r) {
+        recs[r].time_stepped = stepped[g];
+        recs[r].dt = dt2[g];
+        recs[r].sts_s = sr[r];
+        recs[r].subcycles = sdiag[g];
+        c->last_s[r] = sr[r];
+        c->last_cycles[r] = cyc[g];
+      };
+      if (coupled)
+        for (int r = 0; r < nr; r++) put(r);
+      else
+        put(g);
+    }
+    if (cb) cb(user, round, recs.data());
+    for (int g : gs) {
+      if (!auto_sc && cyc[g] == subcycles) done[g] = 1;
+      if (!done[g]) cyc[g] += 1;
+    }
+  }
+  return 0;
+}
+
+int hft_diffusion_stats(hft_ctx *c, int64_t *s, int64_t *cycles) {
+  if (!c) return fail(HFT_E_ARG, "null context");
+  if (s) memcpy(s, c->last_s.data(), c->nr * sizeof(int64_t));
+  if (cycles) memcpy(cycles, c->last_cycles.data(), c->nr * sizeof(int64_t));
+  return 0;
+}
+
+int hft_flux_transport_step(hft_ctx *c, double dtl, int strang, int subcycles, hft_cycle_fn cb,
+                            void *user) {
+  if (!c) return fail(HFT_E_ARG, "null context");
+  int rc;
+  bool adv = c->flow_set, dif = c->diff_set;
+  if (strang) {
+    double h = dtl * half;
+    if (adv && (rc = hft_advection_step(c, h))) return rc;
+    if (dif && (rc = hft_diffusion_step(c, dtl, subcycles, cb, user))) return rc;
+    if (adv && (rc = hft_advection_step(c, h))) return rc;
+  } else {
+    if (adv && (rc = hft_advection_step(c, dtl))) return rc;
+    if (dif && (rc = hft_diffusion_step(c, dtl, subcycles, cb, user))) return rc;
+  }
+  return 0;
+}
+
+int hft_apply_diffusion_operator(hft_ctx *c, const double *x, double *y) {
+  if (!c || !x || !y) return fail(HFT_E_ARG, "null argument");
+  if (!c->diff_set) return fail(HFT_E_STATE, "diffusion not set");
+  std::vector<double> dev;
+  tp_to_dev(c, x, dev);
+  int rc = put_field(c, c->buf0, dev);
+  if (rc) return rc;
+  LapArgs la;
+  la.m = c->m; la.g = tiles_for(c, c->nr); la.x = c->buf0; la.y = c->buf1; la.D = c->d_D;
+  la.order = c->d_order_id; la.rstride = c->rstride; la.absmax = nullptr;
+  if ((rc = launch(c, 2, k_laplacian, nblocks(la.g), la))) return rc;
+  if ((rc = get_field(c, c->buf1, dev))) return rc;
+  dev_to_tp(c, dev, y);
+  return 0;
+}
+
+int hft_apply_advection_operator(hft_ctx *c, const double *x, double *y) {
+  if (!c || !x || !y) return fail(HFT_E_ARG, "null argument");
+  if (!c->flow_set) return fail(HFT_E_STATE, "no flow set");
+  std::vector<double> dev;
+  tp_to_dev(c, x, dev);
+  int rc = put_field(c, c->buf0, dev);
+  if (rc) return rc;
+  if ((rc = advection_stage(c, c->buf0, nullptr, c->buf1, 0, 0, 0, true))) return rc;
+  if ((rc = get_field(c, c->buf1, dev))) return rc;
+  dev_to_tp(c, dev, y);
+  return 0;
+}
+
+static int assimilate_impl(hft_ctx *c, const double *map_dev, int per_real, int custom,
+                           const double *mu_power, const double *mu_limit,
+                           const double *lat_limit, double mult, int add_mode, int balance) {
+  int nr = c->nr, ntm = c->ntm;
+  // lat-limit indices (load_data_assimilation, src/hipft.f90:4066-4079)
+  std::vector<double> ap(nr * 2);
+  std::vector<int> ip(nr * 3);
+  for (int r = 0; r < nr; r++) {
+    ap[r] = mu_power ? mu_power[r] : 4.0;
+    ap[nr + r] = mu_limit ? mu_limit[r] : 0.1;
+    double ll = lat_limit ? lat_limit[r] : 0.0;
+    int t0 = -1, t1 = -1;
+    for (int j = 1; j <= ntm; j++)
+      if (c->t[j] >= ll * d2r && t0 == -1) t0 = j;
+    for (int j = ntm; j >= 1; j--)
+      if (c->t[j] <= (pi - ll * d2r) && t1 == -1) t1 = j;
+    ip[r] = t0;
+    ip[nr + r] = t1;
+    ip[2 * nr + r] = ll > 0 ? 1 : 0;
+  }
+  HIPCHK(hipMemcpyAsync(c->d_apar, ap.data(), ap.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->d_ipar, ip.data(), ip.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  AssimArgs a;
+  a.m = c->m; a.g = tiles_for(c, nr); a.f = c->f; a.delta = balance ? c->buf0 : nullptr;
+  a.map = map_dev; a.map_rstride = per_real ? (size_t)3 * ntm * c->npm1 : 0; a.npm1 = c->npm1;
+  a.custom = custom; a.mu_power = c->d_apar; a.mu_limit = c->d_apar + nr;
+  a.tidx0 = c->d_ipar; a.tidx1 = c->d_ipar + nr; a.lat_on = c->d_ipar + 2 * nr;
+  a.mult = mult; a.add_mode = add_mode; a.order = c->d_order_id; a.rstride = c->rstride;
+  int rc = launch(c, 4, k_assimilate, nblocks(a.g), a);
+  if (rc) return rc;
+  if (balance) {
+    // balance_flux on the increment (src/hipft.f90:4129-4134, 8752-8801)
+    AnaArgs fa{};
+    fa.f = c->buf0; fa.ntm = ntm; fa.nu = c->nu; fa.pitch = c->pitch; fa.nr = nr;
+    fa.rstride = c->rstride; fa.da_t = c->d_da_t; fa.da_p = c->d_da_p; fa.pcol = c->d_pcol;
+    fa.rows_per_block = c->ana_rows; fa.part = c->d_part; fa.nblk = c->ana_nblk;
+    hipLaunchKernelGGL(k_flux_part, dim3(c->ana_nblk, nr), dim3(kBlock), 0, c->stream, fa);
+    HIPCHK(hipGetLastError());
+    std::vector<double> part((size_t)nr * c->ana_nblk * 2);
+    HIPCHK(hipMemcpyAsync(part.data(), c->d_part, part.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    std::vector<double> ratio(nr);
+    for (int r = 0; r < nr; r++) {
+      double fp = 0, fm = 0;
+      for (int b = 0; b < c->ana_nblk; b++) {
+        fp += part[((size_t)r * c->ana_nblk + b) * 2];
+        fm += part[((size_t)r * c->ana_nblk + b) * 2 + 1];
+      }
+      ratio[r] = (fm + fp == zero) ? 0.0 : std::sqrt(std::fabs(fp / fm));
+    }
+    HIPCHK(hipMemcpyAsync(c->d_apar, ratio.data(), nr * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_apply_balanced, dim3(256, nr), dim3(kBlock), 0, c->stream, c->f,
+                       (const double *)c->buf0, (const double *)c->d_apar, ntm, c->nu, c->pitch,
+                       c->rstride);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
+  return 0;
+}
+
+int hft_assimilate(hft_ctx *c, const double *map, int per_real, int custom, const double *mu_power,
+                   const double *mu_limit, const double *lat_limit, double mult, int add_mode,
+                   int balance) {
+  if (!c || !map) return fail(HFT_E_ARG, "hft_assimilate: null argument");
+  size_t n = (size_t)3 * c->ntm * c->npm1 * (per_real ? c->nr : 1);
+  if (n > c->map_cap) {
+    if (c->d_map) hipFree(c->d_map);
+    c->d_map = nullptr;
+    int rc = dalloc(&c->d_map, n);
+    if (rc) return rc;
+    c->map_cap = n;
+  }
+  HIPCHK(hipMemcpyAsync(c->d_map, map, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  int rc = assimilate_impl(c, c->d_map, per_real, custom, mu_power, mu_limit, lat_limit, mult,
+                           add_mode, balance);
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int hft_assimilate_device(hft_ctx *c, const double *map_dev, int per_real, int custom,
+                          const double *mu_power, const double *mu_limit, const double *lat_limit,
+                          double mult, int add_mode, int balance) {
+  if (!c || !map_dev) return fail(HFT_E_ARG, "hft_assimilate_device: null argument");
+  return assimilate_impl(c, map_dev, per_real, custom, mu_power, mu_limit, lat_limit, mult,
+                         add_mode, balance);
+}
+
+int hft_set_validation_field(hft_ctx *c, const double *fval_pt) {
+  if (!c) return fail(HFT_E_ARG, "null context");
+  if (!fval_pt) {
+    c->have_fval = false;
+    return 0;
+  }
+  if (!c->fval) {
+    int rc = dalloc(&c->fval, c->nfield);
+    if (rc) return rc;
+  }
+  std::vector<double> dev;
+  pt_to_dev(c, fval_pt, dev);
+  int rc = put_field(c, c->fval, dev);
+  if (rc) return rc;
+  c->have_fval = true;
+  return 0;
+}
+
+int hft_analysis(hft_ctx *c, const double *fval_pt, const double *fval_scale, double *out) {
+  if (!c || !out) return fail(HFT_E_ARG, "hft_analysis: null argument");
+  int nr = c->nr;
+  const double *fv = nullptr;
+  const double *scale = nullptr;
+  if (fval_pt) {
+    int rc = hft_set_validation_field(c, fval_pt);
+    if (rc) return rc;
+    fv = c->fval;
+  } else if (fval_scale) {
+    if (!c->have_fval) return fail(HFT_E_STATE, "hft_analysis: no validation field set");
+    HIPCHK(hipMemcpyAsync(c->d_apar, fval_scale, nr * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    fv = c->fval;
+    scale = c->d_apar;
+  }
+  AnaArgs a;
+  a.f = c->f; a.fval = fv; a.fval_scale = scale; a.ntm = c->ntm; a.nu = c->nu; a.pitch = c->pitch;
+  a.nr = nr; a.rstride = c->rstride; a.da_t = c->d_da_t; a.sn_t = c->d_sn_t; a.cs_t = c->d_cs_t;
+  a.tt = c->d_tt; a.da_p = c->d_da_p; a.cs_p = c->d_cs_p; a.sn_p = c->d_sn_p; a.pcol = c->d_pcol;
+  a.lat_n = 30.0 * d2r;
+  a.lat_s = pi - 30.0 * d2r;
+  a.rows_per_block = c->ana_rows; a.part = c->d_part; a.nblk = c->ana_nblk;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (c->timing) {
+    int rc = ev_get(c, &e0);
+    if (!rc) rc = ev_get(c, &e1);
+    if (rc) return rc;
+    HIPCHK(hipEventRecord(e0, c->stream));
+  }
+  hipLaunchKernelGGL(k_analysis_part, dim3(c->ana_nblk, nr), dim3(kBlock), 0, c->stream, a);
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(k_analysis_final, dim3((nr + 63) / 64), dim3(64), 0, c->stream,
+                     (const double *)c->d_part, c->ana_nblk, nr, c->ntm, c->npm, c->d_hist);
+  HIPCHK(hipGetLastError());
+  if (c->timing) {
+    HIPCHK(hipEventRecord(e1, c->stream));
+    c->pending.push_back({e0, e1, 3});
+  }
+  HIPCHK(hipMemcpyAsync(out, c->d_hist, (size_t)nr * 14 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int hft_state_device(hft_ctx *c, double **f_dev, int *pitch) {
+  if (!c) return fail(HFT_E_ARG, "null context");
+  if (f_dev) *f_dev = c->f;
+  if (pitch) *pitch = c->pitch;
+  return 0;
+}
+
+int hft_synchronize(hft_ctx *c) {
+  if (!c) return fail(HFT_E_ARG, "null context");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int hft_stream(hft_ctx *c, void **stream) {
+  if (!c || !stream) return fail(HFT_E_ARG, "null argument");
+  *stream = (void *)c->stream;
+  return 0;
+}
+
+int hft_set_timing(hft_ctx *c, int enable) {
+  if (!c) return fail(HFT_E_ARG, "null context");
+  int rc = drain_timing(c);
+  if (rc) return rc;
+  for (int g = 0; g < kNGroups; g++) {
+    c->t_ms[g] = 0;
+    c->t_n[g] = 0;
+  }
+  c->timing = enable != 0;
+  return 0;
+}
+
+int hft_kernel_time(hft_ctx *c, const char *group, double *total_ms, int64_t *launches) {
+  if (!c || !group) return fail(HFT_E_ARG, "null argument");
+  int rc = drain_timing(c);
+  if (rc) return rc;
+  for (int g = 0; g < kNGroups; g++)
+    if (strcmp(group, kGroups[g]) == 0) {
+      if (total_ms) *total_ms = c->t_ms[g];
+      if (launches) *launches = c->t_n[g];
+      return 0;
+    }
+  return fail(HFT_E_ARG, "unknown kernel group '%s'", group);
+}
+
+}  // extern "C"

@@ -1,0 +1,271 @@
+"""HipFT's time loop over libhft: a drop-in for src/hipft.f90:903-948.
+
+``run(cfg)`` takes the reference's namelist parameters (a dict with the
+``hipft.in`` names, see hipft_amd/namelist.py) and advances the batch on the
+GPU, producing the reference's history rows:
+
+* update_step (3338): update_flow (3371-3523) and update_timestep
+  (4308-4461: end-of-run, flow CFL, dt_max, output-time and assimilation-time
+  cuts),
+* flux_transport_step (4463-4507),
+* apply_data_assimilation (4148-4306),
+* analysis_step (3098) and output_histories (2607-2693), including the extra
+  rows written inside diffusion sub-cycles when verbose > 1 (4659-4669).
+
+History rows are [STEP, TIME, 14 diagnostics] (sol) and [STEP, TIME, DTIME,
+DTIME_ADV_STB, DTIME_ADV_USED, DTIME_DIFF_STB, N_DIFF_PER_STEP,
+N_DIFF_CYCLES] (num), per realization.
+"""
+import math
+
+import numpy as np
+
+from . import validation as V
+from .core import DIFF_KM2_S_TO_RS2_HR, FluxTransport
+
+HUGE = 1.7976931348623157e308
+
+DEFAULTS = dict(
+    verbose=0, res_nt=0, res_np=0, n_realizations=1, validation_run=0,
+    validation_run_width=0.03, time_start=0.0, time_end=1.0, dt_max=HUGE,
+    strang_splitting=False, advance_flow=False, flow_vp_rigid_omega=0.0, flow_vt_const=0.0,
+    flow_attenuate=False, flow_attenuate_value=500.0, flow_attenuate_values=None,
+    flow_dr_model=0, flow_mf_model=0,
+    flow_dr_coef_p0_value=46.0, flow_dr_coef_p2_value=-262.0, flow_dr_coef_p4_value=-379.0,
+    flow_dr_coef_p0_values=None, flow_dr_coef_p2_values=None, flow_dr_coef_p4_values=None,
+    flow_mf_coef_p1_value=22.0, flow_mf_coef_p3_value=11.0, flow_mf_coef_p5_value=-28.0,
+    flow_mf_coef_p1_values=None, flow_mf_coef_p3_values=None, flow_mf_coef_p5_values=None,
+    flow_profiles=None,
+    advance_diffusion=False, diffusion_coef_constant=0.0, diffusion_coef_constants=None,
+    diffusion_coef_factor=DIFF_KM2_S_TO_RS2_HR, diffusion_subcycles=0,
+    output_history_time_cadence=0.0, output_map_time_cadence=0.0, output_map_idx_start=1,
+    initial_map=None, initial_scales=None, initial_map_mult_fac=1.0,
+    assimilate_data=False, assimilation_maps=None, assimilation_times=None,
+    assimilate_data_custom_from_mu=False, assimilate_data_mu_power=4.0,
+    assimilate_data_mu_powers=None, assimilate_data_mu_limit=0.1,
+    assimilate_data_mu_limits=None, assimilate_data_lat_limit=0.0,
+    assimilate_data_lat_limits=None, assimilate_data_mult_fac=1.0, assimilate_data_add=False,
+    assimilate_data_balance_flux=False,
+)
+
+
+def per_realization(cfg, name, nr):
+    """Per-realization values: the `<name>s` list where given, else `<name>`
+    (process_input_parameters defaults, src/hipft.f90:8542-8579)."""
+    out = np.full(nr, float(cfg[name]))
+    vals = cfg.get(name + "s")
+    if vals is not None:
+        for i, v in enumerate(list(vals)[:nr]):
+            out[i] = v
+    return out
+
+
+class Run:
+    def __init__(self, cfg, mode="coupled"):
+        c = dict(DEFAULTS)
+        c.update(cfg)
+        self.c = c
+        nr = int(c["n_realizations"])
+        self.vfield = None
+        if c["initial_map"] is None:
+            s1, s2 = V.uniform_scales(c["res_nt"], c["res_np"])
+            f_local = np.zeros((len(s2), len(s1)))
+        else:
+            f_local = np.asarray(c["initial_map"], dtype=np.float64)
+            s1, s2 = (np.asarray(x, dtype=np.float64) for x in c["initial_scales"])
+        if c["validation_run"] == 1:
+            f_local = V.soccer_ball(s1, s2)
+            self.vfield = ("soccer", f_local.copy())
+        elif c["validation_run"] == 2:
+            dur = c["time_end"] - c["time_start"]
+            if abs(c["flow_vt_const"]) > 0:
+                c["flow_vt_const"] = (V.PI_TWO / dur) / V.KM_S_TO_RS_HR
+            if abs(c["flow_vp_rigid_omega"]) > 0:
+                c["flow_vp_rigid_omega"] = (V.TWOPI / dur) / V.KM_S_TO_RS_HR
+            f_local = V.gaussians(s1, s2, c["validation_run_width"])
+            self.vfield = ("gauss", None)
+        self.t, self.p = s2, s1
+        self.model = FluxTransport(s2, s1, nr, mode=mode)
+        self.model.set_state(f_local, layout="pt", mult_fac=c["initial_map_mult_fac"])
+        self.nr = nr
+        self.eta = per_realization(c, "diffusion_coef_constant", nr)
+        if self.vfield is not None and self.vfield[0] == "soccer":
+            self.model.set_validation_field(self.vfield[1])
+        n = len(s2)
+        with np.errstate(divide="ignore"):
+            self.st_i = np.where((np.arange(n) > 0) & (np.arange(n) < n - 1), 1.0 / np.sin(s2), 0.0)
+
+    def analysis(self, time):
+        c = self.c
+        if self.vfield is None:
+            return self.model.analysis()
+        if self.vfield[0] == "soccer":
+            fac = np.exp(-42.0 * self.eta * c["diffusion_coef_factor"] * (time - c["time_start"]))
+            return self.model.analysis(fval_scale=fac)
+        fv = V.gaussians_at(self.t, self.p, self.st_i, c["validation_run_width"],
+                            c["flow_vt_const"], c["flow_vp_rigid_omega"], time - c["time_start"])
+        return self.model.analysis(fval_pt=fv)
+
+
+def run(cfg, max_steps=None, mode="coupled"):
+    """Run a case; returns (sol[nr][rows][16], num[nr][rows][8], Run)."""
+    R = Run(cfg, mode)
+    c, m, nr = R.c, R.model, R.nr
+    st = dict(time=c["time_start"], ntime=0, dtime=0.0, dtmax_flow=0.0, dt_adv_used=0.0,
+              dt_euler=0.0, sts_s=np.zeros(nr, dtype=np.int64),
+              subcycles=np.full(nr, c["diffusion_subcycles"], dtype=np.int64), idx_hist=0,
+              idx_out=c["output_map_idx_start"] - 1, ts_needs=True, tsflow_needs=True,
+              flow_needs=True)
+    sol = [[] for _ in range(nr)]
+    num = [[] for _ in range(nr)]
+    if c["advance_diffusion"]:
+        m.set_diffusion(R.eta, c["diffusion_coef_factor"])
+        st["dt_euler"] = m.dt_euler()[0]
+    coef = {k: per_realization(c, k, nr) for k in (
+        "flow_dr_coef_p0_value", "flow_dr_coef_p2_value", "flow_dr_coef_p4_value",
+        "flow_mf_coef_p1_value", "flow_mf_coef_p3_value", "flow_mf_coef_p5_value",
+        "flow_attenuate_value")}
+    assim_times = list(c["assimilation_times"] or [])
+    assim_idx = 0
+    assimilate = bool(c["assimilate_data"]) and len(assim_times) > 0
+
+    def want(time):
+        cad = c["output_history_time_cadence"]
+        return (time >= c["time_start"] + st["idx_hist"] * cad) if cad > 0 else True
+
+    def emit(time, sts_s=None, subcycles=None):
+        if not want(time):
+            return
+        st["idx_hist"] += 1
+        h = R.analysis(time)
+        for r in range(nr):
+            sol[r].append([st["ntime"], time] + list(h[r]))
+            num[r].append([st["ntime"], time, st["dtime"], st["dtmax_flow"], st["dt_adv_used"],
+                           st["dt_euler"],
+                           st["sts_s"][r] if sts_s is None else sts_s[r],
+                           st["subcycles"][r] if subcycles is None else subcycles[r]])
+
+    def map_due(time):
+        cad = c["output_map_time_cadence"]
+        return cad > 0 and time >= c["time_start"] + (
+            st["idx_out"] - (c["output_map_idx_start"] - 1)) * cad
+
+    emit(st["time"])
+    if map_due(st["time"]):
+        st["idx_out"] += 1
+    steps = 0
+    while True:
+        st["ntime"] += 1
+        time = st["time"]
+        if c["advance_flow"] and st["flow_needs"]:
+            st["flow_needs"] = False
+            st["tsflow_needs"] = True
+            st["ts_needs"] = True
+            dr = c["flow_dr_model"] == 1
+            mf = c["flow_mf_model"] == 1
+            if c["flow_profiles"] is not None and steps == 0:
+                m.set_flow_profiles(*c["flow_profiles"])
+            m.set_flow_analytic(
+                p0=coef["flow_dr_coef_p0_value"] if dr else None,
+                p2=coef["flow_dr_coef_p2_value"] if dr else None,
+                p4=coef["flow_dr_coef_p4_value"] if dr else None,
+                p1=coef["flow_mf_coef_p1_value"] if mf else None,
+                p3=coef["flow_mf_coef_p3_value"] if mf else None,
+                p5=coef["flow_mf_coef_p5_value"] if mf else None,
+                vp_rigid_omega=c["flow_vp_rigid_omega"], vt_const=c["flow_vt_const"])
+            if c["flow_attenuate"]:
+                m.set_flow_attenuation(coef["flow_attenuate_value"])
+                st["flow_needs"] = True
+        if st["ts_needs"]:
+            st["dtime"] = c["time_end"] - time
+            if c["advance_flow"]:
+                st["dtmax_flow"] = HUGE
+                if st["tsflow_needs"]:
+                    st["dtmax_flow"] = m.flow_dtmax(c["strang_splitting"])[0]
+                    st["tsflow_needs"] = False
+                if st["dtmax_flow"] < st["dtime"]:
+                    st["dtime"] = st["dtmax_flow"]
+            if c["dt_max"] < st["dtime"]:
+                st["dtime"] = c["dt_max"]
+            st["ts_needs"] = False
+        cad = c["output_map_time_cadence"]
+        if cad > 0:
+            tnext = c["time_start"] + (st["idx_out"] - (c["output_map_idx_start"] - 1)) * cad
+            if time + st["dtime"] >= tnext:
+                st["dtime"] = tnext - time
+                st["ts_needs"] = True
+                st["tsflow_needs"] = True
+        if assimilate and time + st["dtime"] >= assim_times[assim_idx]:
+            st["dtime"] = assim_times[assim_idx] - time
+            st["ts_needs"] = True
+            st["tsflow_needs"] = True
+        if time + st["dtime"] >= c["time_end"]:
+            st["dtime"] = c["time_end"] - time
+            st["ts_needs"] = False
+        dt = st["dtime"]
+
+        def on_cycle(cycle, rows):
+            s = np.array([rw[2] for rw in rows], dtype=np.int64)
+            sc = np.array([rw[3] for rw in rows], dtype=np.int64)
+            active = np.array([rw[1] != 0.0 for rw in rows])
+            st["sts_s"] = np.where(active, s, st["sts_s"])
+            if c["verbose"] > 0:
+                st["subcycles"] = np.where(active, sc, st["subcycles"])
+            if c["verbose"] > 1:
+                emit(time + rows[0][0])
+
+        if c["advance_flow"]:
+            st["dt_adv_used"] = dt * 0.5 if c["strang_splitting"] else dt
+        m.flux_transport_step(dt, strang=c["strang_splitting"], subcycles=c["diffusion_subcycles"],
+                              callback=on_cycle if c["advance_diffusion"] else None)
+        if c["advance_diffusion"] and c["diffusion_subcycles"] == 0:
+            st["subcycles"] = m.diffusion_stats()[1]
+        st["time"] = time + dt
+        if assimilate and st["time"] >= assim_times[assim_idx]:
+            mp = c["assimilation_maps"](assim_idx) if callable(c["assimilation_maps"]) \
+                else c["assimilation_maps"][assim_idx]
+            m.assimilate(mp, custom_from_mu=c["assimilate_data_custom_from_mu"],
+                         mu_power=per_realization(c, "assimilate_data_mu_power", nr),
+                         mu_limit=per_realization(c, "assimilate_data_mu_limit", nr),
+                         lat_limit=per_realization(c, "assimilate_data_lat_limit", nr),
+                         mult_fac=c["assimilate_data_mult_fac"], add_mode=c["assimilate_data_add"],
+                         balance=c["assimilate_data_balance_flux"])
+            assim_idx += 1
+            if assim_idx >= len(assim_times):
+                assimilate = False
+        emit(st["time"])
+        if map_due(st["time"]):
+            st["idx_out"] += 1
+        steps += 1
+        if st["time"] >= c["time_end"] or (max_steps is not None and steps >= max_steps):
+            break
+    return np.array(sol), np.array(num), R
+
+
+def write_histories(sol, num, directory, first_index=1):
+    """Write hipft_history_{sol,num}_r######.out in the reference's layout
+    (headers src/hipft.f90:1616-1646, formats 2633-2634)."""
+    import os
+    sol_hdr = ["STEP", "TIME", "BR_MIN", "BR_MAX", "BR_ABS_MIN", "FLUX_POSITIVE",
+               "FLUX_NEGATIVE", "NPOLE_FLUX_POSITIVE", "NPOLE_FLUX_NEGATIVE",
+               "SPOLE_FLUX_POSITIVE", "SPOLE_FLUX_NEGATIVE", "NPOLE_AREA", "SPOLE_AREA",
+               "EQ_DIPOLE", "AX_DIPOLE", "VALIDATION_ERR_HHabs"]
+    num_hdr = ["STEP", "TIME", "DTIME", "DTIME_ADV_STB", "DTIME_ADV_USED", "DTIME_DIFF_STB",
+               "N_DIFF_PER_STEP", "N_DIFF_CYCLES"]
+
+    def e(x):  # Fortran 1pe23.15e3
+        mant, ex = f"{x:.15E}".split("E")
+        return f"{mant}E{int(ex):+04d}".rjust(23)
+
+    os.makedirs(directory, exist_ok=True)
+    for r in range(sol.shape[0]):
+        idx = first_index + r
+        with open(os.path.join(directory, f"hipft_history_sol_r{idx:06d}.out"), "w") as fh:
+            fh.write(f"{sol_hdr[0]:>10s} " + " ".join(f"{h:>22s}" for h in sol_hdr[1:]) + " \n")
+            for row in sol[r]:
+                fh.write(f"{int(row[0]):10d} " + "".join(e(x) + " " for x in row[1:]) + "\n")
+        with open(os.path.join(directory, f"hipft_history_num_r{idx:06d}.out"), "w") as fh:
+            fh.write(f"{num_hdr[0]:>10s} " + " ".join(f"{h:>22s}" for h in num_hdr[1:]) + " \n")
+            for row in num[r]:
+                fh.write(f"{int(row[0]):10d} " + "".join(e(x) + " " for x in row[1:6]) +
+                         f"{int(row[6]):15d} {int(row[7]):15d}\n")

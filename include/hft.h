@@ -1,0 +1,189 @@
+/*
+ * hft.h -- C ABI of the MI355X flux-transport core (libhft.so).
+ *
+ * The reference (HipFT, src/hipft.f90) has no plugin layer: its time loop
+ * (src/hipft.f90:903-948) calls Fortran subroutines that act on module state.
+ * This ABI replaces those call sites.  A Fortran driver binds it through
+ * iso_c_binding (include/hft.f90); Python binds it through ctypes
+ * (hipft_amd/core.py).
+ *
+ * Conventions
+ *  - Every function returns int: 0 on success, a negative HFT_E_* code on
+ *    error; hft_last_error() returns the message of the calling thread's last
+ *    error.  Nothing in the library calls exit() or abort().
+ *  - All arrays are double precision.  Host pointers are borrowed for the call
+ *    only; the library copies what it needs.
+ *  - Layout HFT_LAYOUT_TP is the reference's state array f(ntm,npm,nr)
+ *    (Fortran column-major, theta fastest, two-point phi overlap:
+ *    column 1 == column npm-1, column npm == column 2), src/hipft.f90:2114.
+ *    HFT_LAYOUT_PT is a map as stored in files: (npm1,ntm,nr), phi fastest,
+ *    one-point overlap (first and last phi columns are the same meridian).
+ *  - Realizations are the batch axis.  In HFT_MODE_COUPLED the time-step
+ *    controls (Euler limit, PTL sub-cycles, RKG2 stage count, flow CFL) are
+ *    shared by all realizations exactly as in the reference
+ *    (src/hipft.f90:5019-5037, 5094-5143, 7321-7346).  In
+ *    HFT_MODE_INDEPENDENT every realization runs its own controller and gets
+ *    the result a single-realization reference run would give it.
+ *  - One context drives one GPU (the current HIP device at hft_create) from
+ *    one host thread.  Calls are stream-ordered; functions that return host
+ *    data synchronise.
+ */
+#ifndef HFT_H
+#define HFT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HFT_OK 0
+#define HFT_E_ARG -1      /* invalid argument */
+#define HFT_E_HIP -2      /* HIP runtime error */
+#define HFT_E_STATE -3    /* call out of order (e.g. diffusion not loaded) */
+#define HFT_E_NOMEM -4    /* device allocation failed */
+
+#define HFT_LAYOUT_TP 0
+#define HFT_LAYOUT_PT 1
+
+#define HFT_MODE_COUPLED 0
+#define HFT_MODE_INDEPENDENT 1
+
+typedef struct hft_ctx hft_ctx;
+
+/* Number of history diagnostics per realization (analysis_step columns
+ * BR_MIN .. VALIDATION_ERR_HHabs, src/hipft.f90:1630-1646). */
+#define HFT_NHIST 14
+
+/* Per-cycle record of a diffusion step (src/hipft.f90:4607-4683). */
+typedef struct hft_cycle {
+  double time_stepped; /* part of the step covered after this cycle */
+  double dt;           /* cycle time step */
+  int64_t sts_s;       /* RKG2 stage count of this cycle */
+  int64_t subcycles;   /* diagnostic sub-cycle estimate printed in histories */
+} hft_cycle;
+
+/* Called after each round of diffusion sub-cycles with rec[nr]: the record
+ * of every realization that took a cycle in this round (rec[r].dt == 0 for a
+ * realization that had already finished its step).  In coupled mode all
+ * realizations cycle together.  Used to write the extra history rows the
+ * reference writes at verbose>1 (src/hipft.f90:4659-4669). */
+typedef void (*hft_cycle_fn)(void *user, int cycle, const hft_cycle *rec);
+
+/* ---- lifetime ----------------------------------------------------------
+ * Replaces load_initial_condition's mesh set-up and set_mesh
+ * (src/hipft.f90:2104-2136, 6495-6653).  t[ntm] are the theta nodes on
+ * [0,pi] and p[npm1] the phi nodes on [0,2pi] (file scales, one-point
+ * overlap).  mode: HFT_MODE_*. */
+int hft_create(int ntm, int npm1, const double *t, const double *p, int nr, int mode,
+               hft_ctx **out);
+int hft_destroy(hft_ctx *ctx);
+const char *hft_last_error(void);
+int hft_dims(const hft_ctx *ctx, int *ntm, int *npm, int *nr);
+
+/* ---- state ---------------------------------------------------------------
+ * HFT_LAYOUT_TP: f(ntm,npm,nr) as the reference holds it.
+ * HFT_LAYOUT_PT on set: the initial-map ingest of src/hipft.f90:2116-2157,
+ * 2171-2173 (transpose, seam average, pole rows set to their phi average,
+ * times mult_fac).  On get: PT map (npm1,ntm,nr) with the one-point overlap. */
+int hft_set_state(hft_ctx *ctx, const double *f, int layout, double mult_fac);
+int hft_get_state(hft_ctx *ctx, double *f, int layout);
+
+/* ---- flows ---------------------------------------------------------------
+ * update_flow, src/hipft.f90:3371-3523, for time-independent flows.
+ * p0,p2,p4: differential-rotation coefficients (m/s) per realization
+ * (add_flow_differential_rotation_analytic 7219-7251), NULL = none.
+ * p1,p3,p5: meridional-flow coefficients (m/s) per realization
+ * (add_flow_meridianal_analytic 7253-7285), NULL = none.
+ * vp_rigid_omega (km/s) and vt_const (km/s): src/hipft.f90:3436-3448.
+ * Also sets the Lax-Friedrichs alphas (set_lf_alpha 6940-7048). */
+int hft_set_flow_analytic(hft_ctx *ctx, const double *p0, const double *p2, const double *p4,
+                          const double *p1, const double *p3, const double *p5,
+                          double vp_rigid_omega, double vt_const);
+/* Base flow given as theta profiles (phi-independent), in code units
+ * (R_sun/hr): vt[nr][nt] on the theta half mesh, vp[nr][ntm] on the main
+ * theta mesh.  This is what flows from files reduce to when they do not
+ * depend on phi (add_flow_from_files, src/hipft.f90:3525-3673); as there,
+ * the analytic terms of hft_set_flow_analytic are added on top of it
+ * (src/hipft.f90:3421-3448).  NULL vt or vp = zero. */
+int hft_set_flow_profiles(hft_ctx *ctx, const double *vt, const double *vp);
+/* Flow attenuation by |Br| (src/hipft.f90:3489-3515): value[nr] in Gauss
+ * (<= 0 disables for that realization).  When enabled, the flow becomes a
+ * per-step 2-D field recomputed from the state at every hft_update_flow. */
+int hft_set_flow_attenuation(hft_ctx *ctx, const double *value);
+/* Recompute attenuated flows from the current state (update_flow for
+ * flow_attenuate=.true.); a no-op for static flows. */
+int hft_update_flow(hft_ctx *ctx);
+/* get_flow_dtmax (src/hipft.f90:7287-7350, SSPRK(4,3)): per-realization
+ * limits in dt_r[nr] (may be NULL); *dt = min over realizations. */
+int hft_flow_dtmax(hft_ctx *ctx, int strang, double *dt, double *dt_r);
+
+/* ---- diffusion -----------------------------------------------------------
+ * load_diffusion with constant per-realization coefficient eta[nr] (km^2/s),
+ * factor = diffusion_coef_factor (7.43162901307967e-09 converts km^2/s to
+ * R_sun^2/hr), src/hipft.f90:6692-6702, 6821-6843, 4927-4983.  Computes the
+ * explicit Euler limit (get_dtime_diffusion_euler 4985-5040). */
+int hft_set_diffusion(hft_ctx *ctx, const double *eta, double factor);
+/* Euler limit: *dt = coupled (min over realizations), dt_r[nr] per
+ * realization (may be NULL). */
+int hft_dt_euler(hft_ctx *ctx, double *dt, double *dt_r);
+
+/* ---- time step ------------------------------------------------------------
+ * flux_transport_step, src/hipft.f90:4463-4507: Strang A(dt/2) D(dt) A(dt/2)
+ * when strang != 0, else A(dt) D(dt).  subcycles = diffusion_subcycles
+ * (0 = automatic PTL sub-cycling).  cb/user may be NULL. */
+int hft_flux_transport_step(hft_ctx *ctx, double dt, int strang, int subcycles,
+                            hft_cycle_fn cb, void *user);
+/* advection_step (SSPRK(4,3) + WENO3, src/hipft.f90:4509-4549, 7515-7582). */
+int hft_advection_step(hft_ctx *ctx, double dt);
+/* diffusion_step (RKG2 with PTL sub-cycles, src/hipft.f90:4551-4695). */
+int hft_diffusion_step(hft_ctx *ctx, double dt, int subcycles, hft_cycle_fn cb, void *user);
+/* Last diffusion step summary: per realization stage count of the last
+ * cycle (s[nr]) and number of cycles (cycles[nr]); either may be NULL. */
+int hft_diffusion_stats(hft_ctx *ctx, int64_t *s, int64_t *cycles);
+
+/* ---- operators (for tests and adjoint checks) -----------------------------
+ * y = A x for the whole batch, TP layout, x and y host arrays. */
+int hft_apply_diffusion_operator(hft_ctx *ctx, const double *x, double *y);
+int hft_apply_advection_operator(hft_ctx *ctx, const double *x, double *y);
+
+/* ---- data assimilation -----------------------------------------------------
+ * apply_data_assimilation + assimilate_new_data, src/hipft.f90:4148-4306,
+ * 4083-4146.  map: (npm1,ntm,3) PT slices data/weight/mu, one map shared by
+ * all realizations (per_realization=0) or (npm1,ntm,3,nr).  Per-realization
+ * mu_power, mu_limit, lat_limit (degrees); custom_from_mu builds the weight
+ * slice from mu (4242-4252); add_mode is assimilate_data_add; balance is
+ * assimilate_data_balance_flux. */
+int hft_assimilate(hft_ctx *ctx, const double *map, int per_realization, int custom_from_mu,
+                   const double *mu_power, const double *mu_limit, const double *lat_limit,
+                   double mult_fac, int add_mode, int balance);
+/* Same, with the map already in device memory (PT slices, same shapes). */
+int hft_assimilate_device(hft_ctx *ctx, const double *map_dev, int per_realization,
+                          int custom_from_mu, const double *mu_power, const double *mu_limit,
+                          const double *lat_limit, double mult_fac, int add_mode, int balance);
+
+/* ---- analysis ---------------------------------------------------------------
+ * analysis_step, src/hipft.f90:3145-3325: out[nr][HFT_NHIST].
+ * Validation field: fval_pt (npm1,ntm,nr) host array, NULL = none (error 0);
+ * if fval_scale[nr] is non-NULL the field set by hft_set_validation_field is
+ * used, multiplied per realization by fval_scale[r]. */
+int hft_set_validation_field(hft_ctx *ctx, const double *fval_pt);
+int hft_analysis(hft_ctx *ctx, const double *fval_pt, const double *fval_scale, double *out);
+
+/* ---- device access (Python/PyTorch interop, benchmarks) -------------------
+ * Device pointer of the state and its layout: f_dev[(r*ntm + j)*pitch + c],
+ * c = 0..npm-3 the unique phi columns (c <-> reference column c+2). */
+int hft_state_device(hft_ctx *ctx, double **f_dev, int *pitch);
+int hft_synchronize(hft_ctx *ctx);
+/* HIP stream the context launches on (hipStream_t as void*). */
+int hft_stream(hft_ctx *ctx, void **stream);
+/* Average duration in ms of the kernels of the named group since the last
+ * reset ("rkg2", "weno", "ptl", "analysis", "assim"), measured with HIP
+ * events on the context's stream when timing is enabled. */
+int hft_set_timing(hft_ctx *ctx, int enable);
+int hft_kernel_time(hft_ctx *ctx, const char *group, double *total_ms, int64_t *launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

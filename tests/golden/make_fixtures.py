@@ -112,6 +112,8 @@ def main(ref_root):
             arrays["map_pt"] = read_h5_float32(path, "Data")
             arrays["scale_p"] = read_h5_float32(path, "dim1")
             arrays["scale_t"] = read_h5_float32(path, "dim2")
+        with open(os.path.join(indir, "hipft.in")) as fh:
+            arrays["hipft_in"] = np.array(fh.read())
         out = os.path.join(HERE, f"{case}.npz")
         np.savez_compressed(out, **arrays)
         print(case, {k: v.shape for k, v in arrays.items()}, os.path.getsize(out))

@@ -1,0 +1,267 @@
+"""GPU parity: libhft (HIP, gfx950) against the CPU oracle and the reference
+testsuite histories.
+
+Operators and single steps are compared bit for bit with the oracle (same
+inputs, same operation order, no FMA contraction, same polar-sum order).
+Whole runs are compared with the reference's committed histories.
+"""
+import numpy as np
+import pytest
+
+from conftest import golden
+from cases import case_cfg, history_error
+
+pytestmark = pytest.mark.gpu
+
+from hipft_amd import workloads as W  # noqa: E402
+from hipft_amd.core import FluxTransport  # noqa: E402
+from hipft_amd import driver  # noqa: E402
+from oracle.oracle import Oracle, run_case  # noqa: E402
+
+
+def grid(ntm, npm1, clustered=True):
+    t = W.clustered_theta(ntm, 2.0) if clustered else W.uniform_theta(ntm)
+    return t, W.uniform_phi(npm1)
+
+
+def random_state(o, seed, smooth=True):
+    rng = np.random.default_rng(seed)
+    t, p = o.mesh["t"], o.mesh["p"][: o.npm1]
+    if smooth:
+        m = W.bipole_map(t, p, 12, seed, fixed_sigma=None)
+    else:
+        m = rng.standard_normal((o.ntm, o.npm1))
+    maps = np.stack([m * (1 + 0.1 * r) for r in range(o.nr)])
+    o.ingest_pt(maps)
+    return maps
+
+
+def pair(ntm, npm1, nr, seed=1, clustered=True, mode="coupled", smooth=True):
+    t, p = grid(ntm, npm1, clustered)
+    o = Oracle(t, p, nr)
+    maps = random_state(o, seed, smooth)
+    g = FluxTransport(t, p, nr, mode=mode)
+    g.set_state(maps, layout="pt")
+    return o, g
+
+
+def assert_same(a, b, what, rtol=0.0):
+    if rtol == 0.0:
+        bad = np.argwhere(a != b)
+        assert bad.size == 0, f"{what}: {len(bad)} entries differ, first {bad[:3].tolist()}, " \
+                              f"max |diff| {np.abs(a - b).max():.3e}"
+    else:
+        scale = np.abs(b).max()
+        err = np.abs(a - b).max() / scale
+        assert err <= rtol, f"{what}: max rel error {err:.3e} > {rtol:.1e}"
+
+
+@pytest.mark.parametrize("shape", [(33, 65), (91, 181)])
+def test_ingest_and_layout(gpu_available, shape):
+    o, g = pair(*shape, nr=2)
+    assert_same(g.get_state("tp"), o.get_f(), "ingested state")
+
+
+@pytest.mark.parametrize("clustered", [True, False])
+def test_diffusion_operator_bitexact(gpu_available, clustered):
+    o, g = pair(47, 97, nr=3, clustered=clustered, smooth=False)
+    eta = [100.0, 500.0, 1000.0]
+    o.load_diffusion(eta)
+    g.set_diffusion(eta)
+    x = o.get_f()
+    assert_same(g.apply_diffusion_operator(x), o.diffusion_operator(x), "L x")
+    dt, dtr = g.dt_euler()
+    assert dt == o.dt_euler
+    assert all(dtr[r] == o.dt_euler_r(r) for r in range(3))
+
+
+def _flows(o, g, nr, with_profiles=True):
+    t = o.mesh["t"]
+    p0, p2, p4 = (np.array([c * (1.0 + 0.05 * r) for r in range(nr)]) for c in W.dr_coefficients())
+    vt_prof = np.stack([W.meridional_profile(t, 10.0 + 5.0 * r) for r in range(nr)])
+    o.flow_reset()
+    if with_profiles:
+        vt = np.ascontiguousarray(np.broadcast_to(vt_prof[:, None, :], (nr, o.npm, o.nt)))
+        o.flow_set(vt, np.zeros((nr, o.np_, o.ntm)))
+        g.set_flow_profiles(vt_prof, None)
+    o.flow_add_dr(p0, p2, p4)
+    o.set_lf_alpha()
+    g.set_flow_analytic(p0=p0, p2=p2, p4=p4)
+
+
+def test_advection_operator_bitexact(gpu_available):
+    o, g = pair(61, 121, nr=2, smooth=True)
+    _flows(o, g, 2)
+    x = o.get_f()
+    assert_same(g.apply_advection_operator(x), o.advection_operator(x), "A x")
+
+
+def test_advection_operator_mf_rigid_const(gpu_available):
+    o, g = pair(45, 91, nr=2, clustered=False)
+    p1, p3, p5 = [22.0, 17.8], [11.0, 9.9], [-28.0, -30.8]
+    o.flow_reset()
+    o.flow_add_mf(p1, p3, p5)
+    o.flow_add_rigid(0.3)
+    o.flow_add_vt_const(-0.05)
+    o.set_lf_alpha()
+    g.set_flow_analytic(p1=p1, p3=p3, p5=p5, vp_rigid_omega=0.3, vt_const=-0.05)
+    x = o.get_f()
+    assert_same(g.apply_advection_operator(x), o.advection_operator(x), "A x (MF+rigid+vt)")
+    for strang in (0, 1):
+        assert g.flow_dtmax(strang)[0] == o.flow_dtmax(strang)
+
+
+def test_advection_step_bitexact(gpu_available):
+    o, g = pair(61, 121, nr=2)
+    _flows(o, g, 2)
+    for _ in range(3):
+        o.advection_step(0.5)
+        g.advection_step(0.5)
+    assert_same(g.get_state("tp"), o.get_f(), "3 SSPRK(4,3) steps")
+
+
+def test_attenuated_flow_advection(gpu_available):
+    o, g = pair(45, 91, nr=2)
+    p0, p2, p4 = W.dr_coefficients()
+    att = [300.0, 150.0]
+    o.flow_reset()
+    o.flow_add_mf([22.0, 20.0], [11.0, 11.0], [-28.0, -28.0])
+    o.flow_add_dr([p0] * 2, [p2] * 2, [p4] * 2)
+    o.flow_attenuate(att)
+    o.set_lf_alpha()
+    g.set_flow_analytic(p0=p0, p2=p2, p4=p4, p1=[22.0, 20.0], p3=11.0, p5=-28.0)
+    g.set_flow_attenuation(att)
+    x = o.get_f()
+    # tanh of the device math library may differ from glibc in the last bit
+    assert_same(g.apply_advection_operator(x), o.advection_operator(x), "A x (attenuated)", 1e-12)
+    dt_o = o.flow_dtmax(0)
+    dt_g = g.flow_dtmax(0)[0]
+    assert abs(dt_g - dt_o) <= 1e-13 * dt_o
+
+
+def test_rkg2_fixed_cycles_bitexact(gpu_available):
+    o, g = pair(47, 97, nr=2)
+    eta = [300.0, 700.0]
+    o.load_diffusion(eta)
+    g.set_diffusion(eta)
+    res = o.diffusion_step(2.0, subcycles=2)
+    g.diffusion_step(2.0, subcycles=2)
+    s, cyc = g.diffusion_stats()
+    assert list(s) == [res["s"][-1]] * 2 and list(cyc) == [2, 2]
+    assert_same(g.get_state("tp"), o.get_f(), "RKG2 2 cycles")
+
+
+def test_ptl_diffusion_step_bitexact(gpu_available):
+    o, g = pair(91, 181, nr=3)
+    eta = [100.0, 400.0, 900.0]
+    o.load_diffusion(eta)
+    g.set_diffusion(eta)
+    recs = []
+    res = o.diffusion_step(24.0, subcycles=0)
+    g.diffusion_step(24.0, subcycles=0, callback=lambda cyc, rows: recs.append(rows))
+    assert len(recs) == res["ncycles"]
+    for k, rows in enumerate(recs):
+        assert rows[0][0] == res["time"][k]
+        assert rows[0][1] == res["dt"][k]
+        assert rows[0][2] == res["s"][k]
+    assert_same(g.get_state("tp"), o.get_f(), "RKG2 + PTL")
+
+
+def test_independent_mode_matches_single_runs(gpu_available):
+    t, p = grid(61, 121)
+    eta = [100.0, 350.0, 1000.0]
+    g = FluxTransport(t, p, 3, mode="independent")
+    singles = []
+    maps = []
+    for r in range(3):
+        o = Oracle(t, p, 1)
+        m = random_state(o, 10 + r)
+        maps.append(m[0])
+        o.load_diffusion([eta[r]])
+        singles.append(o)
+    g.set_state(np.stack(maps), layout="pt")
+    g.set_diffusion(eta)
+    out = []
+    for r, o in enumerate(singles):
+        out.append(o.diffusion_step(12.0, subcycles=0))
+    g.diffusion_step(12.0, subcycles=0)
+    s, cyc = g.diffusion_stats()
+    f = g.get_state("tp")
+    for r, o in enumerate(singles):
+        assert s[r] == out[r]["s"][-1] and cyc[r] == out[r]["ncycles"]
+        assert_same(f[r:r + 1], o.get_f(), f"independent realization {r}")
+    assert len(set(s.tolist())) > 1  # stage counts differ per realization
+
+
+def test_analysis_matches_oracle(gpu_available):
+    o, g = pair(91, 181, nr=2)
+    ho = o.analysis()
+    hg = g.analysis()
+    assert np.array_equal(hg[:, :3], ho[:, :3])
+    scale = np.maximum(np.abs(ho[:, 3:5]).max(), 1.0)
+    assert np.abs(hg[:, 3:11] - ho[:, 3:11]).max() <= 1e-13 * scale
+    assert np.allclose(hg[:, 11:], ho[:, 11:], rtol=1e-12, atol=1e-14 * np.abs(ho[:, 3]).max() / 4.84416e21)
+
+
+@pytest.mark.parametrize("add_mode,balance,lat", [(False, False, 0.0), (True, False, 0.0),
+                                                  (False, True, 0.0), (False, False, 60.0)])
+def test_assimilation_matches_oracle(gpu_available, add_mode, balance, lat):
+    o, g = pair(61, 121, nr=2)
+    t, p = o.mesh["t"], o.mesh["p"][: o.npm1]
+    rng = np.random.default_rng(5)
+    data = W.bipole_map(t, p, 8, 99) + rng.normal(0, 5.0, (o.ntm, o.npm1))
+    mu = W.mu_map(t, p, 120.0)
+    d3 = np.stack([data, np.zeros_like(data), mu])
+    d4 = np.ascontiguousarray(np.broadcast_to(d3, (2,) + d3.shape))
+    kw = dict(custom_from_mu=True, mu_power=[4.0, 2.0], mu_limit=[0.5, 0.3], lat_limit=lat,
+              mult_fac=1.1, add_mode=add_mode, balance=balance)
+    o.assimilate(d4, **kw)
+    g.assimilate(d3, **kw)
+    assert_same(g.get_state("tp"), o.get_f(), "assimilation", 1e-14)
+
+
+@pytest.mark.parametrize("case", ["advect_gaussians_phi", "advect_gaussians_theta",
+                                  "advect_gaussians_phi_theta", "diffuse_soccer",
+                                  "diffuse_advect_soccer"])
+def test_testsuite_case_vs_reference(gpu_available, case):
+    cfg = case_cfg(case)
+    sol, num, R = driver.run(cfg)
+    ref = golden(case)
+    assert sol.shape == ref["sol"].shape
+    err, verr, _ = history_error(sol, ref["sol"])
+    assert err <= 5e-12, f"{case}: history error {err:.3e}"
+    assert verr <= 1e-6
+    assert np.array_equal(num[..., 6:8], ref["num"][..., 6:8])
+    assert np.allclose(num[..., :6], ref["num"][..., :6], rtol=1e-14, atol=0)
+
+
+@pytest.mark.parametrize("case", ["diffuse_soccer", "advect_gaussians_phi_theta"])
+def test_testsuite_prefix_vs_oracle(gpu_available, case):
+    cfg = case_cfg(case)
+    sol_g, num_g, R = driver.run(cfg, max_steps=12)
+    sol_o, num_o, o = run_case(cfg, max_steps=12)
+    assert np.array_equal(num_g, num_o)
+    assert_same(R.model.get_state("tp"), o.get_f(), f"{case} state after 12 steps")
+    err, _, _ = history_error(sol_g, sol_o)
+    assert err <= 1e-13
+
+
+def test_diffuse_dipole_vs_reference(gpu_available):
+    """512x1024, one 8760 h step: 7 PTL cycles, 36,205 RKG2 stages."""
+    case = "diffuse_dipole"
+    sol, num, R = driver.run(case_cfg(case))
+    ref = golden(case)
+    assert np.array_equal(num[..., 6:8], ref["num"][..., 6:8]), (num[..., 6:8], ref["num"][..., 6:8])
+    assert np.allclose(num[..., 1], ref["num"][..., 1], rtol=1e-12)
+    err, _, cols = history_error(sol, ref["sol"])
+    assert err <= 1e-10, cols
+
+
+def test_atten_map_vs_reference(gpu_available):
+    case = "diffuse_advect_atten_map_1cr"
+    sol, num, R = driver.run(case_cfg(case))
+    ref = golden(case)
+    assert sol.shape == ref["sol"].shape
+    assert np.array_equal(num[..., 6:8], ref["num"][..., 6:8])
+    err, _, cols = history_error(sol, ref["sol"])
+    assert err <= 1e-10, cols
