@@ -1,0 +1,187 @@
+#!/usr/bin/env python3
+"""Benchmark: realization-steps/s of the flux-transport step at 512x1024 FP64.
+
+Workload (batched realizations): theta 512 nodes clustered x2 toward the poles
+(tanh blend), phi 1024 nodes (1023 unique), 64 realizations, FP64.  Each
+realization: initial map of 200 seeded bipoles (sigma log-uniform 1-4 deg,
+|B| 20-500 G, Joy tilt 5 deg sin(lat)); eta log-uniform 100-1000 km^2/s;
+meridional flow v0 sin(2 lat) cos(lat) with v0 uniform 10-20 m/s;
+differential rotation (0.18 - 2.396 cos^2 - 1.787 cos^4) deg/day times a
+scale uniform 0.9-1.1.  dt = 1 h, Strang splitting (SSPRK(4,3)+WENO3
+advection half steps around RKG2 diffusion with PTL sub-cycling), every
+realization with its own stage counts (independent mode), history analysis
+every step.  Inputs are resident in HBM before the timed region.
+
+A "step" is one time step of the whole batch: value = realizations x steps /
+wall time of the timed steps.
+
+Prints one JSON line.  Extra objects:
+  roofline     -- dominant kernel (RKG2 stage): algorithmic bytes / device time
+                  from HIP events on the library's stream over the timed steps
+  cpu_baseline -- the CPU oracle (1 core) on a bounded sample of the workload
+  single       -- the same step for one realization per launch (plain path)
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "realization-steps/s at 512x1024 FP64, %HBM roofline, adjoint/forward cost"
+HBM_PEAK_GBS = 8000.0
+
+
+def make_workload(nr, seed=2024, ntm=512, npm1=1024, nbip=200):
+    from hipft_amd import workloads as W
+    t = W.clustered_theta(ntm, 2.0)
+    p = W.uniform_phi(npm1)
+    rng = np.random.default_rng(seed)
+    eta = np.exp(rng.uniform(math.log(100.0), math.log(1000.0), nr))
+    v0 = rng.uniform(10.0, 20.0, nr)
+    oms = rng.uniform(0.9, 1.1, nr)
+    maps = np.stack([W.bipole_map(t, p, nbip, seed + 1 + r) for r in range(nr)])
+    vt = np.stack([W.meridional_profile(t, v) for v in v0])
+    dr = np.array([W.dr_coefficients(s) for s in oms])  # [nr][3]
+    return dict(t=t, p=p, eta=eta, v0=v0, oms=oms, maps=maps, vt=vt, dr=dr)
+
+
+def setup_model(wl, mode="independent"):
+    from hipft_amd.core import FluxTransport
+    nr = len(wl["eta"])
+    m = FluxTransport(wl["t"], wl["p"], nr, mode=mode)
+    m.set_state(wl["maps"], layout="pt")
+    m.set_flow_profiles(wl["vt"], None)
+    m.set_flow_analytic(p0=wl["dr"][:, 0], p2=wl["dr"][:, 1], p4=wl["dr"][:, 2])
+    m.set_diffusion(wl["eta"])
+    return m
+
+
+def step(m, dt=1.0):
+    m.flux_transport_step(dt, strang=True, subcycles=0)
+    return m.analysis()
+
+
+def time_steps(m, k):
+    m.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(k):
+        step(m)
+    m.synchronize()
+    return time.perf_counter() - t0
+
+
+def cpu_baseline(wl, budget_s=15.0, max_steps=20):
+    """The oracle (CPU restatement, 1 core) on realization 0 of the workload."""
+    from oracle.oracle import Oracle
+    o = Oracle(wl["t"], wl["p"], 1)
+    o.ingest_pt(wl["maps"][0])
+    nt = len(wl["t"]) + 1
+    vt = np.ascontiguousarray(np.broadcast_to(wl["vt"][0][None, None, :], (1, o.npm, nt)))
+    o.flow_reset()
+    o.flow_set(vt, np.zeros((1, o.np_, o.ntm)))
+    o.flow_add_dr(*[[c] for c in wl["dr"][0]])
+    o.set_lf_alpha()
+    o.load_diffusion([wl["eta"][0]])
+    n = 0
+    t0 = time.perf_counter()
+    while n < max_steps:
+        o.advection_step(0.5)
+        o.diffusion_step(1.0, 0)
+        o.advection_step(0.5)
+        o.analysis()
+        n += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    el = time.perf_counter() - t0
+    return dict(value=n / el, unit="realization-steps/s", cores=1, kind="port",
+                sample=f"{n} steps of realization 0 (eta={wl['eta'][0]:.1f} km^2/s) of the "
+                       f"512x1024 workload on one host core, CPU oracle (C restatement)")
+
+
+def load_traffic():
+    path = os.path.join(ROOT, "profiles", "rkg2_stage_pmc.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        d = json.load(fh)
+    return d
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=24)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--nr", type=int, default=64)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-single", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+                                   "(realizations would shard across GPUs with no per-step "
+                                   "communication in independent mode)"}))
+        sys.exit(2)
+    nr = args.nr
+    wl = make_workload(nr)
+    m = setup_model(wl)
+    for _ in range(args.warmup):
+        step(m)
+    # timed region: throughput with per-kernel HIP events on the library's stream
+    m.set_timing(True)
+    el = time_steps(m, args.steps)
+    ms_rkg2, n_rkg2, b_rkg2 = m.kernel_time("rkg2")
+    groups = {}
+    for g in ("rkg2", "weno", "ptl", "analysis"):
+        ms, n, b = m.kernel_time(g)
+        groups[g] = dict(ms=ms, launches=n, alg_GB=b / 1e9,
+                         GBs=(b / 1e9) / (ms / 1e3) if ms > 0 else None)
+    m.set_timing(False)
+    s_last, cyc = m.diffusion_stats()
+    value = nr * args.steps / el
+    avg_ms = ms_rkg2 / max(n_rkg2, 1)
+    achieved = (b_rkg2 / 1e9) / (ms_rkg2 / 1e3) if ms_rkg2 > 0 else 0.0
+    traffic = load_traffic()
+    roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(achieved / HBM_PEAK_GBS, 4),
+                traffic=(traffic or {}).get("hbm_bytes_per_launch"),
+                kernel="k_rkg2_stage (+stage 1)", avg_launch_ms=avg_ms,
+                alg_bytes_per_launch=b_rkg2 / max(n_rkg2, 1), launches=n_rkg2)
+    out = {
+        "metric": METRIC, "value": round(value, 3), "unit": "realization-steps/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(1e3 * el / args.steps, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"batched: 512x1024 theta-clustered x2, {nr} realizations, FP64, "
+                               "dt=1h Strang, WENO3/SSPRK(4,3) + RKG2/PTL, independent "
+                               "per-realization stage counts, 200 seeded bipoles each",
+                   "ntm": 512, "npm1": 1024, "realizations": nr,
+                   "stages_last_step_min_max": [int(s_last.min()), int(s_last.max())],
+                   "parallelism": "realization batch on 1 GPU"},
+        "roofline": roof,
+        "kernels": groups,
+    }
+    if not args.no_single:
+        wl1 = {k: (v[:1] if isinstance(v, np.ndarray) and v.ndim >= 1 and len(v) == nr else v)
+               for k, v in wl.items()}
+        m1 = setup_model(wl1)
+        step(m1)
+        k1 = max(4, args.steps // 2)
+        el1 = time_steps(m1, k1)
+        per_real = el1 / k1
+        out["single"] = dict(value=round(k1 / el1, 3), unit="realization-steps/s",
+                             ms_per_step=round(1e3 * per_real, 3),
+                             batched_speedup_per_realization=round(value * per_real, 2))
+        m1.close()
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(wl)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -79,7 +79,7 @@ _SIGS = [
     ("hft_synchronize", ctypes.c_int, [ctypes.c_void_p]),
     ("hft_stream", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]),
     ("hft_set_timing", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
-    ("hft_kernel_time", ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, _D, _I64]),
+    ("hft_kernel_time", ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, _D, _I64, _D]),
 ]
 
 _LIB = None
@@ -334,8 +334,10 @@ class FluxTransport:
         _check(lib().hft_set_timing(self._h, int(bool(enable))), "hft_set_timing")
 
     def kernel_time(self, group):
+        """(total device ms, launches, algorithmic bytes) of a kernel group."""
         ms = ctypes.c_double()
         n = ctypes.c_int64()
-        _check(lib().hft_kernel_time(self._h, group.encode(), ctypes.byref(ms), ctypes.byref(n)),
-               "hft_kernel_time")
-        return ms.value, n.value
+        b = ctypes.c_double()
+        _check(lib().hft_kernel_time(self._h, group.encode(), ctypes.byref(ms), ctypes.byref(n),
+                                     ctypes.byref(b)), "hft_kernel_time")
+        return ms.value, n.value, b.value

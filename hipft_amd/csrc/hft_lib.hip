@@ -77,6 +77,7 @@ int dalloc(T **p, size_t n) {
 struct TimedPair {
   hipEvent_t a, b;
   int group;
+  double units;  // algorithmic field passes x realizations of the launch
 };
 
 const char *kGroups[] = {"rkg2", "weno", "ptl", "analysis", "assim", "flow"};
@@ -145,6 +146,7 @@ struct hft_ctx {
   std::vector<hipEvent_t> free_ev;
   double t_ms[kNGroups] = {0};
   int64_t t_n[kNGroups] = {0};
+  double t_units[kNGroups] = {0};
 };
 
 namespace {
@@ -167,6 +169,7 @@ int drain_timing(hft_ctx *c) {
     HIPCHK(hipEventElapsedTime(&ms, tp.a, tp.b));
     c->t_ms[tp.group] += ms;
     c->t_n[tp.group] += 1;
+    c->t_units[tp.group] += tp.units;
     c->free_ev.push_back(tp.a);
     c->free_ev.push_back(tp.b);
   }
@@ -175,8 +178,10 @@ int drain_timing(hft_ctx *c) {
 }
 
 // Launch helper: records events around the launch when timing is enabled.
+// units: compulsory field passes (reads + writes of a realization's field)
+// the launch makes, times its realizations; used for algorithmic bytes.
 template <typename K, typename A>
-int launch(hft_ctx *c, int group, K kern, int blocks, const A &args) {
+int launch(hft_ctx *c, int group, K kern, int blocks, const A &args, double units) {
   if (blocks <= 0) return 0;
   hipEvent_t a = nullptr, b = nullptr;
   if (c->timing) {
@@ -193,7 +198,7 @@ int launch(hft_ctx *c, int group, K kern, int blocks, const A &args) {
   HIPCHK(hipGetLastError());
   if (c->timing) {
     HIPCHK(hipEventRecord(b, c->stream));
-    c->pending.push_back({a, b, group});
+    c->pending.push_back({a, b, group, units});
   }
   return 0;
 }
@@ -406,14 +411,20 @@ void dev_to_tp(const hft_ctx *c, const std::vector<double> &dev, double *tp) {
     }
 }
 
-// PT (npm1,ntm,nr) with one-point overlap -> device layout, no ingest
+// Validation field PT (npm1,ntm,nr) -> device layout.  analysis_step pairs
+// state columns 1..npm-2 with field columns 1..npm-2 (src/hipft.f90:3190-3197),
+// so the seam column (device c = nu-1, reference column 1 == npm-1) takes
+// the field's first column (phi = 0), not its last: an analytic field need
+// not be periodic.
 void pt_to_dev(const hft_ctx *c, const double *pt, std::vector<double> &dev) {
   dev.assign(c->nfield, 0.0);
   for (int r = 0; r < c->nr; r++)
     for (int j = 0; j < c->ntm; j++)
-      for (int l = 2; l <= c->npm1; l++)
-        dev[fidx(r, j, l - 2, c->ntm, c->pitch)] =
+      for (int cc = 0; cc < c->nu; cc++) {
+        int l = (cc == c->nu - 1) ? 1 : cc + 2;
+        dev[fidx(r, j, cc, c->ntm, c->pitch)] =
             pt[(l - 1) + (size_t)c->npm1 * (j + (size_t)c->ntm * r)];
+      }
 }
 
 int put_field(hft_ctx *c, double *d, const std::vector<double> &h) {
@@ -631,7 +642,7 @@ int rkg2_superstep(hft_ctx *c, const std::vector<int> &reals, const std::vector<
   a1.m = c->m; a1.g = tiles_for(c, n); a1.f = c->f; a1.dty0 = c->dty0; a1.u1 = c->buf1;
   a1.D = c->d_D; a1.dt = c->d_cdt; a1.ub1 = c->d_ub1; a1.order = c->d_order;
   a1.Af = use_af ? c->dty0 : nullptr; a1.rstride = c->rstride;
-  int rc = launch(c, 0, k_rkg2_stage1, nblocks(a1.g), a1);
+  int rc = launch(c, 0, k_rkg2_stage1, nblocks(a1.g), a1, 3.0 * n);
   if (rc) return rc;
   StageArgs a;
   a.m = c->m; a.f = c->f; a.dty0 = c->dty0; a.D = c->d_D; a.s = c->d_s; a.order = c->d_order;
@@ -645,7 +656,7 @@ int rkg2_superstep(hft_ctx *c, const std::vector<int> &reals, const std::vector<
     a.ukm2 = (k == 2) ? c->f : ((k & 1) ? c->buf1 : c->buf0);
     a.out = (k & 1) ? c->buf1 : c->buf0;
     a.coef = c->d_coef + (size_t)k * nr;
-    rc = launch(c, 0, k_rkg2_stage, nblocks(a.g), a);
+    rc = launch(c, 0, k_rkg2_stage, nblocks(a.g), a, 5.0 * active);
     if (rc) return rc;
   }
   return 0;
@@ -674,13 +685,13 @@ int ptl_pass(hft_ctx *c, const std::vector<int> &reals, bool coupled,
   LapArgs la;
   la.m = c->m; la.g = tiles_for(c, n); la.x = c->f; la.y = c->dty0; la.D = c->d_D;
   la.order = c->d_order; la.rstride = c->rstride; la.absmax = c->d_absmax;
-  int rc = launch(c, 2, k_laplacian, nblocks(la.g), la);
+  int rc = launch(c, 2, k_laplacian, nblocks(la.g), la, 2.0 * n);
   if (rc) return rc;
   PtlArgs pa;
   pa.m = c->m; pa.g = la.g; pa.f = c->f; pa.Af = c->dty0; pa.absmax = c->d_absmax;
   pa.dtmin = c->d_dtmin; pa.order = c->d_order; pa.coupled = coupled ? 1 : 0; pa.nlaunch = n;
   pa.rstride = c->rstride;
-  rc = launch(c, 2, k_ptl_min, nblocks(pa.g), pa);
+  rc = launch(c, 2, k_ptl_min, nblocks(pa.g), pa, 2.0 * n);
   if (rc) return rc;
   std::vector<unsigned long long> mx(nr), mn(nr);
   HIPCHK(hipMemcpyAsync(mx.data(), c->d_absmax, nr * 8, hipMemcpyDeviceToHost, c->stream));
@@ -1032,12 +1043,13 @@ static int advection_stage(hft_ctx *c, const double *x, const double *y, double 
   a.x = x; a.y = y; a.out = out; a.ca = ca; a.cb = cb; a.cdt = cdt;
   a.order = c->d_order_id; a.rstride = c->rstride; a.write_aop = aop_only ? 1 : 0;
   int nb = nblocks(a.g);
+  double u = (y ? 3.0 : 2.0) * c->nr;
   if (c->flow2d) {
-    if (y) return launch(c, 1, k_advection_stage<true, true>, nb, a);
-    return launch(c, 1, k_advection_stage<true, false>, nb, a);
+    if (y) return launch(c, 1, k_advection_stage<true, true>, nb, a, u);
+    return launch(c, 1, k_advection_stage<true, false>, nb, a, u);
   }
-  if (y) return launch(c, 1, k_advection_stage<false, true>, nb, a);
-  return launch(c, 1, k_advection_stage<false, false>, nb, a);
+  if (y) return launch(c, 1, k_advection_stage<false, true>, nb, a, u);
+  return launch(c, 1, k_advection_stage<false, false>, nb, a, u);
 }
 
 int hft_advection_step(hft_ctx *c, double dtl) {
@@ -1176,7 +1188,7 @@ int hft_apply_diffusion_operator(hft_ctx *c, const double *x, double *y) {
   LapArgs la;
   la.m = c->m; la.g = tiles_for(c, c->nr); la.x = c->buf0; la.y = c->buf1; la.D = c->d_D;
   la.order = c->d_order_id; la.rstride = c->rstride; la.absmax = nullptr;
-  if ((rc = launch(c, 2, k_laplacian, nblocks(la.g), la))) return rc;
+  if ((rc = launch(c, 2, k_laplacian, nblocks(la.g), la, 2.0 * c->nr))) return rc;
   if ((rc = get_field(c, c->buf1, dev))) return rc;
   dev_to_tp(c, dev, y);
   return 0;
@@ -1223,7 +1235,7 @@ static int assimilate_impl(hft_ctx *c, const double *map_dev, int per_real, int 
   a.custom = custom; a.mu_power = c->d_apar; a.mu_limit = c->d_apar + nr;
   a.tidx0 = c->d_ipar; a.tidx1 = c->d_ipar + nr; a.lat_on = c->d_ipar + 2 * nr;
   a.mult = mult; a.add_mode = add_mode; a.order = c->d_order_id; a.rstride = c->rstride;
-  int rc = launch(c, 4, k_assimilate, nblocks(a.g), a);
+  int rc = launch(c, 4, k_assimilate, nblocks(a.g), a, 4.0 * nr);
   if (rc) return rc;
   if (balance) {
     // balance_flux on the increment (src/hipft.f90:4129-4134, 8752-8801)
@@ -1337,7 +1349,7 @@ int hft_analysis(hft_ctx *c, const double *fval_pt, const double *fval_scale, do
   HIPCHK(hipGetLastError());
   if (c->timing) {
     HIPCHK(hipEventRecord(e1, c->stream));
-    c->pending.push_back({e0, e1, 3});
+    c->pending.push_back({e0, e1, 3, 1.0 * nr});
   }
   HIPCHK(hipMemcpyAsync(out, c->d_hist, (size_t)nr * 14 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -1370,12 +1382,14 @@ int hft_set_timing(hft_ctx *c, int enable) {
   for (int g = 0; g < kNGroups; g++) {
     c->t_ms[g] = 0;
     c->t_n[g] = 0;
+    c->t_units[g] = 0;
   }
   c->timing = enable != 0;
   return 0;
 }
 
-int hft_kernel_time(hft_ctx *c, const char *group, double *total_ms, int64_t *launches) {
+int hft_kernel_time(hft_ctx *c, const char *group, double *total_ms, int64_t *launches,
+                    double *alg_bytes) {
   if (!c || !group) return fail(HFT_E_ARG, "null argument");
   int rc = drain_timing(c);
   if (rc) return rc;
@@ -1383,6 +1397,7 @@ int hft_kernel_time(hft_ctx *c, const char *group, double *total_ms, int64_t *la
     if (strcmp(group, kGroups[g]) == 0) {
       if (total_ms) *total_ms = c->t_ms[g];
       if (launches) *launches = c->t_n[g];
+      if (alg_bytes) *alg_bytes = c->t_units[g] * 8.0 * (double)c->ntm * (double)c->nu;
       return 0;
     }
   return fail(HFT_E_ARG, "unknown kernel group '%s'", group);

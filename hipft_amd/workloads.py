@@ -58,9 +58,10 @@ def bipole_map(t, p, nbip, seed, sigma_deg=(1.0, 4.0), bmax=(20.0, 500.0), lat_m
                sep_deg=4.0, joy_deg=5.0, log_sigma=True, fixed_sigma=None, fixed_b=None):
     """PT map (ntm, npm1) [nt][np] in Gauss: sum of nbip seeded bipoles."""
     rng = np.random.default_rng(seed)
-    T, P = np.meshgrid(t, p, indexing="ij")
-    ct, st = np.cos(T), np.sin(T)
-    out = np.zeros_like(T)
+    t = np.asarray(t, dtype=np.float64)
+    p = np.asarray(p, dtype=np.float64)
+    out = np.zeros((len(t), len(p)))
+    ct_all, st_all = np.cos(t), np.sin(t)
     for _ in range(nbip):
         lat = rng.uniform(-lat_max, lat_max)
         lon = rng.uniform(0.0, 360.0)
@@ -78,10 +79,28 @@ def bipole_map(t, p, nbip, seed, sigma_deg=(1.0, 4.0), bmax=(20.0, 500.0), lat_m
         for sgn, dlon, dlat in ((hemi, half * math.cos(tilt), -hemi * half * math.sin(tilt)),
                                 (-hemi, -half * math.cos(tilt), hemi * half * math.sin(tilt))):
             th0 = math.radians(90.0 - (lat + dlat))
-            ph0 = math.radians(lon + dlon)
-            cosd = ct * math.cos(th0) + st * math.sin(th0) * np.cos(P - ph0)
+            ph0 = math.radians(lon + dlon) % TWOPI
+            # evaluate within 8 sigma of the centre (exp(-32) ~ 1e-14 relative)
+            rad = 8.0 * math.radians(sig)
+            jt = np.nonzero(np.abs(t - th0) <= rad)[0]
+            if rad >= th0 or th0 + rad >= PI:
+                kp = np.arange(len(p))  # the window contains a pole
+            else:
+                # generous phi half-width: twice the cap radius over the
+                # smallest sin(theta) inside the cap
+                smin = min(math.sin(th0 - rad), math.sin(th0 + rad))
+                dphi = 2.0 * rad / smin
+                if dphi >= PI:
+                    kp = np.arange(len(p))
+                else:
+                    dd = np.abs(((p - ph0) + PI) % TWOPI - PI)
+                    kp = np.nonzero(dd <= dphi)[0]
+            if len(jt) == 0 or len(kp) == 0:
+                continue
+            ct, st = ct_all[jt][:, None], st_all[jt][:, None]
+            cosd = ct * math.cos(th0) + st * math.sin(th0) * np.cos(p[kp][None, :] - ph0)
             d = np.arccos(np.clip(cosd, -1.0, 1.0))
-            out += sgn * b * np.exp(-0.5 * (d / math.radians(sig)) ** 2)
+            out[np.ix_(jt, kp)] += sgn * b * np.exp(-0.5 * (d / math.radians(sig)) ** 2)
     return out
 
 

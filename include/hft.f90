@@ -178,12 +178,12 @@ module hft
       type(c_ptr), value :: ctx
       integer(c_int), value :: enable
     end function
-    integer(c_int) function hft_kernel_time(ctx, group, total_ms, launches) &
+    integer(c_int) function hft_kernel_time(ctx, group, total_ms, launches, alg_bytes) &
         bind(c, name="hft_kernel_time")
       import :: c_int, c_double, c_ptr, c_char, c_int64_t
       type(c_ptr), value :: ctx
       character(kind=c_char), intent(in) :: group(*)
-      real(c_double), intent(out) :: total_ms
+      real(c_double), intent(out) :: total_ms, alg_bytes
       integer(c_int64_t), intent(out) :: launches
     end function
   end interface

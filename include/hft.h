@@ -177,11 +177,15 @@ int hft_state_device(hft_ctx *ctx, double **f_dev, int *pitch);
 int hft_synchronize(hft_ctx *ctx);
 /* HIP stream the context launches on (hipStream_t as void*). */
 int hft_stream(hft_ctx *ctx, void **stream);
-/* Average duration in ms of the kernels of the named group since the last
- * reset ("rkg2", "weno", "ptl", "analysis", "assim"), measured with HIP
- * events on the context's stream when timing is enabled. */
+/* Kernel timing with HIP events on the context's stream.  hft_set_timing
+ * resets the counters and turns per-launch events on or off.
+ * hft_kernel_time returns, for a kernel group ("rkg2", "weno", "ptl",
+ * "analysis", "assim") since the reset: total device time (ms), number of
+ * launches, and the algorithmic bytes those launches move (compulsory
+ * reads + writes of FP64 fields over the ntm x (npm-2) unique points). */
 int hft_set_timing(hft_ctx *ctx, int enable);
-int hft_kernel_time(hft_ctx *ctx, const char *group, double *total_ms, int64_t *launches);
+int hft_kernel_time(hft_ctx *ctx, const char *group, double *total_ms, int64_t *launches,
+                    double *alg_bytes);
 
 #ifdef __cplusplus
 }

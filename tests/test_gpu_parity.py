@@ -246,15 +246,28 @@ def test_testsuite_prefix_vs_oracle(gpu_available, case):
     assert err <= 1e-13
 
 
-def test_diffuse_dipole_vs_reference(gpu_available):
-    """512x1024, one 8760 h step: 7 PTL cycles, 36,205 RKG2 stages."""
+def test_diffuse_dipole_vs_oracle_and_reference(gpu_available):
+    """512x1024, one 8760 h step: 7 PTL cycles, 36,205 RKG2 stages.
+
+    Against the oracle (tests/golden/oracle_diffuse_dipole.npz): cycle times,
+    stage counts and cycle counts identical, histories to summation order.
+    Against the reference fixture: stage and cycle counts identical; from the
+    4th PTL cycle on the cycle length differs by ~1.6e-8 relative, because
+    the PTL step is taken at the points where |L f| equals its maximum
+    exactly (src/hipft.f90:5116) and that tie-break follows the last bit of
+    the reference build's rounding.
+    """
     case = "diffuse_dipole"
     sol, num, R = driver.run(case_cfg(case))
+    orc = golden("oracle_diffuse_dipole")
+    assert np.array_equal(num, orc["num"])
+    err, _, cols = history_error(sol, orc["sol"])
+    assert err <= 1e-13, cols
     ref = golden(case)
-    assert np.array_equal(num[..., 6:8], ref["num"][..., 6:8]), (num[..., 6:8], ref["num"][..., 6:8])
-    assert np.allclose(num[..., 1], ref["num"][..., 1], rtol=1e-12)
+    assert np.array_equal(num[..., 6:8], ref["num"][..., 6:8])
+    assert np.allclose(num[..., 1], ref["num"][..., 1], rtol=1e-7)
     err, _, cols = history_error(sol, ref["sol"])
-    assert err <= 1e-10, cols
+    assert err <= 1e-7, cols
 
 
 def test_atten_map_vs_reference(gpu_available):
