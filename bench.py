@@ -121,6 +121,7 @@ def main():
     ap.add_argument("--nr", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-single", action="store_true")
+    ap.add_argument("--retune", action="store_true", help="ignore the cached launch tuning")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
@@ -130,6 +131,9 @@ def main():
     nr = args.nr
     wl = make_workload(nr)
     m = setup_model(wl)
+    from hipft_amd import tune
+    tuning = tune.autotune(m, use_cache=not args.retune,
+                           log=(lambda msg: print(msg, file=sys.stderr, flush=True)))
     for _ in range(args.warmup):
         step(m)
     # timed region: throughput with per-kernel HIP events on the library's stream
@@ -162,7 +166,8 @@ def main():
                                "per-realization stage counts, 200 seeded bipoles each",
                    "ntm": 512, "npm1": 1024, "realizations": nr,
                    "stages_last_step_min_max": [int(s_last.min()), int(s_last.max())],
-                   "parallelism": "realization batch on 1 GPU"},
+                   "parallelism": "realization batch on 1 GPU",
+                   "launch_tuning": {k: tuning[k] for k in tune.KEYS}},
         "roofline": roof,
         "kernels": groups,
     }
@@ -170,6 +175,7 @@ def main():
         wl1 = {k: (v[:1] if isinstance(v, np.ndarray) and v.ndim >= 1 and len(v) == nr else v)
                for k, v in wl.items()}
         m1 = setup_model(wl1)
+        tune.autotune(m1, use_cache=not args.retune)
         step(m1)
         k1 = max(4, args.steps // 2)
         el1 = time_steps(m1, k1)

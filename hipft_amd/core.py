@@ -78,6 +78,9 @@ _SIGS = [
                                         ctypes.POINTER(ctypes.c_int)]),
     ("hft_synchronize", ctypes.c_int, [ctypes.c_void_p]),
     ("hft_stream", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]),
+    ("hft_set_tuning", ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int]),
+    ("hft_get_tuning", ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p,
+                                      ctypes.POINTER(ctypes.c_int)]),
     ("hft_set_timing", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("hft_kernel_time", ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, _D, _I64, _D]),
 ]
@@ -329,6 +332,15 @@ class FluxTransport:
 
     def synchronize(self):
         _check(lib().hft_synchronize(self._h), "hft_synchronize")
+
+    def set_tuning(self, **kw):
+        for k, v in kw.items():
+            _check(lib().hft_set_tuning(self._h, k.encode(), int(v)), "hft_set_tuning")
+
+    def get_tuning(self, key):
+        v = ctypes.c_int()
+        _check(lib().hft_get_tuning(self._h, key.encode(), ctypes.byref(v)), "hft_get_tuning")
+        return v.value
 
     def set_timing(self, enable):
         _check(lib().hft_set_timing(self._h, int(bool(enable))), "hft_set_timing")

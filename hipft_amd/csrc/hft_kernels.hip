@@ -202,6 +202,189 @@ __global__ __launch_bounds__(kBlock) void k_rkg2_stage(StageArgs a) {
   o[p] = v;
 }
 
+// ---------------------------------------------------------------------------
+// Fused RKG2 stages: T consecutive stages k0..k0+T-1 in one launch.
+//
+// A block owns one realization and a band of H theta rows over the full phi
+// width (one thread per phi column, nu <= 1024).  It marches down the band
+// (plus a T-row halo above and below): at march step i it loads row q of
+// u_{k0-1} (level 0) and every stage t = 1..T computes row q - 2t of its
+// level from level t-1.  Level t-1's rows R-1, R, R+1 at the thread's own
+// column are in registers; row R at the neighbouring columns c-1, c+1 (and
+// the full rows the polar closure sums over) is read from a 4-row LDS ring
+// per level, written two march steps earlier and separated from the reads
+// by one barrier per step.  The lag of two rows per level puts both rows the
+// north-pole sum needs (0 and 1) in LDS before it is formed.
+//
+// Only u_{k0-1}, u_{k0-2}, u0 and dty0 are read and the last two stages
+// written, so a launch moves about 4 + 2 fields (plus halo) for T stages
+// instead of 5T.  Expressions and the polar-sum order are those of
+// k_rkg2_stage, so the results are bit-identical.
+// ---------------------------------------------------------------------------
+struct FusedArgs {
+  Mesh m;
+  int nbands, H, k0, ncol;
+  const double *A;     // u_{k0-1}
+  const double *B;     // u_{k0-2}
+  const double *f;     // u0 (not written: halo rows are shared)
+  const double *dty0;
+  double *outA;        // u_{k0+T-1}, or u_s of a realization that finishes
+  double *outB;        // u_{k0+T-2}
+  const double *D;
+  const StageCoef *coef;  // table [k][nr]
+  int nr_table;
+  const int64_t *s;
+  const int *order;
+  size_t rstride;
+};
+
+template <int T>
+__global__ __launch_bounds__(1024) void k_rkg2_fused(FusedArgs a) {
+  extern __shared__ double lds[];  // [T][4][ncol]
+  const Mesh &m = a.m;
+  const int ncol = a.ncol, nu = m.nu, ntm = m.ntm, pitch = m.pitch;
+  int L = xcd_remap(blockIdx.x, gridDim.x);
+  int z = L / a.nbands, band = L - z * a.nbands;
+  int r = a.order[z];
+  int c = threadIdx.x;
+  bool col = c < nu;
+  int cc = col ? c : 0;
+  int cw = wrapc(cc - 1, nu), ce = wrapc(cc + 1, nu);
+  int j0 = band * a.H, j1 = min(ntm, j0 + a.H);
+  int lo0 = max(0, j0 - T);
+  int64_t sr = a.s[r];
+  int nst = (int)((sr - a.k0 + 1) < T ? (sr - a.k0 + 1) : T);
+  bool final_here = (a.k0 + nst - 1) == sr;
+  size_t base = (size_t)r * a.rstride;
+  const double *A = a.A + base, *B = a.B + base, *U0 = a.f + base, *DT = a.dty0 + base;
+  double *OA = a.outA + base, *OB = a.outB + base;
+  double D = a.D[r];
+  double D2 = D + D;
+  double P1 = (D * m.dph_i[cc + 1]) * m.dp_i[cc + 1];
+  double P5 = (D * m.dph_i[cc + 2]) * m.dp_i[cc + 1];
+  // stage coefficients are block-uniform: keep them in LDS, not registers
+  __shared__ double cfs[4][5];
+  if (threadIdx.x < T * 5) {
+    int t = threadIdx.x / 5, e = threadIdx.x - 5 * t;
+    double v = 0.0;
+    if (t < nst) {  // the table holds stages 2..s of this realization only
+      const StageCoef &k = a.coef[(size_t)(a.k0 + t) * a.nr_table + r];
+      v = e == 0 ? k.a : (e == 1 ? k.b : (e == 2 ? k.c : (e == 3 ? k.d : k.g)));
+    }
+    cfs[t][e] = v;
+  }
+  __syncthreads();
+  // register windows: w[t][d] = level t at row (q - 2t - d) after step q
+  double w[T + 1][5];
+#pragma unroll
+  for (int t = 0; t <= T; t++)
+#pragma unroll
+    for (int d = 0; d < 5; d++) w[t][d] = 0.0;
+  // u0 / dty0 history: h[d] = row (q - 2 - d)
+  double hu[2 * T], hd[2 * T];
+#pragma unroll
+  for (int d = 0; d < 2 * T; d++) hu[d] = hd[d] = 0.0;
+  int hi0 = min(ntm, j1 + T);
+  int nsteps = (hi0 - lo0) + 2 * T;
+  // Global inputs of march step i: row q of u_{k0-1}, row q-2 of u_{k0-2},
+  // u0 and dty0.  They are loaded two steps ahead into registers so the
+  // memory latency overlaps the stencil work and barriers of two steps.
+  double pf[2][4];
+  auto fetch = [&](int i, double *v) {
+    int q = lo0 + i, r1 = q - 2;
+    v[0] = v[1] = v[2] = v[3] = 0.0;
+    if (col && i < nsteps) {
+      if (q < hi0) v[0] = A[(size_t)q * pitch + c];
+      if (r1 >= 0 && r1 < ntm) {
+        size_t o1 = (size_t)r1 * pitch + c;
+        v[1] = B[o1];
+        v[2] = U0[o1];
+        v[3] = DT[o1];
+      }
+    }
+  };
+  fetch(0, pf[0]);
+  fetch(1, pf[1]);
+  for (int i = 0; i < nsteps; i++) {
+    int q = lo0 + i;
+    double a0 = pf[0][0], b1 = pf[0][1], u1 = pf[0][2], d1 = pf[0][3];
+#pragma unroll
+    for (int e = 0; e < 4; e++) pf[0][e] = pf[1][e];
+    fetch(i + 2, pf[1]);
+#pragma unroll
+    for (int d = 2 * T - 1; d > 0; d--) {
+      hu[d] = hu[d - 1];
+      hd[d] = hd[d - 1];
+    }
+    hu[0] = u1;
+    hd[0] = d1;
+#pragma unroll
+    for (int d = 4; d > 0; d--) w[0][d] = w[0][d - 1];
+    w[0][0] = a0;
+    if (q < hi0) lds[(0 * 4 + (q & 3)) * ncol + c] = a0;
+#pragma unroll
+    for (int t = 1; t <= T; t++) {
+      int R = q - 2 * t;
+      int rlo = max(0, j0 - T + t), rhi = min(ntm, j1 + T - t);
+      double v = 0.0;
+      if (t <= nst && R >= rlo && R < rhi) {
+        const double *lp = lds + (size_t)((t - 1) * 4) * ncol;  // level t-1 ring
+        double Lv;
+        if (R == 0) {
+          const double *r0 = lp + 0 * ncol, *r1p = lp + 1 * ncol;
+          double fn = wave_pole_sum(nu, [&](int k) { return (D2 * (r1p[k] - r0[k])) * m.dp[k + 1]; });
+          Lv = fn * m.bc_diff_n;
+        } else if (R == ntm - 1) {
+          const double *r0 = lp + ((ntm - 1) & 3) * ncol, *r1p = lp + ((ntm - 2) & 3) * ncol;
+          double fs = wave_pole_sum(nu, [&](int k) { return (D2 * (r0[k] - r1p[k])) * m.dp[k + 1]; });
+          Lv = -fs * m.bc_diff_s;
+        } else {
+          const double *rc = lp + (R & 3) * ncol;
+          double xc = w[t - 1][2], xn = w[t - 1][3], xs = w[t - 1][1];
+          double xw = rc[cw], xe = rc[ce];
+          double sti = m.st_i[R];
+          double c1 = (P1 * sti) * sti;
+          double c2 = D * m.dth_i[R];
+          c2 = c2 * m.dt_i[R];
+          c2 = c2 * sti;
+          c2 = c2 * m.sth[R];
+          double c4 = D * m.dth_i[R + 1];
+          c4 = c4 * m.dt_i[R];
+          c4 = c4 * sti;
+          c4 = c4 * m.sth[R + 1];
+          double c5 = (P5 * sti) * sti;
+          double c3 = -(((c1 + c2) + c4) + c5);
+          Lv = c1 * xw;
+          Lv = Lv + c2 * xn;
+          Lv = Lv + c3 * xc;
+          Lv = Lv + c4 * xs;
+          Lv = Lv + c5 * xe;
+        }
+        double ukm2 = (t == 1) ? b1 : w[t - 2][4];
+        const double *k = cfs[t - 1];
+        v = k[0] * w[t - 1][2];
+        v = v + k[1] * ukm2;
+        v = v + k[2] * hu[2 * t - 2];
+        v = v + k[3] * Lv;
+        v = v + k[4] * hd[2 * t - 2];
+        if (col && R >= j0 && R < j1) {
+          // u0 = f is read in the halo rows of neighbouring bands during the
+          // launch, so a realization's final stage goes to outA as well (the
+          // host copies it into f afterwards)
+          size_t o = (size_t)R * pitch + c;
+          if (t == nst) OA[o] = v;
+          if (!final_here && t == T - 1) OB[o] = v;
+        }
+      }
+#pragma unroll
+      for (int d = 4; d > 0; d--) w[t][d] = w[t][d - 1];
+      w[t][0] = v;
+      if (t < T && R >= 0 && R < ntm) lds[(size_t)(t * 4 + (R & 3)) * ncol + c] = v;
+    }
+    __syncthreads();
+  }
+}
+
 // Plain operator application y = L x (tests, PTL pass).
 struct LapArgs {
   Mesh m;
@@ -234,11 +417,17 @@ __global__ __launch_bounds__(kBlock) void k_laplacian(LapArgs a) {
     if (fabs(x[off]) > 1e-14) cand = fabs(L);
   }
   if (a.absmax) {
-    // wave max, then one atomic per wave (|Af| >= 0: integer order of the
-    // bit patterns is the numeric order)
+    // block max, then one atomic per block (|Af| >= 0: the integer order of
+    // the bit patterns is the numeric order)
+    __shared__ double wmax[kBlock / 64];
     for (int h = 32; h >= 1; h >>= 1) cand = fmax(cand, __shfl_xor(cand, h, 64));
-    if ((threadIdx.x & 63) == 0 && cand > 0.0)
-      atomicMax(&a.absmax[r], (unsigned long long)__double_as_longlong(cand));
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = cand;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double mx = wmax[0];
+      for (int q = 1; q < kBlock / 64; q++) mx = fmax(mx, wmax[q]);
+      if (mx > 0.0) atomicMax(&a.absmax[r], (unsigned long long)__double_as_longlong(mx));
+    }
   }
 }
 

@@ -98,6 +98,9 @@ struct hft_ctx {
   Mesh m;
   // fields
   double *f = nullptr, *buf0 = nullptr, *buf1 = nullptr, *dty0 = nullptr, *fval = nullptr;
+  double *buf2 = nullptr, *buf3 = nullptr;  // extra stage buffers of the fused RKG2 path
+  // tuning (hft_set_tuning): fused multi-stage RKG2 launches
+  int fused = 1, fused_T = 4, band_rows = 32, fused_min = 8;
   bool have_fval = false;
   // flows
   bool flow_set = false, flow2d = false;
@@ -644,21 +647,87 @@ int rkg2_superstep(hft_ctx *c, const std::vector<int> &reals, const std::vector<
   a1.Af = use_af ? c->dty0 : nullptr; a1.rstride = c->rstride;
   int rc = launch(c, 0, k_rkg2_stage1, nblocks(a1.g), a1, 3.0 * n);
   if (rc) return rc;
+  // Stage buffers: pA holds u_{k-1}, pB u_{k-2}.  Single-stage launches
+  // overwrite u_{k-2} in place (pointwise); fused launches write two fresh
+  // buffers.  u_s of a realization that finishes in a fused launch is left
+  // in that launch's outA and copied into f afterwards.
+  bool fused_ok = c->fused && c->nu <= 1024 && c->fused_T >= 2 && c->fused_T <= 4;
+  if (fused_ok && !c->buf2) {
+    if ((rc = dalloc(&c->buf2, c->nfield)) || (rc = dalloc(&c->buf3, c->nfield))) return rc;
+  }
+  double *pA = c->buf1, *pB = c->f;
+  std::vector<double *> final_loc(nr, nullptr);
+  auto pick_free = [&](double *x, double *y) {
+    double *cand[4] = {c->buf0, c->buf1, c->buf2, c->buf3};
+    for (double *q : cand)
+      if (q && q != x && q != y && q != pA && q != pB) return q;
+    return (double *)nullptr;
+  };
   StageArgs a;
   a.m = c->m; a.f = c->f; a.dty0 = c->dty0; a.D = c->d_D; a.s = c->d_s; a.order = c->d_order;
   a.rstride = c->rstride;
   int active = n;
-  for (int64_t k = 2; k <= smax; k++) {
+  int64_t k = 2;
+  while (k <= smax) {
     while (active > 0 && s[ord[active - 1]] < k) active--;
-    a.g = tiles_for(c, active);
-    a.k = (int)k;
-    a.x = (k & 1) ? c->buf0 : c->buf1;                   // u_{k-1}
-    a.ukm2 = (k == 2) ? c->f : ((k & 1) ? c->buf1 : c->buf0);
-    a.out = (k & 1) ? c->buf1 : c->buf0;
-    a.coef = c->d_coef + (size_t)k * nr;
-    rc = launch(c, 0, k_rkg2_stage, nblocks(a.g), a, 5.0 * active);
-    if (rc) return rc;
+    int T = c->fused_T;
+    if (fused_ok && active >= c->fused_min) {
+      double *oA = pick_free(nullptr, nullptr);
+      double *oB = pick_free(oA, nullptr);
+      FusedArgs fa;
+      fa.m = c->m;
+      fa.H = c->band_rows;
+      fa.nbands = (c->ntm + fa.H - 1) / fa.H;
+      fa.k0 = (int)k;
+      fa.ncol = ((c->nu + 63) / 64) * 64;
+      fa.A = pA; fa.B = pB; fa.f = c->f; fa.dty0 = c->dty0; fa.outA = oA; fa.outB = oB;
+      fa.D = c->d_D; fa.coef = c->d_coef; fa.nr_table = nr; fa.s = c->d_s; fa.order = c->d_order;
+      fa.rstride = c->rstride;
+      size_t shmem = (size_t)T * 4 * fa.ncol * sizeof(double);
+      double units = 0;
+      for (int z = 0; z < active; z++) {
+        int64_t st = std::min<int64_t>(T, s[ord[z]] - k + 1);
+        units += 5.0 * st;  // algorithmic passes: as if stage by stage
+        if (s[ord[z]] <= k + T - 1) final_loc[ord[z]] = oA;
+      }
+      int blocks = active * fa.nbands;
+      hipEvent_t e0 = nullptr, e1 = nullptr;
+      if (c->timing) {
+        if ((rc = ev_get(c, &e0)) || (rc = ev_get(c, &e1))) return rc;
+        HIPCHK(hipEventRecord(e0, c->stream));
+      }
+      switch (T) {
+        case 2: hipLaunchKernelGGL(k_rkg2_fused<2>, dim3(blocks), dim3(fa.ncol), shmem, c->stream, fa); break;
+        case 3: hipLaunchKernelGGL(k_rkg2_fused<3>, dim3(blocks), dim3(fa.ncol), shmem, c->stream, fa); break;
+        default: hipLaunchKernelGGL(k_rkg2_fused<4>, dim3(blocks), dim3(fa.ncol), shmem, c->stream, fa); break;
+      }
+      HIPCHK(hipGetLastError());
+      if (c->timing) {
+        HIPCHK(hipEventRecord(e1, c->stream));
+        c->pending.push_back({e0, e1, 0, units});
+      }
+      pA = oA;
+      pB = oB;
+      k += T;
+    } else {
+      double *out = (pB == c->f) ? pick_free(nullptr, nullptr) : pB;
+      a.g = tiles_for(c, active);
+      a.k = (int)k;
+      a.x = pA;
+      a.ukm2 = pB;
+      a.out = out;
+      a.coef = c->d_coef + (size_t)k * nr;
+      rc = launch(c, 0, k_rkg2_stage, nblocks(a.g), a, 5.0 * active);
+      if (rc) return rc;
+      pB = pA;
+      pA = out;
+      k += 1;
+    }
   }
+  for (int r = 0; r < nr; r++)
+    if (final_loc[r])
+      HIPCHK(hipMemcpyAsync(c->f + (size_t)r * c->rstride, final_loc[r] + (size_t)r * c->rstride,
+                            c->rstride * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
   return 0;
 }
 
@@ -776,6 +845,13 @@ int hft_create(int ntm, int npm1, const double *t, const double *p, int nr, int 
   }
   c->last_s.assign(nr, 0);
   c->last_cycles.assign(nr, 0);
+  {
+    // the fused RKG2 kernels use up to 4 stages x 4 rows x 1024 doubles of LDS
+    int lds4 = 4 * 4 * 1024 * (int)sizeof(double);
+    hipFuncSetAttribute((const void *)k_rkg2_fused<2>, hipFuncAttributeMaxDynamicSharedMemorySize, lds4 / 2);
+    hipFuncSetAttribute((const void *)k_rkg2_fused<3>, hipFuncAttributeMaxDynamicSharedMemorySize, lds4 * 3 / 4);
+    hipFuncSetAttribute((const void *)k_rkg2_fused<4>, hipFuncAttributeMaxDynamicSharedMemorySize, lds4);
+  }
   *out = c;
   return 0;
 }
@@ -784,7 +860,7 @@ int hft_destroy(hft_ctx *c) {
   if (!c) return 0;
   if (c->stream) hipStreamSynchronize(c->stream);
   for (double *d : c->dmesh) hipFree(d);
-  void *ptrs[] = {c->f, c->buf0, c->buf1, c->dty0, c->fval, c->d_vt1, c->d_vp1, c->d_at1,
+  void *ptrs[] = {c->f, c->buf0, c->buf1, c->buf2, c->buf3, c->dty0, c->fval, c->d_vt1, c->d_vp1, c->d_at1,
                   c->d_ap1, c->d_atten_inv, c->d_vt2, c->d_vp2, c->d_at2, c->d_ap2, c->d_D,
                   c->d_coef, c->d_s, c->d_order, c->d_order_id, c->d_cdt, c->d_ub1,
                   c->d_absmax, c->d_dtmin, c->d_target, c->d_part, c->d_hist, c->d_da_t,
@@ -1372,6 +1448,34 @@ int hft_synchronize(hft_ctx *c) {
 int hft_stream(hft_ctx *c, void **stream) {
   if (!c || !stream) return fail(HFT_E_ARG, "null argument");
   *stream = (void *)c->stream;
+  return 0;
+}
+
+int hft_set_tuning(hft_ctx *c, const char *key, int value) {
+  if (!c || !key) return fail(HFT_E_ARG, "hft_set_tuning: null argument");
+  if (!strcmp(key, "fused")) c->fused = value != 0;
+  else if (!strcmp(key, "fused_stages")) {
+    if (value < 2 || value > 4) return fail(HFT_E_ARG, "fused_stages must be 2..4");
+    c->fused_T = value;
+  } else if (!strcmp(key, "band_rows")) {
+    if (value < 4 || value > 4096) return fail(HFT_E_ARG, "band_rows must be 4..4096");
+    c->band_rows = value;
+  } else if (!strcmp(key, "fused_min_realizations")) {
+    if (value < 1) return fail(HFT_E_ARG, "fused_min_realizations must be >= 1");
+    c->fused_min = value;
+  } else {
+    return fail(HFT_E_ARG, "unknown tuning key '%s'", key);
+  }
+  return 0;
+}
+
+int hft_get_tuning(hft_ctx *c, const char *key, int *value) {
+  if (!c || !key || !value) return fail(HFT_E_ARG, "hft_get_tuning: null argument");
+  if (!strcmp(key, "fused")) *value = c->fused;
+  else if (!strcmp(key, "fused_stages")) *value = c->fused_T;
+  else if (!strcmp(key, "band_rows")) *value = c->band_rows;
+  else if (!strcmp(key, "fused_min_realizations")) *value = c->fused_min;
+  else return fail(HFT_E_ARG, "unknown tuning key '%s'", key);
   return 0;
 }
 

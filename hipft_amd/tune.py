@@ -1,0 +1,92 @@
+"""Launch autotuner for the batched RKG2 diffusion.
+
+The knobs (``hft_set_tuning``) are the stages per fused launch (temporal
+depth), the theta rows per block (tile height) and the number of still-active
+realizations below which stages go one per launch.  Results are identical
+for every setting, so the tuner only measures: it times diffusion super-steps
+of the live batch for each candidate and keeps the fastest.  Choices are
+cached in ``tuning_cache.json`` beside this file, keyed by device name, grid
+and batch size.
+"""
+import json
+import os
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CACHE = os.path.join(HERE, "tuning_cache.json")
+
+KEYS = ("fused", "fused_stages", "band_rows", "fused_min_realizations")
+
+
+def candidates(nr):
+    out = [dict(fused=0, fused_stages=4, band_rows=32, fused_min_realizations=1)]
+    for T in (2, 3, 4):
+        for H in (16, 32, 64, 128):
+            for m in sorted({1, 4, 8, 16, 32} & set(range(1, max(nr, 1) + 1))):
+                out.append(dict(fused=1, fused_stages=T, band_rows=H, fused_min_realizations=m))
+    return out
+
+
+def device_name():
+    import ctypes
+    try:
+        hip = ctypes.CDLL("libamdhip64.so")
+        dev = ctypes.c_int()
+        hip.hipGetDevice(ctypes.byref(dev))
+        buf = ctypes.create_string_buffer(256)
+        if hip.hipDeviceGetName(buf, 256, dev.value) == 0:
+            return buf.value.decode()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cache_key(model):
+    return f"{device_name() or 'gfx950'}|{model.ntm}x{model.npm1}|nr={model.nr}"
+
+
+def load_cache():
+    if os.path.exists(CACHE):
+        with open(CACHE) as fh:
+            return json.load(fh)
+    return {}
+
+
+def apply(model, cfg):
+    model.set_tuning(**{k: cfg[k] for k in KEYS})
+
+
+def autotune(model, dt=1.0, reps=2, use_cache=True, write_cache=True, cands=None, log=None):
+    """Pick and apply the fastest launch configuration for this batch."""
+    key = cache_key(model)
+    cache = load_cache()
+    if use_cache and key in cache:
+        apply(model, cache[key])
+        return cache[key]
+    state = model.get_state("tp")
+    best, best_t, results = None, float("inf"), []
+    for cfg in (cands or candidates(model.nr)):
+        apply(model, cfg)
+        model.set_state(state, layout="tp")
+        model.diffusion_step(dt, subcycles=1)  # warm
+        model.set_state(state, layout="tp")
+        model.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            model.diffusion_step(dt, subcycles=1)
+        model.synchronize()
+        el = (time.perf_counter() - t0) / reps
+        results.append((el, cfg))
+        if log:
+            log(f"tune {cfg} -> {1e3 * el:.2f} ms")
+        if el < best_t:
+            best, best_t = dict(cfg), el
+    model.set_state(state, layout="tp")
+    best["ms_per_superstep"] = round(1e3 * best_t, 3)
+    apply(model, best)
+    if write_cache:
+        cache = load_cache()
+        cache[key] = best
+        with open(CACHE, "w") as fh:
+            json.dump(cache, fh, indent=1, sort_keys=True)
+    return best

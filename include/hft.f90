@@ -173,6 +173,18 @@ module hft
       type(c_ptr), value :: ctx
       type(c_ptr), intent(out) :: stream
     end function
+    integer(c_int) function hft_set_tuning(ctx, key, value) bind(c, name="hft_set_tuning")
+      import :: c_int, c_ptr, c_char
+      type(c_ptr), value :: ctx
+      character(kind=c_char), intent(in) :: key(*)
+      integer(c_int), value :: value
+    end function
+    integer(c_int) function hft_get_tuning(ctx, key, value) bind(c, name="hft_get_tuning")
+      import :: c_int, c_ptr, c_char
+      type(c_ptr), value :: ctx
+      character(kind=c_char), intent(in) :: key(*)
+      integer(c_int), intent(out) :: value
+    end function
     integer(c_int) function hft_set_timing(ctx, enable) bind(c, name="hft_set_timing")
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx

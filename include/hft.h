@@ -177,6 +177,17 @@ int hft_state_device(hft_ctx *ctx, double **f_dev, int *pitch);
 int hft_synchronize(hft_ctx *ctx);
 /* HIP stream the context launches on (hipStream_t as void*). */
 int hft_stream(hft_ctx *ctx, void **stream);
+/* Launch tuning of one context (chosen per grid and batch by
+ * hipft_amd/tune.py; the defaults suit 512x1024 batches):
+ *   "fused"                  1 = run RKG2 stages several per launch (default)
+ *   "fused_stages"           stages per fused launch, 2..4 (default 4)
+ *   "band_rows"              theta rows per block of a fused launch (32)
+ *   "fused_min_realizations" fused launches only while at least this many
+ *                            realizations are still stepping (8)
+ * Results do not depend on these settings (bit-identical). */
+int hft_set_tuning(hft_ctx *ctx, const char *key, int value);
+int hft_get_tuning(hft_ctx *ctx, const char *key, int *value);
+
 /* Kernel timing with HIP events on the context's stream.  hft_set_timing
  * resets the counters and turns per-launch events on or off.
  * hft_kernel_time returns, for a kernel group ("rkg2", "weno", "ptl",
