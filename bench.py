@@ -104,6 +104,24 @@ def cpu_baseline(wl, budget_s=15.0, max_steps=20):
                        f"512x1024 workload on one host core, CPU oracle (C restatement)")
 
 
+def pmc_probe(m, nbytes=256 << 20):
+    """Counter calibration copy of a known size, then one workload step."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    assert hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) == 0  # device to device
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    step(m)
+    m.synchronize()
+    print(json.dumps({"pmc_probe": "done", "copy_bytes": nbytes}), flush=True)
+
+
 def load_traffic():
     path = os.path.join(ROOT, "profiles", "rkg2_stage_pmc.json")
     if not os.path.exists(path):
@@ -122,6 +140,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-single", action="store_true")
     ap.add_argument("--retune", action="store_true", help="ignore the cached launch tuning")
+    ap.add_argument("--pmc-probe", action="store_true",
+                    help="for rocprofv3 --pmc runs: a 256 MiB device-to-device copy (counter "
+                         "calibration) followed by one step of the workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
@@ -134,6 +155,9 @@ def main():
     from hipft_amd import tune
     tuning = tune.autotune(m, use_cache=not args.retune,
                            log=(lambda msg: print(msg, file=sys.stderr, flush=True)))
+    if args.pmc_probe:
+        pmc_probe(m)
+        return
     for _ in range(args.warmup):
         step(m)
     # timed region: throughput with per-kernel HIP events on the library's stream

@@ -207,14 +207,18 @@ __global__ __launch_bounds__(kBlock) void k_rkg2_stage(StageArgs a) {
 //
 // A block owns one realization and a band of H theta rows over the full phi
 // width (one thread per phi column, nu <= 1024).  It marches down the band
-// (plus a T-row halo above and below): at march step i it loads row q of
-// u_{k0-1} (level 0) and every stage t = 1..T computes row q - 2t of its
-// level from level t-1.  Level t-1's rows R-1, R, R+1 at the thread's own
-// column are in registers; row R at the neighbouring columns c-1, c+1 (and
-// the full rows the polar closure sums over) is read from a 4-row LDS ring
-// per level, written two march steps earlier and separated from the reads
-// by one barrier per step.  The lag of two rows per level puts both rows the
-// north-pole sum needs (0 and 1) in LDS before it is formed.
+// plus a T-row halo above and below: at march step i it loads row q of
+// u_{k0-1} (level 0) and stage t = 1..T computes row q - t of its level from
+// level t-1.  Level t-1's rows R-1, R, R+1 at the thread's own column are in
+// registers (R+1 was computed earlier in the same step); row R at the
+// neighbouring columns c-1, c+1 and the full rows the polar sums need come
+// from a 3-row LDS ring per level, written in an earlier step (one barrier
+// per step).  The north-pole row of stage t needs level t-1's rows 0 and 1
+// at every column, so it is formed one step late, together with row 1.
+//
+// Register windows and histories are circular buffers whose slots rotate
+// with the step index; the march is unrolled by 4 so every slot index is a
+// compile-time constant (no register moves).
 //
 // Only u_{k0-1}, u_{k0-2}, u0 and dty0 are read and the last two stages
 // written, so a launch moves about 4 + 2 fields (plus halo) for T stages
@@ -238,11 +242,147 @@ struct FusedArgs {
   size_t rstride;
 };
 
+constexpr int kFusedMaxRows = 160;  // H + 2T rows of per-row coefficients
+
 template <int T>
-__global__ __launch_bounds__(1024) void k_rkg2_fused(FusedArgs a) {
-  extern __shared__ double lds[];  // [T][4][ncol]
+struct FusedState {
+  double w[T][4];     // level t (0..T-1) own-column values, slot = age-rotated
+  double p0[T];       // level t, north-pole row 0 (formed late, kept aside)
+  double hu[4], hd[4];  // u0 and dty0 at rows q-1-d
+  double pf[2][4];    // prefetched inputs of the next two steps
+};
+
+template <int T, int U>
+__device__ __forceinline__ void fused_step(const FusedArgs &a, FusedState<T> &S, double *lds,
+                                           const double *rowc, int i, int nsteps, int lo0,
+                                           int hi0, int j0, int j1, int nst, bool final_here,
+                                           bool col, int c, int cw, int ce, double D2,
+                                           double P1, double P5, const double *A,
+                                           const double *B, const double *U0,
+                                           const double *DT, double *OA, double *OB,
+                                           double u0_pole, double dt_pole,
+                                           const double (*cfs)[5]) {
   const Mesh &m = a.m;
   const int ncol = a.ncol, nu = m.nu, ntm = m.ntm, pitch = m.pitch;
+  // slot of age d at this sub-step: (U - d) & 3
+  constexpr int s0 = U & 3, s1 = (U + 3) & 3, s2 = (U + 2) & 3;
+  constexpr int p0s = U & 1, p1s = (U + 1) & 1;
+  int q = lo0 + i;
+  double a0 = S.pf[p0s][0], b1 = S.pf[p0s][1], u1 = S.pf[p0s][2], d1 = S.pf[p0s][3];
+  {
+    // prefetch step i+2 into the slot just consumed
+    int qn = q + 2, rn = qn - 1;
+    double v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0;
+    if (col && i + 2 < nsteps) {
+      if (qn < hi0) v0 = A[(size_t)qn * pitch + c];
+      if (rn >= 0 && rn < ntm) {
+        size_t o = (size_t)rn * pitch + c;
+        v1 = B[o];
+        v2 = U0[o];
+        v3 = DT[o];
+      }
+    }
+    S.pf[p0s][0] = v0;
+    S.pf[p0s][1] = v1;
+    S.pf[p0s][2] = v2;
+    S.pf[p0s][3] = v3;
+  }
+  (void)p1s;
+  // u0/dty0 history: slot of row q-1-d is (U - d) & 3
+  S.hu[s0] = u1;
+  S.hd[s0] = d1;
+  // level 0
+  S.w[0][s0] = a0;
+  if (q < hi0) {
+    lds[(size_t)(0 * 3 + (q % 3)) * ncol + c] = a0;
+    if (q == 0) S.p0[0] = a0;
+  }
+#pragma unroll
+  for (int t = 1; t <= T; t++) {
+    int R = q - t;
+    int rlo = max(0, j0 - T + t), rhi = min(ntm, j1 + T - t);
+    const double *lp = lds + (size_t)((t - 1) * 3) * ncol;  // level t-1 ring
+    const double *k = cfs[t - 1];
+    // u0/dty0 at row R = q - t: age t-1
+    double u0v = (t == 1) ? S.hu[s0] : (t == 2 ? S.hu[s1] : (t == 3 ? S.hu[s2] : S.hu[(U + 1) & 3]));
+    double dtv = (t == 1) ? S.hd[s0] : (t == 2 ? S.hd[s1] : (t == 3 ? S.hd[s2] : S.hd[(U + 1) & 3]));
+    bool act = t <= nst && R >= rlo && R < rhi;
+    double v = 0.0;
+    if (act && R == 1 && rlo == 0) {
+      // late north-pole row 0 of stage t, from level t-1 rows 0 and 1 in LDS
+      const double *r0 = lp + 0 * ncol, *r1p = lp + 1 * ncol;
+      double fn = wave_pole_sum(nu, [&](int kk) { return (D2 * (r1p[kk] - r0[kk])) * m.dp[kk + 1]; });
+      double Lv = fn * m.bc_diff_n;
+      double xc = (t == 1) ? S.p0[0] : S.p0[t - 1];
+      double ukm2 = (t == 1) ? B[c < nu ? c : 0] : S.p0[t - 2];
+      double vv = k[0] * xc;
+      vv = vv + k[1] * ukm2;
+      vv = vv + k[2] * u0_pole;
+      vv = vv + k[3] * Lv;
+      vv = vv + k[4] * dt_pole;
+      if (t < T) S.p0[t] = vv;
+      if (t < T) lds[(size_t)(t * 3 + 0) * ncol + c] = vv;
+      if (col && 0 >= j0 && 0 < j1) {
+        if (t == nst) OA[c] = vv;
+        if (!final_here && t == T - 1) OB[c] = vv;
+      }
+    }
+    if (act && R > 0) {
+      double Lv;
+      if (R == ntm - 1) {
+        const double *r0 = lp + ((ntm - 1) % 3) * ncol, *r1p = lp + ((ntm - 2) % 3) * ncol;
+        double fs = wave_pole_sum(nu, [&](int kk) { return (D2 * (r0[kk] - r1p[kk])) * m.dp[kk + 1]; });
+        Lv = -fs * m.bc_diff_s;
+      } else {
+        const double *rc = lp + (R % 3) * ncol;
+        double xc = S.w[t - 1][s1];
+        double xs = S.w[t - 1][s0];
+        double xn = (R == 1) ? S.p0[t - 1] : S.w[t - 1][s2];
+        double xw = rc[cw], xe = rc[ce];
+        const double *rr = rowc + 4 * (R - lo0);
+        double sti = rr[0];
+        double c1 = (P1 * sti) * sti;
+        double c2 = rr[1];
+        double c4 = rr[2];
+        double c5 = (P5 * sti) * sti;
+        double c3 = -(((c1 + c2) + c4) + c5);
+        Lv = c1 * xw;
+        Lv = Lv + c2 * xn;
+        Lv = Lv + c3 * xc;
+        Lv = Lv + c4 * xs;
+        Lv = Lv + c5 * xe;
+      }
+      double xc = S.w[t - 1][s1];
+      double ukm2;
+      if (t == 1) ukm2 = b1;
+      else if (R == 1 && t >= 2) ukm2 = S.w[t - 2][s2];
+      else ukm2 = S.w[t - 2][s2];
+      v = k[0] * xc;
+      v = v + k[1] * ukm2;
+      v = v + k[2] * u0v;
+      v = v + k[3] * Lv;
+      v = v + k[4] * dtv;
+      if (col && R >= j0 && R < j1) {
+        size_t o = (size_t)R * pitch + c;
+        if (t == nst) OA[o] = v;
+        if (!final_here && t == T - 1) OB[o] = v;
+      }
+    }
+    if (t < T) {
+      S.w[t][s0] = v;
+      if (act && R > 0) lds[(size_t)(t * 3 + (R % 3)) * ncol + c] = v;
+    }
+  }
+  __syncthreads();
+}
+
+template <int T>
+__global__ __launch_bounds__(1024) void k_rkg2_fused(FusedArgs a) {
+  extern __shared__ double lds[];  // [T][3][ncol]
+  __shared__ double cfs[4][5];
+  __shared__ double rowc[4 * kFusedMaxRows];
+  const Mesh &m = a.m;
+  const int nu = m.nu, ntm = m.ntm;
   int L = xcd_remap(blockIdx.x, gridDim.x);
   int z = L / a.nbands, band = L - z * a.nbands;
   int r = a.order[z];
@@ -251,7 +391,7 @@ __global__ __launch_bounds__(1024) void k_rkg2_fused(FusedArgs a) {
   int cc = col ? c : 0;
   int cw = wrapc(cc - 1, nu), ce = wrapc(cc + 1, nu);
   int j0 = band * a.H, j1 = min(ntm, j0 + a.H);
-  int lo0 = max(0, j0 - T);
+  int lo0 = max(0, j0 - T), hi0 = min(ntm, j1 + T);
   int64_t sr = a.s[r];
   int nst = (int)((sr - a.k0 + 1) < T ? (sr - a.k0 + 1) : T);
   bool final_here = (a.k0 + nst - 1) == sr;
@@ -262,8 +402,7 @@ __global__ __launch_bounds__(1024) void k_rkg2_fused(FusedArgs a) {
   double D2 = D + D;
   double P1 = (D * m.dph_i[cc + 1]) * m.dp_i[cc + 1];
   double P5 = (D * m.dph_i[cc + 2]) * m.dp_i[cc + 1];
-  // stage coefficients are block-uniform: keep them in LDS, not registers
-  __shared__ double cfs[4][5];
+  // stage coefficients (block-uniform) and per-row coefficient factors
   if (threadIdx.x < T * 5) {
     int t = threadIdx.x / 5, e = threadIdx.x - 5 * t;
     double v = 0.0;
@@ -273,115 +412,73 @@ __global__ __launch_bounds__(1024) void k_rkg2_fused(FusedArgs a) {
     }
     cfs[t][e] = v;
   }
-  __syncthreads();
-  // register windows: w[t][d] = level t at row (q - 2t - d) after step q
-  double w[T + 1][5];
+  for (int R = lo0 + (int)threadIdx.x; R < hi0; R += blockDim.x) {
+    double *rr = rowc + 4 * (R - lo0);
+    if (R > 0 && R < ntm - 1) {
+      double sti = m.st_i[R];
+      double c2 = D * m.dth_i[R];
+      c2 = c2 * m.dt_i[R];
+      c2 = c2 * sti;
+      c2 = c2 * m.sth[R];
+      double c4 = D * m.dth_i[R + 1];
+      c4 = c4 * m.dt_i[R];
+      c4 = c4 * sti;
+      c4 = c4 * m.sth[R + 1];
+      rr[0] = sti;
+      rr[1] = c2;
+      rr[2] = c4;
+    } else {
+      rr[0] = rr[1] = rr[2] = 0.0;
+    }
+    rr[3] = 0.0;
+  }
+  FusedState<T> S;
 #pragma unroll
-  for (int t = 0; t <= T; t++)
+  for (int t = 0; t < T; t++) {
+    S.p0[t] = 0.0;
 #pragma unroll
-    for (int d = 0; d < 5; d++) w[t][d] = 0.0;
-  // u0 / dty0 history: h[d] = row (q - 2 - d)
-  double hu[2 * T], hd[2 * T];
+    for (int d = 0; d < 4; d++) S.w[t][d] = 0.0;
+  }
 #pragma unroll
-  for (int d = 0; d < 2 * T; d++) hu[d] = hd[d] = 0.0;
-  int hi0 = min(ntm, j1 + T);
-  int nsteps = (hi0 - lo0) + 2 * T;
-  // Global inputs of march step i: row q of u_{k0-1}, row q-2 of u_{k0-2},
-  // u0 and dty0.  They are loaded two steps ahead into registers so the
-  // memory latency overlaps the stencil work and barriers of two steps.
-  double pf[2][4];
-  auto fetch = [&](int i, double *v) {
-    int q = lo0 + i, r1 = q - 2;
-    v[0] = v[1] = v[2] = v[3] = 0.0;
+  for (int d = 0; d < 4; d++) S.hu[d] = S.hd[d] = 0.0;
+  double u0_pole = 0.0, dt_pole = 0.0;
+  if (lo0 == 0 && col) {
+    u0_pole = U0[c];
+    dt_pole = DT[c];
+  }
+  int nsteps = (hi0 - lo0) + T;
+  // steps 0 and 1 inputs: pf slot (i & 1)
+#pragma unroll
+  for (int i = 0; i < 2; i++) {
+    int q = lo0 + i, rn = q - 1;
+    double v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0;
     if (col && i < nsteps) {
-      if (q < hi0) v[0] = A[(size_t)q * pitch + c];
-      if (r1 >= 0 && r1 < ntm) {
-        size_t o1 = (size_t)r1 * pitch + c;
-        v[1] = B[o1];
-        v[2] = U0[o1];
-        v[3] = DT[o1];
+      if (q < hi0) v0 = A[(size_t)q * m.pitch + c];
+      if (rn >= 0 && rn < ntm) {
+        size_t o = (size_t)rn * m.pitch + c;
+        v1 = B[o];
+        v2 = U0[o];
+        v3 = DT[o];
       }
     }
-  };
-  fetch(0, pf[0]);
-  fetch(1, pf[1]);
-  for (int i = 0; i < nsteps; i++) {
-    int q = lo0 + i;
-    double a0 = pf[0][0], b1 = pf[0][1], u1 = pf[0][2], d1 = pf[0][3];
-#pragma unroll
-    for (int e = 0; e < 4; e++) pf[0][e] = pf[1][e];
-    fetch(i + 2, pf[1]);
-#pragma unroll
-    for (int d = 2 * T - 1; d > 0; d--) {
-      hu[d] = hu[d - 1];
-      hd[d] = hd[d - 1];
-    }
-    hu[0] = u1;
-    hd[0] = d1;
-#pragma unroll
-    for (int d = 4; d > 0; d--) w[0][d] = w[0][d - 1];
-    w[0][0] = a0;
-    if (q < hi0) lds[(0 * 4 + (q & 3)) * ncol + c] = a0;
-#pragma unroll
-    for (int t = 1; t <= T; t++) {
-      int R = q - 2 * t;
-      int rlo = max(0, j0 - T + t), rhi = min(ntm, j1 + T - t);
-      double v = 0.0;
-      if (t <= nst && R >= rlo && R < rhi) {
-        const double *lp = lds + (size_t)((t - 1) * 4) * ncol;  // level t-1 ring
-        double Lv;
-        if (R == 0) {
-          const double *r0 = lp + 0 * ncol, *r1p = lp + 1 * ncol;
-          double fn = wave_pole_sum(nu, [&](int k) { return (D2 * (r1p[k] - r0[k])) * m.dp[k + 1]; });
-          Lv = fn * m.bc_diff_n;
-        } else if (R == ntm - 1) {
-          const double *r0 = lp + ((ntm - 1) & 3) * ncol, *r1p = lp + ((ntm - 2) & 3) * ncol;
-          double fs = wave_pole_sum(nu, [&](int k) { return (D2 * (r0[k] - r1p[k])) * m.dp[k + 1]; });
-          Lv = -fs * m.bc_diff_s;
-        } else {
-          const double *rc = lp + (R & 3) * ncol;
-          double xc = w[t - 1][2], xn = w[t - 1][3], xs = w[t - 1][1];
-          double xw = rc[cw], xe = rc[ce];
-          double sti = m.st_i[R];
-          double c1 = (P1 * sti) * sti;
-          double c2 = D * m.dth_i[R];
-          c2 = c2 * m.dt_i[R];
-          c2 = c2 * sti;
-          c2 = c2 * m.sth[R];
-          double c4 = D * m.dth_i[R + 1];
-          c4 = c4 * m.dt_i[R];
-          c4 = c4 * sti;
-          c4 = c4 * m.sth[R + 1];
-          double c5 = (P5 * sti) * sti;
-          double c3 = -(((c1 + c2) + c4) + c5);
-          Lv = c1 * xw;
-          Lv = Lv + c2 * xn;
-          Lv = Lv + c3 * xc;
-          Lv = Lv + c4 * xs;
-          Lv = Lv + c5 * xe;
-        }
-        double ukm2 = (t == 1) ? b1 : w[t - 2][4];
-        const double *k = cfs[t - 1];
-        v = k[0] * w[t - 1][2];
-        v = v + k[1] * ukm2;
-        v = v + k[2] * hu[2 * t - 2];
-        v = v + k[3] * Lv;
-        v = v + k[4] * hd[2 * t - 2];
-        if (col && R >= j0 && R < j1) {
-          // u0 = f is read in the halo rows of neighbouring bands during the
-          // launch, so a realization's final stage goes to outA as well (the
-          // host copies it into f afterwards)
-          size_t o = (size_t)R * pitch + c;
-          if (t == nst) OA[o] = v;
-          if (!final_here && t == T - 1) OB[o] = v;
-        }
-      }
-#pragma unroll
-      for (int d = 4; d > 0; d--) w[t][d] = w[t][d - 1];
-      w[t][0] = v;
-      if (t < T && R >= 0 && R < ntm) lds[(size_t)(t * 4 + (R & 3)) * ncol + c] = v;
-    }
-    __syncthreads();
+    S.pf[i][0] = v0;
+    S.pf[i][1] = v1;
+    S.pf[i][2] = v2;
+    S.pf[i][3] = v3;
+  }
+  __syncthreads();
+  for (int i = 0; i < nsteps; i += 4) {
+    fused_step<T, 0>(a, S, lds, rowc, i, nsteps, lo0, hi0, j0, j1, nst, final_here, col, c, cw,
+                     ce, D2, P1, P5, A, B, U0, DT, OA, OB, u0_pole, dt_pole, cfs);
+    if (i + 1 < nsteps)
+      fused_step<T, 1>(a, S, lds, rowc, i + 1, nsteps, lo0, hi0, j0, j1, nst, final_here, col,
+                       c, cw, ce, D2, P1, P5, A, B, U0, DT, OA, OB, u0_pole, dt_pole, cfs);
+    if (i + 2 < nsteps)
+      fused_step<T, 2>(a, S, lds, rowc, i + 2, nsteps, lo0, hi0, j0, j1, nst, final_here, col,
+                       c, cw, ce, D2, P1, P5, A, B, U0, DT, OA, OB, u0_pole, dt_pole, cfs);
+    if (i + 3 < nsteps)
+      fused_step<T, 3>(a, S, lds, rowc, i + 3, nsteps, lo0, hi0, j0, j1, nst, final_here, col,
+                       c, cw, ce, D2, P1, P5, A, B, U0, DT, OA, OB, u0_pole, dt_pole, cfs);
   }
 }
 

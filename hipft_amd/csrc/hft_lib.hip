@@ -651,7 +651,8 @@ int rkg2_superstep(hft_ctx *c, const std::vector<int> &reals, const std::vector<
   // overwrite u_{k-2} in place (pointwise); fused launches write two fresh
   // buffers.  u_s of a realization that finishes in a fused launch is left
   // in that launch's outA and copied into f afterwards.
-  bool fused_ok = c->fused && c->nu <= 1024 && c->fused_T >= 2 && c->fused_T <= 4;
+  bool fused_ok = c->fused && c->nu <= 1024 && c->fused_T >= 2 && c->fused_T <= 4 &&
+                  c->band_rows + 2 * c->fused_T <= kFusedMaxRows;
   if (fused_ok && !c->buf2) {
     if ((rc = dalloc(&c->buf2, c->nfield)) || (rc = dalloc(&c->buf3, c->nfield))) return rc;
   }
@@ -683,7 +684,7 @@ int rkg2_superstep(hft_ctx *c, const std::vector<int> &reals, const std::vector<
       fa.A = pA; fa.B = pB; fa.f = c->f; fa.dty0 = c->dty0; fa.outA = oA; fa.outB = oB;
       fa.D = c->d_D; fa.coef = c->d_coef; fa.nr_table = nr; fa.s = c->d_s; fa.order = c->d_order;
       fa.rstride = c->rstride;
-      size_t shmem = (size_t)T * 4 * fa.ncol * sizeof(double);
+      size_t shmem = (size_t)T * 3 * fa.ncol * sizeof(double);
       double units = 0;
       for (int z = 0; z < active; z++) {
         int64_t st = std::min<int64_t>(T, s[ord[z]] - k + 1);
@@ -846,11 +847,11 @@ int hft_create(int ntm, int npm1, const double *t, const double *p, int nr, int 
   c->last_s.assign(nr, 0);
   c->last_cycles.assign(nr, 0);
   {
-    // the fused RKG2 kernels use up to 4 stages x 4 rows x 1024 doubles of LDS
-    int lds4 = 4 * 4 * 1024 * (int)sizeof(double);
-    hipFuncSetAttribute((const void *)k_rkg2_fused<2>, hipFuncAttributeMaxDynamicSharedMemorySize, lds4 / 2);
-    hipFuncSetAttribute((const void *)k_rkg2_fused<3>, hipFuncAttributeMaxDynamicSharedMemorySize, lds4 * 3 / 4);
-    hipFuncSetAttribute((const void *)k_rkg2_fused<4>, hipFuncAttributeMaxDynamicSharedMemorySize, lds4);
+    // the fused RKG2 kernels use T stages x 3 rows x up to 1024 doubles of LDS
+    int row = 3 * 1024 * (int)sizeof(double);
+    hipFuncSetAttribute((const void *)k_rkg2_fused<2>, hipFuncAttributeMaxDynamicSharedMemorySize, 2 * row);
+    hipFuncSetAttribute((const void *)k_rkg2_fused<3>, hipFuncAttributeMaxDynamicSharedMemorySize, 3 * row);
+    hipFuncSetAttribute((const void *)k_rkg2_fused<4>, hipFuncAttributeMaxDynamicSharedMemorySize, 4 * row);
   }
   *out = c;
   return 0;
