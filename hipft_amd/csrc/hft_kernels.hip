@@ -807,6 +807,162 @@ __global__ __launch_bounds__(kBlock) void k_advection_stage(AdvArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// Banded WENO3 stage: each face flux computed once.
+//
+// A block owns one realization and a band of theta rows over the full phi
+// width (one thread per column) and marches down the band.  Theta: the
+// thread keeps its column's Lax-Friedrichs split fluxes LP/LN of four rows in
+// registers and computes the face below the current row once, carrying the
+// face above from the previous row.  Phi: each row's split fluxes go to LDS;
+// a thread forms its cell's west face, publishes it in LDS and takes the east
+// face from its neighbour.  Pole rows sum the upwind pole-face fluxes over the
+// full row (fixed wave order).  Expressions are those of advection_op.
+// ---------------------------------------------------------------------------
+struct AdvBandArgs {
+  Mesh m;
+  Flow1D f1;
+  Flow2D f2;
+  const double *x, *y;
+  double *out;
+  double ca, cb, cdt;
+  int write_aop;
+  int nbands, H, ncol;
+  const int *order;
+  size_t rstride;
+};
+
+template <bool F2D, bool HAS_Y>
+__global__ __launch_bounds__(1024) void k_advection_band(AdvBandArgs a) {
+  extern __shared__ double sh[];  // phi split fluxes LN, LP and faces: 3 x ncol
+  const Mesh &m = a.m;
+  const int ncol = a.ncol, nu = m.nu, ntm = m.ntm, nt = m.nt, pitch = m.pitch;
+  double *sLN = sh, *sLP = sh + ncol, *sFP = sh + 2 * ncol;
+  int L = xcd_remap(blockIdx.x, gridDim.x);
+  int z = L / a.nbands, band = L - z * a.nbands;
+  int r = a.order[z];
+  int c = threadIdx.x;
+  bool col = c < nu;
+  int cc = col ? c : 0;
+  FlowAcc<F2D> fa;
+  fa.f1 = a.f1;
+  fa.f2 = a.f2;
+  fa.nt = nt;
+  fa.ntm = ntm;
+  fa.pitch = pitch;
+  fa.nu = nu;
+  fa.rstride = a.rstride;
+  fa.r = r;
+  size_t base = (size_t)r * a.rstride;
+  const double *x = a.x + base;
+  const double half = 0.5;
+  int j0 = band * a.H, j1 = min(ntm, j0 + a.H);
+  // theta split fluxes of the own column: LN/LP of reference cell jr
+  auto ln_t = [&](int j0r, double xv) { return half * xv * (fa.vt(j0r, cc) + fa.at(j0r, cc)); };
+  auto lp_t = [&](int j0r, double xv) { return half * xv * (fa.vt(j0r + 1, cc) - fa.at(j0r, cc)); };
+  auto xat = [&](int j) { return (j >= 0 && j < ntm) ? x[(size_t)j * pitch + cc] : 0.0; };
+  // theta face flux below row j (between rows j and j+1, reference face j+2)
+  auto tface = [&](int j, const double *LN4, const double *LP4, double xj, double xj1) -> double {
+    int jf = j + 2;  // reference face
+    if (jf == 2 || jf == nt - 1) {
+      const double one = 1.0;
+      double v = fa.vt(jf - 1, cc);
+      double cct = fsign1(v);
+      // face 2: (one-cct)*x(row1) + (one+cct)*x(row0); face nt-1: rows ntm-1, ntm-2
+      return v * half * ((one - cct) * xj1 + (one + cct) * xj);
+    }
+    WenoW w = weno_w(m.tCP, m.tMC, m.tMT, m.tCPT, m.tPT, m.tMCT, m.teps, jf - 2, jf - 1);
+    // LN at rows j-1, j, j+1; LP at rows j, j+1, j+2 (slots 0..3 = rows j-1..j+2)
+    return weno_face(LN4[0], LN4[1], LN4[2], LP4[1], LP4[2], LP4[3], w);
+  };
+  // rows j-1..j+2 of the own column
+  double X[4], LN4[4], LP4[4];
+  for (int d = 0; d < 4; d++) {
+    int jj = j0 - 1 + d;
+    X[d] = xat(jj);
+    LN4[d] = (jj >= 0 && jj < ntm) ? ln_t(jj, X[d]) : 0.0;
+    LP4[d] = (jj >= 0 && jj < ntm) ? lp_t(jj, X[d]) : 0.0;
+  }
+  // face above the first row (between j0-1 and j0)
+  double ft_up = 0.0;
+  if (j0 >= 1) {
+    double Xm[4], LNm[4], LPm[4];
+    for (int d = 0; d < 4; d++) {
+      int jj = j0 - 2 + d;
+      Xm[d] = xat(jj);
+      LNm[d] = (jj >= 0 && jj < ntm) ? ln_t(jj, Xm[d]) : 0.0;
+      LPm[d] = (jj >= 0 && jj < ntm) ? lp_t(jj, Xm[d]) : 0.0;
+    }
+    ft_up = tface(j0 - 1, LNm, LPm, Xm[1], Xm[2]);
+  }
+  for (int j = j0; j < j1; j++) {
+    // X[1] = row j, X[0] = j-1, X[2] = j+1, X[3] = j+2
+    bool pole = (j == 0 || j == ntm - 1);
+    double ft_dn = 0.0;
+    if (j < ntm - 1) ft_dn = tface(j, LN4, LP4, X[1], X[2]);
+    double A;
+    if (pole) {
+      // upwind pole-face flux of the own column, summed over the row
+      double fface = (j == 0) ? ft_dn : ft_up;
+      if (col) sFP[c] = fface * m.dp[c + 1];
+      __syncthreads();
+      double s = wave_pole_sum(nu, [&](int k) { return sFP[k]; });
+      A = (j == 0) ? s * m.bc_flow_n : -s * m.bc_flow_s;
+      __syncthreads();
+    } else {
+      // phi split fluxes of row j (own cell), shared through LDS
+      double xv = X[1];
+      double av = fa.ap(j, cc);
+      sLN[c] = half * xv * (fa.vp(j, cc) + av);
+      sLP[c] = half * xv * (fa.vp(j, wrapc(cc + 1, nu)) - av);
+      __syncthreads();
+      // west face of the own cell: face q = c
+      double fpw = 0.0;
+      if (col) {
+        int c2 = wrapc(cc - 2, nu), c1 = wrapc(cc - 1, nu), cp = wrapc(cc + 1, nu);
+        WenoW w = weno_w(m.pCP, m.pMC, m.pMT, m.pCPT, m.pPT, m.pMCT, m.peps, cc, cc + 1);
+        fpw = weno_face(sLN[c2], sLN[c1], sLN[cc], sLP[c1], sLP[cc], sLP[cp], w);
+      }
+      sFP[c] = fpw;
+      __syncthreads();
+      double fpe = sFP[wrapc(cc + 1, nu)];
+      double sti = m.st_i[j];
+      double at_ = (m.sth[j + 1] * ft_dn - m.sth[j] * ft_up) * sti;
+      at_ = at_ * m.dt_i[j];
+      double ap_ = (fpe - fpw) * sti;
+      ap_ = ap_ * m.dp_i[cc + 1];
+      A = at_ + ap_;
+      __syncthreads();
+    }
+    if (col) {
+      size_t off = (size_t)j * pitch + c;
+      double v;
+      if (a.write_aop) {
+        v = A;
+      } else if (HAS_Y) {
+        v = a.ca * a.y[base + off] + a.cb * X[1];
+        v = v - a.cdt * A;
+      } else {
+        v = X[1] - a.cdt * A;
+      }
+      a.out[base + off] = v;
+    }
+    // advance the window by one row
+    ft_up = ft_dn;
+    int jn = j + 3;
+    double xn = xat(jn);
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      X[d] = X[d + 1];
+      LN4[d] = LN4[d + 1];
+      LP4[d] = LP4[d + 1];
+    }
+    X[3] = xn;
+    LN4[3] = (jn < ntm) ? ln_t(jn, xn) : 0.0;
+    LP4[3] = (jn < ntm) ? lp_t(jn, xn) : 0.0;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Analysis (analysis_step, src/hipft.f90:3145-3325): per-block partials then
 // a fixed-order final reduction.
 // ---------------------------------------------------------------------------

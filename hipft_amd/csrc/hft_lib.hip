@@ -80,8 +80,10 @@ struct TimedPair {
   double units;  // algorithmic field passes x realizations of the launch
 };
 
-const char *kGroups[] = {"rkg2", "weno", "ptl", "analysis", "assim", "flow"};
-constexpr int kNGroups = 6;
+// Kernel timing groups; launches timed under "rkg2_fused" also count in "rkg2".
+const char *kGroups[] = {"rkg2", "weno", "ptl", "analysis", "assim", "flow", "rkg2_fused"};
+constexpr int kNGroups = 7;
+constexpr int kGroupFused = 6;
 
 }  // namespace
 
@@ -101,6 +103,8 @@ struct hft_ctx {
   double *buf2 = nullptr, *buf3 = nullptr;  // extra stage buffers of the fused RKG2 path
   // tuning (hft_set_tuning): fused multi-stage RKG2 launches
   int fused = 1, fused_T = 4, band_rows = 32, fused_min = 8;
+  int fused_target_blocks = 0;  // shrink bands while fewer blocks than this (0: off)
+  int adv_band = 1, adv_band_rows = 16;  // banded WENO3 stage kernel
   bool have_fval = false;
   // flows
   bool flow_set = false, flow2d = false;
@@ -173,6 +177,11 @@ int drain_timing(hft_ctx *c) {
     c->t_ms[tp.group] += ms;
     c->t_n[tp.group] += 1;
     c->t_units[tp.group] += tp.units;
+    if (tp.group == kGroupFused) {
+      c->t_ms[0] += ms;
+      c->t_n[0] += 1;
+      c->t_units[0] += tp.units;
+    }
     c->free_ev.push_back(tp.a);
     c->free_ev.push_back(tp.b);
   }
@@ -184,7 +193,8 @@ int drain_timing(hft_ctx *c) {
 // units: compulsory field passes (reads + writes of a realization's field)
 // the launch makes, times its realizations; used for algorithmic bytes.
 template <typename K, typename A>
-int launch(hft_ctx *c, int group, K kern, int blocks, const A &args, double units) {
+int launch_ex(hft_ctx *c, int group, K kern, int blocks, int threads, size_t shmem, const A &args,
+              double units) {
   if (blocks <= 0) return 0;
   hipEvent_t a = nullptr, b = nullptr;
   if (c->timing) {
@@ -197,13 +207,18 @@ int launch(hft_ctx *c, int group, K kern, int blocks, const A &args, double unit
     if (rc) return rc;
     HIPCHK(hipEventRecord(a, c->stream));
   }
-  hipLaunchKernelGGL(kern, dim3(blocks), dim3(kBlock), 0, c->stream, args);
+  hipLaunchKernelGGL(kern, dim3(blocks), dim3(threads), shmem, c->stream, args);
   HIPCHK(hipGetLastError());
   if (c->timing) {
     HIPCHK(hipEventRecord(b, c->stream));
     c->pending.push_back({a, b, group, units});
   }
   return 0;
+}
+
+template <typename K, typename A>
+int launch(hft_ctx *c, int group, K kern, int blocks, const A &args, double units) {
+  return launch_ex(c, group, kern, blocks, kBlock, 0, args, units);
 }
 
 Tiles tiles_for(const hft_ctx *c, int nreal) {
@@ -678,6 +693,12 @@ int rkg2_superstep(hft_ctx *c, const std::vector<int> &reals, const std::vector<
       FusedArgs fa;
       fa.m = c->m;
       fa.H = c->band_rows;
+      if (c->fused_target_blocks > 0 &&
+          (int64_t)active * ((c->ntm + fa.H - 1) / fa.H) < c->fused_target_blocks) {
+        // few realizations left: thinner bands so the launch still fills the GPU
+        int64_t h = (int64_t)c->ntm * active / c->fused_target_blocks;
+        fa.H = (int)std::max<int64_t>(4, std::min<int64_t>(fa.H, h));
+      }
       fa.nbands = (c->ntm + fa.H - 1) / fa.H;
       fa.k0 = (int)k;
       fa.ncol = ((c->nu + 63) / 64) * 64;
@@ -692,21 +713,12 @@ int rkg2_superstep(hft_ctx *c, const std::vector<int> &reals, const std::vector<
         if (s[ord[z]] <= k + T - 1) final_loc[ord[z]] = oA;
       }
       int blocks = active * fa.nbands;
-      hipEvent_t e0 = nullptr, e1 = nullptr;
-      if (c->timing) {
-        if ((rc = ev_get(c, &e0)) || (rc = ev_get(c, &e1))) return rc;
-        HIPCHK(hipEventRecord(e0, c->stream));
-      }
       switch (T) {
-        case 2: hipLaunchKernelGGL(k_rkg2_fused<2>, dim3(blocks), dim3(fa.ncol), shmem, c->stream, fa); break;
-        case 3: hipLaunchKernelGGL(k_rkg2_fused<3>, dim3(blocks), dim3(fa.ncol), shmem, c->stream, fa); break;
-        default: hipLaunchKernelGGL(k_rkg2_fused<4>, dim3(blocks), dim3(fa.ncol), shmem, c->stream, fa); break;
+        case 2: rc = launch_ex(c, kGroupFused, k_rkg2_fused<2>, blocks, fa.ncol, shmem, fa, units); break;
+        case 3: rc = launch_ex(c, kGroupFused, k_rkg2_fused<3>, blocks, fa.ncol, shmem, fa, units); break;
+        default: rc = launch_ex(c, kGroupFused, k_rkg2_fused<4>, blocks, fa.ncol, shmem, fa, units); break;
       }
-      HIPCHK(hipGetLastError());
-      if (c->timing) {
-        HIPCHK(hipEventRecord(e1, c->stream));
-        c->pending.push_back({e0, e1, 0, units});
-      }
+      if (rc) return rc;
       pA = oA;
       pB = oB;
       k += T;
@@ -1111,6 +1123,28 @@ int hft_dt_euler(hft_ctx *c, double *dt, double *dt_r) {
 
 static int advection_stage(hft_ctx *c, const double *x, const double *y, double *out, double ca,
                            double cb, double cdt, bool aop_only) {
+  double u = (y ? 3.0 : 2.0) * c->nr;
+  if (c->adv_band && c->nu <= 1024) {
+    AdvBandArgs a;
+    a.m = c->m;
+    a.f1.vt = c->d_vt1; a.f1.at = c->d_at1; a.f1.vp = c->d_vp1; a.f1.ap = c->d_ap1;
+    a.f2.vt = c->d_vt2; a.f2.vp = c->d_vp2; a.f2.at = c->d_at2; a.f2.ap = c->d_ap2;
+    a.f2.vt_rstride = c->vt_rstride;
+    a.x = x; a.y = y; a.out = out; a.ca = ca; a.cb = cb; a.cdt = cdt;
+    a.write_aop = aop_only ? 1 : 0;
+    a.H = c->adv_band_rows;
+    a.nbands = (c->ntm + a.H - 1) / a.H;
+    a.ncol = ((c->nu + 63) / 64) * 64;
+    a.order = c->d_order_id; a.rstride = c->rstride;
+    int blocks = c->nr * a.nbands;
+    size_t shmem = (size_t)3 * a.ncol * sizeof(double);
+    if (c->flow2d) {
+      if (y) return launch_ex(c, 1, k_advection_band<true, true>, blocks, a.ncol, shmem, a, u);
+      return launch_ex(c, 1, k_advection_band<true, false>, blocks, a.ncol, shmem, a, u);
+    }
+    if (y) return launch_ex(c, 1, k_advection_band<false, true>, blocks, a.ncol, shmem, a, u);
+    return launch_ex(c, 1, k_advection_band<false, false>, blocks, a.ncol, shmem, a, u);
+  }
   AdvArgs a;
   a.m = c->m;
   a.g = tiles_for(c, c->nr);
@@ -1120,7 +1154,6 @@ static int advection_stage(hft_ctx *c, const double *x, const double *y, double 
   a.x = x; a.y = y; a.out = out; a.ca = ca; a.cb = cb; a.cdt = cdt;
   a.order = c->d_order_id; a.rstride = c->rstride; a.write_aop = aop_only ? 1 : 0;
   int nb = nblocks(a.g);
-  double u = (y ? 3.0 : 2.0) * c->nr;
   if (c->flow2d) {
     if (y) return launch(c, 1, k_advection_stage<true, true>, nb, a, u);
     return launch(c, 1, k_advection_stage<true, false>, nb, a, u);
@@ -1464,6 +1497,14 @@ int hft_set_tuning(hft_ctx *c, const char *key, int value) {
   } else if (!strcmp(key, "fused_min_realizations")) {
     if (value < 1) return fail(HFT_E_ARG, "fused_min_realizations must be >= 1");
     c->fused_min = value;
+  } else if (!strcmp(key, "fused_target_blocks")) {
+    if (value < 0) return fail(HFT_E_ARG, "fused_target_blocks must be >= 0");
+    c->fused_target_blocks = value;
+  } else if (!strcmp(key, "advection_band")) {
+    c->adv_band = value != 0;
+  } else if (!strcmp(key, "advection_band_rows")) {
+    if (value < 1 || value > 4096) return fail(HFT_E_ARG, "advection_band_rows must be 1..4096");
+    c->adv_band_rows = value;
   } else {
     return fail(HFT_E_ARG, "unknown tuning key '%s'", key);
   }
@@ -1476,6 +1517,9 @@ int hft_get_tuning(hft_ctx *c, const char *key, int *value) {
   else if (!strcmp(key, "fused_stages")) *value = c->fused_T;
   else if (!strcmp(key, "band_rows")) *value = c->band_rows;
   else if (!strcmp(key, "fused_min_realizations")) *value = c->fused_min;
+  else if (!strcmp(key, "fused_target_blocks")) *value = c->fused_target_blocks;
+  else if (!strcmp(key, "advection_band")) *value = c->adv_band;
+  else if (!strcmp(key, "advection_band_rows")) *value = c->adv_band_rows;
   else return fail(HFT_E_ARG, "unknown tuning key '%s'", key);
   return 0;
 }

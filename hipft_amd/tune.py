@@ -1,8 +1,10 @@
 """Launch autotuner for the batched RKG2 diffusion.
 
 The knobs (``hft_set_tuning``) are the stages per fused launch (temporal
-depth), the theta rows per block (tile height) and the number of still-active
-realizations below which stages go one per launch.  Results are identical
+depth), the theta rows per block (tile height), the number of still-active
+realizations below which stages go one per launch, and the block count below
+which a fused launch thins its bands (the tail of a batch whose
+realizations have different stage counts).  Results are identical
 for every setting, so the tuner only measures: it times diffusion super-steps
 of the live batch for each candidate and keeps the fastest.  Choices are
 cached in ``tuning_cache.json`` beside this file, keyed by device name, grid
@@ -15,15 +17,18 @@ import time
 HERE = os.path.dirname(os.path.abspath(__file__))
 CACHE = os.path.join(HERE, "tuning_cache.json")
 
-KEYS = ("fused", "fused_stages", "band_rows", "fused_min_realizations")
+KEYS = ("fused", "fused_stages", "band_rows", "fused_min_realizations", "fused_target_blocks")
 
 
 def candidates(nr):
-    out = [dict(fused=0, fused_stages=4, band_rows=32, fused_min_realizations=1)]
-    for T in (2, 3, 4):
-        for H in (16, 32, 64, 128):
+    out = [dict(fused=0, fused_stages=4, band_rows=32, fused_min_realizations=1,
+                fused_target_blocks=0)]
+    for T in (3, 4):
+        for H in (32, 64, 128):
             for m in sorted({1, 4, 8, 16, 32} & set(range(1, max(nr, 1) + 1))):
-                out.append(dict(fused=1, fused_stages=T, band_rows=H, fused_min_realizations=m))
+                for B in (0, 256, 512):
+                    out.append(dict(fused=1, fused_stages=T, band_rows=H,
+                                    fused_min_realizations=m, fused_target_blocks=B))
     return out
 
 
@@ -53,7 +58,7 @@ def load_cache():
 
 
 def apply(model, cfg):
-    model.set_tuning(**{k: cfg[k] for k in KEYS})
+    model.set_tuning(**{k: cfg.get(k, 0) for k in KEYS})
 
 
 def autotune(model, dt=1.0, reps=2, use_cache=True, write_cache=True, cands=None, log=None):

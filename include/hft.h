@@ -184,6 +184,12 @@ int hft_stream(hft_ctx *ctx, void **stream);
  *   "band_rows"              theta rows per block of a fused launch (32)
  *   "fused_min_realizations" fused launches only while at least this many
  *                            realizations are still stepping (8)
+ *   "fused_target_blocks"    when a fused launch would have fewer blocks
+ *                            (realizations x bands) than this, its bands are
+ *                            made thinner (>= 4 rows) to fill the GPU (0: off)
+ *   "advection_band"         1 = WENO3 stages by theta bands computing each
+ *                            face once (default); 0 = one thread per point
+ *   "advection_band_rows"    theta rows per block of the banded WENO3 (16)
  * Results do not depend on these settings (bit-identical). */
 int hft_set_tuning(hft_ctx *ctx, const char *key, int value);
 int hft_get_tuning(hft_ctx *ctx, const char *key, int *value);
@@ -191,7 +197,8 @@ int hft_get_tuning(hft_ctx *ctx, const char *key, int *value);
 /* Kernel timing with HIP events on the context's stream.  hft_set_timing
  * resets the counters and turns per-launch events on or off.
  * hft_kernel_time returns, for a kernel group ("rkg2", "weno", "ptl",
- * "analysis", "assim") since the reset: total device time (ms), number of
+ * "analysis", "assim", "flow", or "rkg2_fused" = the multi-stage RKG2
+ * launches, which "rkg2" includes) since the reset: total device time (ms), number of
  * launches, and the algorithmic bytes those launches move (compulsory
  * reads + writes of FP64 fields over the ntm x (npm-2) unique points). */
 int hft_set_timing(hft_ctx *ctx, int enable);

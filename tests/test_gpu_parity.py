@@ -280,18 +280,21 @@ def test_atten_map_vs_reference(gpu_available):
     assert err <= 1e-10, cols
 
 
-@pytest.mark.parametrize("T,H", [(2, 4), (3, 7), (4, 32), (4, 5)])
-def test_fused_rkg2_bitexact(gpu_available, T, H):
+@pytest.mark.parametrize("T,H,B", [(2, 4, 0), (3, 7, 0), (4, 32, 0), (4, 5, 0), (4, 32, 40),
+                                   (3, 64, 1000)])
+def test_fused_rkg2_bitexact(gpu_available, T, H, B):
     """Several RKG2 stages per launch (LDS row rings, band halos) give the
-    oracle's result bit for bit, across band boundaries and the poles."""
+    oracle's result bit for bit, across band boundaries and the poles, also
+    when the bands thin out as realizations finish (B = target blocks)."""
     o, g = pair(91, 181, nr=3)
     eta = [150.0, 450.0, 950.0]
     o.load_diffusion(eta)
     g.set_diffusion(eta)
-    g.set_tuning(fused=1, fused_stages=T, band_rows=H, fused_min_realizations=1)
+    g.set_tuning(fused=1, fused_stages=T, band_rows=H, fused_min_realizations=1,
+                 fused_target_blocks=B)
     o.diffusion_step(6.0, subcycles=0)
     g.diffusion_step(6.0, subcycles=0)
-    assert_same(g.get_state("tp"), o.get_f(), f"fused T={T} H={H}")
+    assert_same(g.get_state("tp"), o.get_f(), f"fused T={T} H={H} B={B}")
 
 
 def test_fused_matches_single_stage_full_size(gpu_available):
@@ -306,10 +309,51 @@ def test_fused_matches_single_stage_full_size(gpu_available):
         g = FluxTransport(t, p, nr, mode="independent")
         g.set_state(maps, layout="pt")
         g.set_diffusion(eta)
-        g.set_tuning(fused=fused, fused_min_realizations=1)
+        g.set_tuning(fused=fused, fused_min_realizations=1, fused_target_blocks=512 * fused)
         g.diffusion_step(1.0, subcycles=0)
         out.append(g.get_state("tp"))
         s, _ = g.diffusion_stats()
         g.close()
     assert len(set(s.tolist())) == nr
     assert_same(out[1], out[0], "fused vs single-stage at 512x1024")
+
+
+@pytest.mark.parametrize("H", [1, 2, 5, 16, 61])
+def test_banded_advection_bitexact(gpu_available, H):
+    """Banded WENO3 stages (faces computed once, bands marching in theta) give
+    the oracle's operator and SSPRK(4,3) steps bit for bit for every band
+    height, including bands that begin or end on a pole row."""
+    o, g = pair(61, 121, nr=2)
+    _flows(o, g, 2)
+    g.set_tuning(advection_band=1, advection_band_rows=H)
+    x = o.get_f()
+    assert_same(g.apply_advection_operator(x), o.advection_operator(x), f"A x band H={H}")
+    for _ in range(2):
+        o.advection_step(0.5)
+        g.advection_step(0.5)
+    assert_same(g.get_state("tp"), o.get_f(), f"SSPRK(4,3) band H={H}")
+
+
+@pytest.mark.parametrize("atten", [False, True])
+def test_banded_advection_matches_pointwise_full_size(gpu_available, atten):
+    """512x1024, 1-D flow profiles and attenuated (2-D) flows: banded and
+    one-thread-per-point WENO3 stages agree bit for bit."""
+    t, p = grid(512, 1024)
+    nr = 3
+    maps = np.stack([W.bipole_map(t, p, 40, 200 + r) for r in range(nr)])
+    vt = np.stack([W.meridional_profile(t, 12.0 + 3.0 * r) for r in range(nr)])
+    p0, p2, p4 = (np.array([c * (1.0 + 0.04 * r) for r in range(nr)]) for c in W.dr_coefficients())
+    out = []
+    for band in (0, 1):
+        g = FluxTransport(t, p, nr, mode="independent")
+        g.set_state(maps, layout="pt")
+        g.set_flow_profiles(vt, None)
+        g.set_flow_analytic(p0=p0, p2=p2, p4=p4)
+        if atten:
+            g.set_flow_attenuation([300.0, 200.0, 150.0])
+            g.update_flow()
+        g.set_tuning(advection_band=band)
+        g.advection_step(0.5)
+        out.append(g.get_state("tp"))
+        g.close()
+    assert_same(out[1], out[0], "banded vs pointwise WENO3 at 512x1024")
