@@ -207,7 +207,7 @@ def main():
                    "ntm": 512, "npm1": 1024, "realizations": nr,
                    "stages_last_step_min_max": [int(s_last.min()), int(s_last.max())],
                    "parallelism": "realization batch on 1 GPU",
-                   "launch_tuning": {k: tuning.get(k, 0) for k in tune.KEYS}},
+                   "launch_tuning": {k: tuning.get(k, tune.DEFAULTS[k]) for k in tune.KEYS}},
         "roofline": roof,
         "kernels": groups,
     }

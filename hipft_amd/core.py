@@ -83,7 +83,15 @@ _SIGS = [
                                       ctypes.POINTER(ctypes.c_int)]),
     ("hft_set_timing", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("hft_kernel_time", ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, _D, _I64, _D]),
+    ("hft_set_state_device", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]),
+    ("hft_get_state_device", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]),
+    ("hft_set_stream", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]),
+    ("hft_step_adjoint", ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_int,
+                                        ctypes.c_int, ctypes.c_void_p, ctypes.c_int64, _D,
+                                        ctypes.c_int]),
 ]
+
+ADJ_FROZEN_WENO = 1
 
 _LIB = None
 
@@ -195,6 +203,7 @@ class FluxTransport:
                                            _p(_vec(p4, n)), _p(_vec(p1, n)), _p(_vec(p3, n)),
                                            _p(_vec(p5, n)), float(vp_rigid_omega),
                                            float(vt_const)), "hft_set_flow_analytic")
+        self._flow = True
 
     def set_flow_profiles(self, vt=None, vp=None):
         a = None if vt is None else np.ascontiguousarray(
@@ -202,6 +211,10 @@ class FluxTransport:
         b = None if vp is None else np.ascontiguousarray(
             np.broadcast_to(np.asarray(vp, dtype=np.float64), (self.nr, self.ntm)))
         _check(lib().hft_set_flow_profiles(self._h, _p(a), _p(b)), "hft_set_flow_profiles")
+        self._flow = True
+
+    def has_flow(self):
+        return getattr(self, "_flow", False)
 
     def set_flow_attenuation(self, values):
         _check(lib().hft_set_flow_attenuation(self._h, _p(_vec(values, self.nr))),
@@ -344,6 +357,35 @@ class FluxTransport:
 
     def set_timing(self, enable):
         _check(lib().hft_set_timing(self._h, int(bool(enable))), "hft_set_timing")
+
+    # ---- device-resident state and the adjoint -----------------------------
+    def set_state_device(self, ptr, ld):
+        """Copy a device array [nr][ntm][ld] (unique phi columns) into the state."""
+        _check(lib().hft_set_state_device(self._h, ctypes.c_void_p(ptr), int(ld)),
+               "hft_set_state_device")
+
+    def get_state_device(self, ptr, ld):
+        _check(lib().hft_get_state_device(self._h, ctypes.c_void_p(ptr), int(ld)),
+               "hft_get_state_device")
+
+    def set_stream(self, stream_ptr):
+        """Launch on a caller's HIP stream (0 is the null stream, as torch's
+        default stream reports it); None: the context's own stream again."""
+        if stream_ptr is None:
+            _check(lib().hft_set_stream(self._h, None, 1), "hft_set_stream")
+        else:
+            _check(lib().hft_set_stream(self._h, ctypes.c_void_p(stream_ptr or None), 0),
+                   "hft_set_stream")
+
+    def step_adjoint(self, dt, lam_ptr, ld, strang=True, subcycles=1, frozen_weno=False):
+        """Adjoint of one flux_transport_step from the current state.  lam (device
+        [nr][ntm][ld]) goes from dJ/dx_{n+1} to dJ/dx_n in place; returns
+        [nr][3] = dJ/dlog(eta), dJ/dlog(vt scale), dJ/dlog(vp scale)."""
+        g = np.zeros((self.nr, 3))
+        _check(lib().hft_step_adjoint(self._h, float(dt), int(bool(strang)), int(subcycles),
+                                      ctypes.c_void_p(lam_ptr), int(ld), _p(g),
+                                      ADJ_FROZEN_WENO if frozen_weno else 0), "hft_step_adjoint")
+        return g
 
     def kernel_time(self, group):
         """(total device ms, launches, algorithmic bytes) of a kernel group."""

@@ -1,0 +1,396 @@
+// hft_adjoint.hip -- host controller of the discrete adjoint of one
+// flux-transport step (part of the libhft translation unit: included by
+// hft_lib.hip after the forward controller).
+//
+// hft_step_adjoint recomputes the step from the context's state, storing what
+// the transposed sweeps need (the four SSPRK(4,3) stage inputs of each
+// advection step, the input of every diffusion cycle and, per cycle, RKG2
+// stage checkpoints), then runs the sweeps backwards:
+//   Strang:  lam <- A'^T lam, D'^T lam, A'^T lam
+// The RKG2 recurrence is transposed stage by stage (k = s..1); its stage
+// states come from segment recomputation: pairs (u_{k0-1}, u_{k0-2}) are kept
+// at segment starts and each segment is recomputed before its adjoint stages
+// (a single segment holding every stage when memory allows).
+
+namespace {
+
+double *adj_buf(hft_ctx *c, std::vector<double *> &v, size_t i, int *rc) {
+  while (v.size() <= i) v.push_back(nullptr);
+  if (!v[i]) {
+    *rc = dalloc(&v[i], c->nfield);
+    if (*rc) return nullptr;
+  }
+  return v[i];
+}
+
+// Fixed adjoint buffers
+enum {
+  AB_XA = 0,     // 4: stage inputs of the first advection step
+  AB_XB = 4,     // 4: stage inputs of the second advection step (Strang)
+  AB_T1 = 8,
+  AB_T2 = 9,
+  AB_LF = 10,
+  AB_LAM = 11,   // adjoint state (pitched)
+  AB_DTY = 12,   // dty0 of the cycle
+  AB_LB = 13,    // 3: rolling RKG2 stage adjoints
+  AB_L0 = 16,
+  AB_M = 17,
+  AB_R = 18,     // 3: rolling stage states of the checkpoint pass
+  AB_U1 = 21,
+  AB_COUNT = 22
+};
+
+int fill_dev(hft_ctx *c, double *p, size_t n, double v) {
+  if (n == 0) return 0;
+  size_t nb = (n + kBlock - 1) / kBlock;
+  hipLaunchKernelGGL(k_fill, dim3((unsigned)nb), dim3(kBlock), 0, c->stream, p, n, v);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int copy_real(hft_ctx *c, double *dst, const double *src, int r) {
+  size_t off = (size_t)r * c->rstride;
+  HIPCHK(hipMemcpyAsync(dst + off, src + off, c->rstride * sizeof(double),
+                        hipMemcpyDeviceToDevice, c->stream));
+  return 0;
+}
+
+// Forward advection step from f, keeping the stage inputs X[0..3]
+// (X[0] = step input); same launches as hft_advection_step.
+int advection_store(hft_ctx *c, double dtl, double *X[4]) {
+  int rc;
+  HIPCHK(hipMemcpyAsync(X[0], c->f, c->nfield * sizeof(double), hipMemcpyDeviceToDevice,
+                        c->stream));
+  if ((rc = advection_stage(c, X[0], nullptr, X[1], 0, 0, half * dtl, false))) return rc;
+  if ((rc = advection_stage(c, X[1], nullptr, X[2], 0, 0, half * dtl, false))) return rc;
+  if ((rc = advection_stage(c, X[2], X[0], X[3], two_third, third, sixth * dtl, false))) return rc;
+  return advection_stage(c, X[3], nullptr, c->f, 0, 0, half * dtl, false);
+}
+
+int vjp_stage(hft_ctx *c, const double *x, const double *lam, const double *P, double *out,
+              double *acc, double ca, double cb, double cdt, bool frozen) {
+  AdvVjpArgs a;
+  a.m = c->m;
+  a.f1.vt = c->d_vt1; a.f1.at = c->d_at1; a.f1.vp = c->d_vp1; a.f1.ap = c->d_ap1;
+  a.x = x; a.lam = lam; a.P = P; a.out = out; a.acc = acc;
+  a.ca = ca; a.cb = cb; a.cdt = cdt;
+  a.H = c->adj_band_rows;
+  a.nbands = (c->ntm + a.H - 1) / a.H;
+  a.ncol = ((c->nu + 63) / 64) * 64;
+  a.rstride = c->rstride;
+  a.part = c->d_apA;
+  int blocks = c->nr * a.nbands;
+  size_t shmem = ((size_t)9 * a.ncol + 16) * sizeof(double);
+  double u = 4.0 * c->nr;
+  if (frozen) {
+    if (P) return launch_ex(c, kGroupAdjoint, k_advection_vjp_band<true, true>, blocks, a.ncol, shmem, a, u);
+    return launch_ex(c, kGroupAdjoint, k_advection_vjp_band<true, false>, blocks, a.ncol, shmem, a, u);
+  }
+  if (P) return launch_ex(c, kGroupAdjoint, k_advection_vjp_band<false, true>, blocks, a.ncol, shmem, a, u);
+  return launch_ex(c, kGroupAdjoint, k_advection_vjp_band<false, false>, blocks, a.ncol, shmem, a, u);
+}
+
+// Adjoint of one SSPRK(4,3) step (forward: X1 = X0 - h W X0, X2 = X1 - h W X1,
+// X3 = 2/3 X0 + 1/3 X2 - dtl/6 W X2, out = X3 - h W X3), lam in place.
+int advection_adjoint(hft_ctx *c, double dtl, double *X[4], double *lam, bool frozen) {
+  int rc;
+  double h = half * dtl;
+  double *T1 = adj_buf(c, c->adj_fix, AB_T1, &rc);
+  if (!T1) return rc;
+  double *T2 = adj_buf(c, c->adj_fix, AB_T2, &rc);
+  if (!T2) return rc;
+  double *LF = adj_buf(c, c->adj_fix, AB_LF, &rc);
+  if (!LF) return rc;
+  if ((rc = vjp_stage(c, X[3], lam, nullptr, T1, nullptr, 0, one, h, frozen))) return rc;
+  if ((rc = vjp_stage(c, X[2], T1, nullptr, T2, LF, two_third, third, sixth * dtl, frozen))) return rc;
+  if ((rc = vjp_stage(c, X[1], T2, nullptr, T1, nullptr, 0, one, h, frozen))) return rc;
+  return vjp_stage(c, X[0], T1, LF, lam, nullptr, 0, one, h, frozen);
+}
+
+struct CycleRec {
+  std::vector<int> reals;
+  std::vector<double> dt;     // [nr]
+  std::vector<int64_t> s;     // [nr]
+};
+
+// Adjoint of one round of RKG2 cycles from its input U0 (realizations of
+// the round only; the others pass through unchanged).
+int cycle_adjoint(hft_ctx *c, const CycleRec &rec, const double *U0, double *lam) {
+  int nr = c->nr, rc;
+  std::vector<int> ord;
+  if ((rc = upload_rkg2(c, rec.reals, rec.dt, rec.s, ord))) return rc;
+  int n = (int)ord.size();
+  const std::vector<int64_t> &s = rec.s;
+  int64_t smax = s[ord[0]];
+  auto nactive = [&](int64_t kmin) {  // realizations with s_r >= kmin (a prefix of ord)
+    int q = 0;
+    while (q < n && s[ord[q]] >= kmin) q++;
+    return q;
+  };
+  // segment length
+  int64_t K = c->adj_seg;
+  if (K <= 0) {
+    double field_mb = (double)c->nfield * 8.0 / 1048576.0;
+    double all_mb = (double)(smax + 24) * field_mb;
+    K = all_mb <= (double)c->adj_mem_mb ? smax - 1
+                                        : (int64_t)std::ceil(std::sqrt((double)(smax - 1)));
+  }
+  if (K < 1) K = 1;
+  int64_t nseg = (smax - 1 + K - 1) / K;
+  double *DTY = adj_buf(c, c->adj_fix, AB_DTY, &rc);
+  if (!DTY) return rc;
+  double *U1 = adj_buf(c, c->adj_fix, AB_U1, &rc);
+  if (!U1) return rc;
+  double *LB[3], *R[3];
+  for (int i = 0; i < 3; i++) {
+    if (!(LB[i] = adj_buf(c, c->adj_fix, AB_LB + i, &rc))) return rc;
+  }
+  double *L0 = adj_buf(c, c->adj_fix, AB_L0, &rc);
+  if (!L0) return rc;
+  double *M = adj_buf(c, c->adj_fix, AB_M, &rc);
+  if (!M) return rc;
+  // stage 1: dty0 = dt L u0, u1 = u0 + ub1 dty0
+  Stage1Args a1;
+  a1.m = c->m; a1.g = tiles_for(c, n); a1.f = U0; a1.dty0 = DTY; a1.u1 = U1;
+  a1.D = c->d_D; a1.dt = c->d_cdt; a1.ub1 = c->d_ub1; a1.order = c->d_order;
+  a1.Af = nullptr; a1.rstride = c->rstride;
+  if ((rc = launch(c, 0, k_rkg2_stage1, nblocks(a1.g), a1, 3.0 * n))) return rc;
+  StageArgs sa;
+  sa.m = c->m; sa.f = const_cast<double *>(U0); sa.dty0 = DTY; sa.D = c->d_D; sa.s = c->d_s;
+  sa.order = c->d_order; sa.rstride = c->rstride;
+  auto fwd_stage = [&](int64_t k, const double *x, const double *xm2, double *out) -> int {
+    int na = nactive(k + 1);  // u_k is needed by realizations with s_r >= k+1
+    if (na == 0) return 0;
+    sa.g = tiles_for(c, na);
+    sa.k = (int)k;
+    sa.x = x;
+    sa.ukm2 = xm2;
+    sa.out = out;
+    sa.coef = c->d_coef + (size_t)k * nr;
+    return launch(c, 0, k_rkg2_stage, nblocks(sa.g), sa, 5.0 * na);
+  };
+  // checkpoint pass: pairs (u_{k0-1}, u_{k0-2}) at segment starts k0 = 2 + i K
+  std::vector<const double *> ck1(nseg), ck2(nseg);
+  ck1[0] = U1;
+  ck2[0] = U0;
+  if (nseg > 1) {
+    for (int i = 0; i < 3; i++)
+      if (!(R[i] = adj_buf(c, c->adj_fix, AB_R + i, &rc))) return rc;
+    const double *pm2 = U0, *pm1 = U1;
+    int ri = 0;
+    int64_t klast = 2 + (nseg - 1) * K;
+    for (int64_t k = 2; k <= klast - 1; k++) {
+      double *out = R[ri];
+      ri = (ri + 1) % 3;
+      if ((rc = fwd_stage(k, pm1, pm2, out))) return rc;
+      pm2 = pm1;
+      pm1 = out;
+      if ((k - 1) % K == 0) {  // k = k0 - 1 of segment i = (k-1)/K
+        int64_t i = (k - 1) / K;
+        double *b1 = adj_buf(c, c->adj_ck, 2 * i, &rc);
+        if (!b1) return rc;
+        double *b2 = adj_buf(c, c->adj_ck, 2 * i + 1, &rc);
+        if (!b2) return rc;
+        HIPCHK(hipMemcpyAsync(b1, pm1, c->nfield * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(b2, pm2, c->nfield * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        ck1[i] = b1;
+        ck2[i] = b2;
+      }
+    }
+  }
+  // adjoint initial values: lam_{s_r} = lam_r, partial lam_{s_r - 1} = 0
+  if ((rc = fill_dev(c, L0, c->nfield, 0.0))) return rc;
+  if ((rc = fill_dev(c, M, c->nfield, 0.0))) return rc;
+  for (int z = 0; z < n; z++) {
+    int r = ord[z];
+    if ((rc = copy_real(c, LB[s[r] % 3], lam, r))) return rc;
+    if ((rc = fill_dev(c, LB[(s[r] - 1) % 3] + (size_t)r * c->rstride, c->rstride, 0.0))) return rc;
+  }
+  AdjStageArgs ja;
+  ja.m = c->m; ja.L0 = L0; ja.M = M; ja.D = c->d_D; ja.ub1 = c->d_ub1; ja.order = c->d_order;
+  ja.rstride = c->rstride; ja.part = c->d_apD;
+  for (int64_t i = nseg - 1; i >= 0; i--) {
+    int64_t k0 = 2 + i * K, k1 = std::min<int64_t>(k0 + K - 1, smax);
+    // W[t] = u_{k0-2+t}
+    std::vector<const double *> W(k1 - k0 + 2);
+    W[0] = ck2[i];
+    W[1] = ck1[i];
+    for (int64_t k = k0; k <= k1 - 1; k++) {
+      double *out = adj_buf(c, c->adj_w, (size_t)(k - k0), &rc);
+      if (!out) return rc;
+      if ((rc = fwd_stage(k, W[k - k0 + 1], W[k - k0], out))) return rc;
+      W[k - k0 + 2] = out;
+    }
+    for (int64_t k = k1; k >= k0; k--) {
+      int na = nactive(k);
+      if (na == 0) continue;
+      ja.g = tiles_for(c, na);
+      ja.k = (int)k;
+      ja.lam = LB[k % 3];
+      ja.P = LB[(k - 1) % 3];
+      ja.Pn = LB[(k - 2) % 3];
+      ja.u = W[k - k0 + 1];
+      ja.coef = c->d_coef + (size_t)k * nr;
+      if ((rc = launch(c, kGroupAdjoint, k_rkg2_adj_stage, nblocks(ja.g), ja, 9.0 * na))) return rc;
+    }
+  }
+  AdjCloseArgs ca;
+  ca.m = c->m; ca.g = tiles_for(c, n); ca.M = M; ca.L0 = L0; ca.u0 = U0; ca.D = c->d_D;
+  ca.dt = c->d_cdt; ca.order = c->d_order; ca.rstride = c->rstride; ca.part = c->d_apD;
+  if ((rc = launch(c, kGroupAdjoint, k_rkg2_adj_close, nblocks(ca.g), ca, 4.0 * n))) return rc;
+  for (int z = 0; z < n; z++)
+    if ((rc = copy_real(c, lam, L0, ord[z]))) return rc;
+  return 0;
+}
+
+}  // namespace
+
+void set_adjoint_attributes() {
+  int lds = (9 * 1024 + 16) * (int)sizeof(double);
+  hipFuncSetAttribute((const void *)k_advection_vjp_band<true, true>,
+                      hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+  hipFuncSetAttribute((const void *)k_advection_vjp_band<true, false>,
+                      hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+  hipFuncSetAttribute((const void *)k_advection_vjp_band<false, true>,
+                      hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+  hipFuncSetAttribute((const void *)k_advection_vjp_band<false, false>,
+                      hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+}
+
+extern "C" {
+
+int hft_set_stream(hft_ctx *c, void *stream, int own) {
+  if (!c) return fail(HFT_E_ARG, "null context");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (!own) {
+    if (!c->own_stream) c->own_stream = c->stream;
+    c->stream = (hipStream_t)stream;  // NULL: the null stream
+  } else if (c->own_stream) {
+    c->stream = c->own_stream;
+    c->own_stream = nullptr;
+  }
+  return 0;
+}
+
+int hft_set_state_device(hft_ctx *c, const double *src, int64_t ld) {
+  if (!c || !src) return fail(HFT_E_ARG, "null argument");
+  if (ld < c->nu) return fail(HFT_E_ARG, "hft_set_state_device: ld < %d", c->nu);
+  HIPCHK(hipMemcpy2DAsync(c->f, c->pitch * sizeof(double), src, (size_t)ld * sizeof(double),
+                          c->nu * sizeof(double), (size_t)c->nr * c->ntm, hipMemcpyDeviceToDevice,
+                          c->stream));
+  return 0;
+}
+
+int hft_get_state_device(hft_ctx *c, double *dst, int64_t ld) {
+  if (!c || !dst) return fail(HFT_E_ARG, "null argument");
+  if (ld < c->nu) return fail(HFT_E_ARG, "hft_get_state_device: ld < %d", c->nu);
+  HIPCHK(hipMemcpy2DAsync(dst, (size_t)ld * sizeof(double), c->f, c->pitch * sizeof(double),
+                          c->nu * sizeof(double), (size_t)c->nr * c->ntm, hipMemcpyDeviceToDevice,
+                          c->stream));
+  return 0;
+}
+
+int hft_step_adjoint(hft_ctx *c, double dtl, int strang, int subcycles, double *lam_dev,
+                     int64_t ld, double *grad, int flags) {
+  if (!c || !lam_dev) return fail(HFT_E_ARG, "null argument");
+  if (ld < c->nu) return fail(HFT_E_ARG, "hft_step_adjoint: ld < %d", c->nu);
+  if (subcycles < 0) return fail(HFT_E_ARG, "hft_step_adjoint: subcycles < 0");
+  bool adv = c->flow_set, dif = c->diff_set;
+  if (adv && (c->flow2d || c->atten))
+    return fail(HFT_E_STATE, "hft_step_adjoint: attenuated (state-dependent) flows have no adjoint");
+  if (adv && c->nu > 1024)
+    return fail(HFT_E_ARG, "hft_step_adjoint: more than 1024 unique phi columns");
+  bool frozen = (flags & HFT_ADJ_FROZEN_WENO) != 0;
+  int nr = c->nr, rc;
+  // partial sums
+  int ctiles = (c->nu + kBlock - 1) / kBlock;
+  size_t nD = (size_t)nr * c->ntm * ctiles;
+  int nbands = (c->ntm + c->adj_band_rows - 1) / c->adj_band_rows;
+  size_t nA = (size_t)nr * nbands * 2;
+  if (!c->d_apD) {
+    if ((rc = dalloc(&c->d_apD, nD)) || (rc = dalloc(&c->d_gout, (size_t)nr * 3))) return rc;
+  }
+  if (nA > c->apA_cap) {
+    if (c->d_apA) hipFree(c->d_apA);
+    c->d_apA = nullptr;
+    if ((rc = dalloc(&c->d_apA, nA))) return rc;
+    c->apA_cap = nA;
+  }
+  if ((rc = fill_dev(c, c->d_apD, nD, 0.0)) || (rc = fill_dev(c, c->d_apA, nA, 0.0)) ||
+      (rc = fill_dev(c, c->d_gout, (size_t)nr * 3, 0.0)))
+    return rc;
+  double *LAM = adj_buf(c, c->adj_fix, AB_LAM, &rc);
+  if (!LAM) return rc;
+  HIPCHK(hipMemcpy2DAsync(LAM, c->pitch * sizeof(double), lam_dev, (size_t)ld * sizeof(double),
+                          c->nu * sizeof(double), (size_t)nr * c->ntm, hipMemcpyDeviceToDevice,
+                          c->stream));
+  double *XA[4], *XB[4];
+  for (int i = 0; i < 4; i++) {
+    if (!(XA[i] = adj_buf(c, c->adj_fix, AB_XA + i, &rc))) return rc;
+    if (strang && !(XB[i] = adj_buf(c, c->adj_fix, AB_XB + i, &rc))) return rc;
+  }
+  // forward recomputation with storage
+  std::vector<CycleRec> recs;
+  double adt = strang ? dtl * half : dtl;
+  if (adv && (rc = advection_store(c, adt, XA))) return rc;
+  if (dif) {
+    std::function<int(int)> pre = [&](int round) -> int {
+      int rc2;
+      double *u = adj_buf(c, c->adj_uc, (size_t)(round - 1), &rc2);
+      if (!u) return rc2;
+      HIPCHK(hipMemcpyAsync(u, c->f, c->nfield * sizeof(double), hipMemcpyDeviceToDevice,
+                            c->stream));
+      return 0;
+    };
+    struct Rec {
+      std::vector<CycleRec> *recs;
+      int nr;
+    } rr{&recs, nr};
+    auto cb = [](void *user, int round, const hft_cycle *rec) {
+      Rec *q = (Rec *)user;
+      CycleRec cr;
+      cr.dt.assign(q->nr, 0.0);
+      cr.s.assign(q->nr, 0);
+      for (int r = 0; r < q->nr; r++)
+        if (rec[r].dt > 0.0) {
+          cr.reals.push_back(r);
+          cr.dt[r] = rec[r].dt;
+          cr.s[r] = rec[r].sts_s;
+        }
+      (void)round;
+      q->recs->push_back(cr);
+    };
+    if ((rc = diffusion_impl(c, dtl, subcycles, cb, &rr, &pre))) return rc;
+  }
+  if (strang && adv && (rc = advection_store(c, adt, XB))) return rc;
+  // backward sweeps
+  if (strang && adv && (rc = advection_adjoint(c, adt, XB, LAM, frozen))) return rc;
+  if (dif) {
+    for (int q = (int)recs.size() - 1; q >= 0; q--) {
+      if (recs[q].reals.empty()) continue;
+      if ((rc = cycle_adjoint(c, recs[q], c->adj_uc[q], LAM))) return rc;
+    }
+  }
+  if (adv && (rc = advection_adjoint(c, adt, XA, LAM, frozen))) return rc;
+  HIPCHK(hipMemcpy2DAsync(lam_dev, (size_t)ld * sizeof(double), LAM, c->pitch * sizeof(double),
+                          c->nu * sizeof(double), (size_t)nr * c->ntm, hipMemcpyDeviceToDevice,
+                          c->stream));
+  // parameter sensitivities
+  hipLaunchKernelGGL(k_sum_parts, dim3(nr), dim3(256), 0, c->stream, (const double *)c->d_apD,
+                     c->ntm * ctiles, c->ntm * ctiles, 0, 1, c->d_gout, 3, 0);
+  hipLaunchKernelGGL(k_sum_parts, dim3(nr), dim3(256), 0, c->stream, (const double *)c->d_apA,
+                     nbands, nbands * 2, 0, 2, c->d_gout, 3, 1);
+  hipLaunchKernelGGL(k_sum_parts, dim3(nr), dim3(256), 0, c->stream, (const double *)c->d_apA,
+                     nbands, nbands * 2, 1, 2, c->d_gout, 3, 2);
+  HIPCHK(hipGetLastError());
+  if (grad) {
+    std::vector<double> g((size_t)nr * 3);
+    HIPCHK(hipMemcpyAsync(g.data(), c->d_gout, g.size() * sizeof(double), hipMemcpyDeviceToHost,
+                          c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (size_t i = 0; i < g.size(); i++) grad[i] += g[i];
+  }
+  return 0;
+}
+
+}  // extern "C"

@@ -7,11 +7,13 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <string>
 #include <vector>
 
 #include "../../include/hft.h"
 #include "hft_kernels.hip"
+#include "hft_adjoint_kernels.hip"
 
 using namespace hft;
 
@@ -81,9 +83,11 @@ struct TimedPair {
 };
 
 // Kernel timing groups; launches timed under "rkg2_fused" also count in "rkg2".
-const char *kGroups[] = {"rkg2", "weno", "ptl", "analysis", "assim", "flow", "rkg2_fused"};
-constexpr int kNGroups = 7;
+const char *kGroups[] = {"rkg2", "weno", "ptl", "analysis", "assim", "flow", "rkg2_fused",
+                         "adjoint"};
+constexpr int kNGroups = 8;
 constexpr int kGroupFused = 6;
+constexpr int kGroupAdjoint = 7;
 
 }  // namespace
 
@@ -147,6 +151,13 @@ struct hft_ctx {
   size_t map_cap = 0;
   double *d_apar = nullptr;
   int *d_ipar = nullptr;
+  // adjoint (hft_step_adjoint): field buffers allocated on first use
+  std::vector<double *> adj_fix, adj_uc, adj_ck, adj_w;
+  double *d_apD = nullptr, *d_apA = nullptr, *d_gout = nullptr;
+  size_t apA_cap = 0;
+  int adj_band_rows = 16, adj_seg = 0;
+  int64_t adj_mem_mb = 32768;
+  hipStream_t own_stream = nullptr;  // the context's stream when a caller's is in use
   // timing
   bool timing = false;
   std::vector<TimedPair> pending;
@@ -597,16 +608,14 @@ struct CoefCache {
   std::vector<double> u, v, ub, g;
 };
 
-// One RKG2 super-step for the listed realizations, each with its own cycle
-// time step dt2[r] and stage count s[r] (diffusion_step_sts_cd,
-// src/hipft.f90:5163-5242).  use_af: dty0 already holds L(f) from the PTL
-// pass (same operator, so identical to recomputing it).
-int rkg2_superstep(hft_ctx *c, const std::vector<int> &reals, const std::vector<double> &dt2,
-                   const std::vector<int64_t> &s, bool use_af) {
+// Stage tables of a batched RKG2 super-step: realizations sorted by stage
+// count (descending) into ord / d_order, coefficients of stage k of
+// realization r at d_coef[k*nr + r], s, cycle dt and ubj(1) per realization.
+int upload_rkg2(hft_ctx *c, const std::vector<int> &reals, const std::vector<double> &dt2,
+                const std::vector<int64_t> &s, std::vector<int> &ord) {
   int n = (int)reals.size();
-  if (n == 0) return 0;
   int nr = c->nr;
-  std::vector<int> ord(reals);
+  ord = reals;
   std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return s[a] > s[b]; });
   int64_t smax = s[ord[0]];
   size_t need = (size_t)(smax + 1) * nr;
@@ -655,7 +664,22 @@ int rkg2_superstep(hft_ctx *c, const std::vector<int> &reals, const std::vector<
   HIPCHK(hipMemcpyAsync(c->d_cdt, c->h_smalld, nr * sizeof(double), hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(c->d_ub1, c->h_smalld + nr, nr * sizeof(double), hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipEventRecord(c->coef_ev, c->stream));
+  return 0;
+}
 
+// One RKG2 super-step for the listed realizations, each with its own cycle
+// time step dt2[r] and stage count s[r] (diffusion_step_sts_cd,
+// src/hipft.f90:5163-5242).  use_af: dty0 already holds L(f) from the PTL
+// pass (same operator, so identical to recomputing it).
+int rkg2_superstep(hft_ctx *c, const std::vector<int> &reals, const std::vector<double> &dt2,
+                   const std::vector<int64_t> &s, bool use_af) {
+  int n = (int)reals.size();
+  if (n == 0) return 0;
+  int nr = c->nr;
+  std::vector<int> ord;
+  int rc0 = upload_rkg2(c, reals, dt2, s, ord);
+  if (rc0) return rc0;
+  int64_t smax = s[ord[0]];
   Stage1Args a1;
   a1.m = c->m; a1.g = tiles_for(c, n); a1.f = c->f; a1.dty0 = c->dty0; a1.u1 = c->buf1;
   a1.D = c->d_D; a1.dt = c->d_cdt; a1.ub1 = c->d_ub1; a1.order = c->d_order;
@@ -814,6 +838,8 @@ int ptl_pass(hft_ctx *c, const std::vector<int> &reals, bool coupled,
 // ============================================================================
 // C ABI
 // ============================================================================
+void set_adjoint_attributes();
+
 extern "C" {
 
 const char *hft_last_error(void) { return g_err.c_str(); }
@@ -864,6 +890,7 @@ int hft_create(int ntm, int npm1, const double *t, const double *p, int nr, int 
     hipFuncSetAttribute((const void *)k_rkg2_fused<2>, hipFuncAttributeMaxDynamicSharedMemorySize, 2 * row);
     hipFuncSetAttribute((const void *)k_rkg2_fused<3>, hipFuncAttributeMaxDynamicSharedMemorySize, 3 * row);
     hipFuncSetAttribute((const void *)k_rkg2_fused<4>, hipFuncAttributeMaxDynamicSharedMemorySize, 4 * row);
+    set_adjoint_attributes();
   }
   *out = c;
   return 0;
@@ -890,7 +917,17 @@ int hft_destroy(hft_ctx *c) {
   }
   for (auto e : c->free_ev) hipEventDestroy(e);
   if (c->coef_ev) hipEventDestroy(c->coef_ev);
-  if (c->stream) hipStreamDestroy(c->stream);
+  for (auto *vec : {&c->adj_fix, &c->adj_uc, &c->adj_ck, &c->adj_w})
+    for (double *q : *vec)
+      if (q) hipFree(q);
+  for (double *q : {c->d_apD, c->d_apA, c->d_gout})
+    if (q) hipFree(q);
+  if (c->own_stream) {
+    hipStreamSynchronize(c->own_stream);
+    hipStreamDestroy(c->own_stream);  // c->stream belongs to the caller
+  } else if (c->stream) {
+    hipStreamDestroy(c->stream);
+  }
   delete c;
   return 0;
 }
@@ -1174,7 +1211,17 @@ int hft_advection_step(hft_ctx *c, double dtl) {
   return 0;
 }
 
+// Controller of hft_diffusion_step; pre(round), when given, runs before each
+// round of cycles (the adjoint stores the cycle inputs there).
+static int diffusion_impl(hft_ctx *c, double dtl, int subcycles, hft_cycle_fn cb, void *user,
+                          const std::function<int(int)> *pre);
+
 int hft_diffusion_step(hft_ctx *c, double dtl, int subcycles, hft_cycle_fn cb, void *user) {
+  return diffusion_impl(c, dtl, subcycles, cb, user, nullptr);
+}
+
+static int diffusion_impl(hft_ctx *c, double dtl, int subcycles, hft_cycle_fn cb, void *user,
+                          const std::function<int(int)> *pre) {
   if (!c) return fail(HFT_E_ARG, "null context");
   if (!c->diff_set) return fail(HFT_E_STATE, "hft_diffusion_step: diffusion not set");
   if (subcycles < 0) return fail(HFT_E_ARG, "hft_diffusion_step: subcycles < 0");
@@ -1198,6 +1245,10 @@ int hft_diffusion_step(hft_ctx *c, double dtl, int subcycles, hft_cycle_fn cb, v
     for (int g = 0; g < ng; g++)
       if (!done[g]) gs.push_back(g);
     if (gs.empty()) break;
+    if (pre) {
+      int rc = (*pre)(round);
+      if (rc) return rc;
+    }
     for (int g : gs) {
       if (coupled)
         for (int r = 0; r < nr; r++) reals.push_back(r);
@@ -1500,6 +1551,15 @@ int hft_set_tuning(hft_ctx *c, const char *key, int value) {
   } else if (!strcmp(key, "fused_target_blocks")) {
     if (value < 0) return fail(HFT_E_ARG, "fused_target_blocks must be >= 0");
     c->fused_target_blocks = value;
+  } else if (!strcmp(key, "adjoint_segment")) {
+    if (value < 0) return fail(HFT_E_ARG, "adjoint_segment must be >= 0");
+    c->adj_seg = value;
+  } else if (!strcmp(key, "adjoint_mem_mb")) {
+    if (value < 0) return fail(HFT_E_ARG, "adjoint_mem_mb must be >= 0");
+    c->adj_mem_mb = value;
+  } else if (!strcmp(key, "adjoint_band_rows")) {
+    if (value < 1 || value > 4096) return fail(HFT_E_ARG, "adjoint_band_rows must be 1..4096");
+    c->adj_band_rows = value;
   } else if (!strcmp(key, "advection_band")) {
     c->adv_band = value != 0;
   } else if (!strcmp(key, "advection_band_rows")) {
@@ -1518,6 +1578,9 @@ int hft_get_tuning(hft_ctx *c, const char *key, int *value) {
   else if (!strcmp(key, "band_rows")) *value = c->band_rows;
   else if (!strcmp(key, "fused_min_realizations")) *value = c->fused_min;
   else if (!strcmp(key, "fused_target_blocks")) *value = c->fused_target_blocks;
+  else if (!strcmp(key, "adjoint_segment")) *value = c->adj_seg;
+  else if (!strcmp(key, "adjoint_mem_mb")) *value = (int)c->adj_mem_mb;
+  else if (!strcmp(key, "adjoint_band_rows")) *value = c->adj_band_rows;
   else if (!strcmp(key, "advection_band")) *value = c->adv_band;
   else if (!strcmp(key, "advection_band_rows")) *value = c->adv_band_rows;
   else return fail(HFT_E_ARG, "unknown tuning key '%s'", key);
@@ -1553,3 +1616,5 @@ int hft_kernel_time(hft_ctx *c, const char *group, double *total_ms, int64_t *la
 }
 
 }  // extern "C"
+
+#include "hft_adjoint.hip"

@@ -4,7 +4,9 @@ The knobs (``hft_set_tuning``) are the stages per fused launch (temporal
 depth), the theta rows per block (tile height), the number of still-active
 realizations below which stages go one per launch, and the block count below
 which a fused launch thins its bands (the tail of a batch whose
-realizations have different stage counts).  Results are identical
+realizations have different stage counts).  The WENO3 stages have their own
+knobs (banded or one thread per point, rows per band), timed on advection
+steps.  Results are identical
 for every setting, so the tuner only measures: it times diffusion super-steps
 of the live batch for each candidate and keeps the fastest.  Choices are
 cached in ``tuning_cache.json`` beside this file, keyed by device name, grid
@@ -17,7 +19,17 @@ import time
 HERE = os.path.dirname(os.path.abspath(__file__))
 CACHE = os.path.join(HERE, "tuning_cache.json")
 
-KEYS = ("fused", "fused_stages", "band_rows", "fused_min_realizations", "fused_target_blocks")
+DIFF_KEYS = ("fused", "fused_stages", "band_rows", "fused_min_realizations",
+             "fused_target_blocks")
+ADV_KEYS = ("advection_band", "advection_band_rows")
+KEYS = DIFF_KEYS + ADV_KEYS
+DEFAULTS = dict(fused=1, fused_stages=4, band_rows=32, fused_min_realizations=8,
+                fused_target_blocks=0, advection_band=1, advection_band_rows=16)
+
+
+def adv_candidates():
+    return [dict(advection_band=0, advection_band_rows=16)] + \
+        [dict(advection_band=1, advection_band_rows=h) for h in (4, 8, 16, 32)]
 
 
 def candidates(nr):
@@ -58,7 +70,7 @@ def load_cache():
 
 
 def apply(model, cfg):
-    model.set_tuning(**{k: cfg.get(k, 0) for k in KEYS})
+    model.set_tuning(**{k: cfg.get(k, DEFAULTS[k]) for k in KEYS})
 
 
 def autotune(model, dt=1.0, reps=2, use_cache=True, write_cache=True, cands=None, log=None):
@@ -69,23 +81,33 @@ def autotune(model, dt=1.0, reps=2, use_cache=True, write_cache=True, cands=None
         apply(model, cache[key])
         return cache[key]
     state = model.get_state("tp")
-    best, best_t, results = None, float("inf"), []
-    for cfg in (cands or candidates(model.nr)):
-        apply(model, cfg)
-        model.set_state(state, layout="tp")
-        model.diffusion_step(dt, subcycles=1)  # warm
-        model.set_state(state, layout="tp")
-        model.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(reps):
-            model.diffusion_step(dt, subcycles=1)
-        model.synchronize()
-        el = (time.perf_counter() - t0) / reps
-        results.append((el, cfg))
-        if log:
-            log(f"tune {cfg} -> {1e3 * el:.2f} ms")
-        if el < best_t:
-            best, best_t = dict(cfg), el
+
+    def measure(cands, run, what):
+        best, best_t = None, float("inf")
+        for cfg in cands:
+            apply(model, cfg)
+            model.set_state(state, layout="tp")
+            run()  # warm
+            model.set_state(state, layout="tp")
+            model.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                run()
+            model.synchronize()
+            el = (time.perf_counter() - t0) / reps
+            if log:
+                log(f"tune {what} {cfg} -> {1e3 * el:.2f} ms")
+            if el < best_t:
+                best, best_t = dict(cfg), el
+        return best, best_t
+
+    best, best_t = measure([dict(DEFAULTS, **c) for c in (cands or candidates(model.nr))],
+                           lambda: model.diffusion_step(dt, subcycles=1), "diffusion")
+    if model.has_flow():
+        adv, adv_t = measure([dict(best, **c) for c in adv_candidates()],
+                             lambda: model.advection_step(0.5 * dt), "advection")
+        best = adv
+        best["ms_per_advection_step"] = round(1e3 * adv_t, 3)
     model.set_state(state, layout="tp")
     best["ms_per_superstep"] = round(1e3 * best_t, 3)
     apply(model, best)

@@ -15,6 +15,7 @@ module hft
   integer(c_int), parameter :: HFT_LAYOUT_TP = 0, HFT_LAYOUT_PT = 1
   integer(c_int), parameter :: HFT_MODE_COUPLED = 0, HFT_MODE_INDEPENDENT = 1
   integer(c_int), parameter :: HFT_NHIST = 14
+  integer(c_int), parameter :: HFT_ADJ_FROZEN_WENO = 1
 
   type, bind(c) :: hft_cycle
     real(c_double) :: time_stepped, dt
@@ -197,6 +198,31 @@ module hft
       character(kind=c_char), intent(in) :: group(*)
       real(c_double), intent(out) :: total_ms, alg_bytes
       integer(c_int64_t), intent(out) :: launches
+    end function
+    ! device-resident state and the discrete adjoint of one step
+    integer(c_int) function hft_set_state_device(ctx, src, ld) bind(c, name="hft_set_state_device")
+      import :: c_int, c_ptr, c_int64_t
+      type(c_ptr), value :: ctx, src
+      integer(c_int64_t), value :: ld
+    end function
+    integer(c_int) function hft_get_state_device(ctx, dst, ld) bind(c, name="hft_get_state_device")
+      import :: c_int, c_ptr, c_int64_t
+      type(c_ptr), value :: ctx, dst
+      integer(c_int64_t), value :: ld
+    end function
+    integer(c_int) function hft_set_stream(ctx, stream, own) bind(c, name="hft_set_stream")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, stream
+      integer(c_int), value :: own
+    end function
+    integer(c_int) function hft_step_adjoint(ctx, dt, strang, subcycles, lam, ld, grad, flags) &
+        bind(c, name="hft_step_adjoint")
+      import :: c_int, c_double, c_ptr, c_int64_t
+      type(c_ptr), value :: ctx, lam
+      real(c_double), value :: dt
+      integer(c_int), value :: strang, subcycles, flags
+      integer(c_int64_t), value :: ld
+      real(c_double), intent(inout) :: grad(*)
     end function
   end interface
 end module hft

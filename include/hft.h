@@ -190,6 +190,11 @@ int hft_stream(hft_ctx *ctx, void **stream);
  *   "advection_band"         1 = WENO3 stages by theta bands computing each
  *                            face once (default); 0 = one thread per point
  *   "advection_band_rows"    theta rows per block of the banded WENO3 (16)
+ *   "adjoint_segment"        RKG2 stages per recomputed segment of the
+ *                            adjoint (0 = all stages of a cycle when they fit
+ *                            in "adjoint_mem_mb", else sqrt(s))
+ *   "adjoint_mem_mb"         device memory for stored RKG2 stages (32768)
+ *   "adjoint_band_rows"      theta rows per block of the WENO3 adjoint (16)
  * Results do not depend on these settings (bit-identical). */
 int hft_set_tuning(hft_ctx *ctx, const char *key, int value);
 int hft_get_tuning(hft_ctx *ctx, const char *key, int *value);
@@ -197,13 +202,39 @@ int hft_get_tuning(hft_ctx *ctx, const char *key, int *value);
 /* Kernel timing with HIP events on the context's stream.  hft_set_timing
  * resets the counters and turns per-launch events on or off.
  * hft_kernel_time returns, for a kernel group ("rkg2", "weno", "ptl",
- * "analysis", "assim", "flow", or "rkg2_fused" = the multi-stage RKG2
- * launches, which "rkg2" includes) since the reset: total device time (ms), number of
+ * "analysis", "assim", "flow", "adjoint", or "rkg2_fused" = the multi-stage
+ * RKG2 launches, which "rkg2" includes) since the reset: total device time (ms), number of
  * launches, and the algorithmic bytes those launches move (compulsory
  * reads + writes of FP64 fields over the ntm x (npm-2) unique points). */
 int hft_set_timing(hft_ctx *ctx, int enable);
 int hft_kernel_time(hft_ctx *ctx, const char *group, double *total_ms, int64_t *launches,
                     double *alg_bytes);
+
+/* ---- differentiation (no counterpart in the reference) ---------------------
+ * Device-resident state for callers that keep maps in HBM (PyTorch): src/dst
+ * are device arrays [r][j][c] with row length ld >= npm-2 (unique phi
+ * columns only), copied on the context's stream. */
+int hft_set_state_device(hft_ctx *ctx, const double *src, int64_t ld);
+int hft_get_state_device(hft_ctx *ctx, double *dst, int64_t ld);
+/* own = 0: launch on the caller's HIP stream (hipStream_t as void*; NULL is
+ * the null stream); own = 1: on the context's own stream again. */
+int hft_set_stream(hft_ctx *ctx, void *stream, int own);
+
+/* Discrete adjoint of one hft_flux_transport_step(ctx, dt, strang,
+ * subcycles, ...) taken from the current state x_n.
+ *   lam  (device, layout as hft_set_state_device) in: dJ/dx_{n+1};
+ *        out: dJ/dx_n
+ *   grad (host [nr][3], accumulated +=): dJ/dlog of the per-realization
+ *        diffusivity (eta), of a common scale of the theta velocity vt and
+ *        of a common scale of the phi velocity vp (alpha scales with them).
+ * The step's discrete choices (sub-cycle time steps, RKG2 stage counts,
+ * upwind signs) are those of the recomputed forward step and are held
+ * fixed.  On return the state is x_{n+1}.  Flows must not be attenuated.
+ * flags: HFT_ADJ_FROZEN_WENO linearises WENO3 with frozen nonlinear weights
+ * (otherwise the exact Jacobian of the weights is used). */
+#define HFT_ADJ_FROZEN_WENO 1
+int hft_step_adjoint(hft_ctx *ctx, double dt, int strang, int subcycles, double *lam,
+                     int64_t ld, double *grad, int flags);
 
 #ifdef __cplusplus
 }

@@ -245,6 +245,33 @@ int orc_mesh(const orc *o, int which, double *out) {
   return len[which];
 }
 
+int orc_weno(const orc *o, int which, double *out) {
+  const double *src[] = {o->D_C_CPt, o->D_C_MCt, o->D_M_Tt, o->D_CP_Tt, o->D_P_Tt, o->D_MC_Tt,
+                         o->eps_t, o->D_C_CPp, o->D_C_MCp, o->D_M_Tp, o->D_CP_Tp, o->D_P_Tp,
+                         o->D_MC_Tp, o->eps_p};
+  if (which < 0 || which > 13) return -1;
+  int n = which < 7 ? o->nt : o->np;
+  memcpy(out, src[which] + 1, sizeof(double) * (size_t)n);
+  return n;
+}
+
+void orc_alpha(const orc *o, double *alpha_t, double *alpha_p) {
+  size_t n = (size_t)o->ntm * o->npm * o->nr;
+  memcpy(alpha_t, o->alpha_t, sizeof(double) * n);
+  memcpy(alpha_p, o->alpha_p, sizeof(double) * n);
+}
+
+void orc_coef(const orc *o, double *coef) {
+  memcpy(coef, o->coef, sizeof(double) * (size_t)(o->ntm - 2) * (o->npm - 2) * 5 * o->nr);
+}
+
+void orc_pole_fac(const orc *o, double *out4) {
+  out4[0] = o->bc_flow_npole_fac;
+  out4[1] = o->bc_flow_spole_fac;
+  out4[2] = o->bc_diff_npole_fac;
+  out4[3] = o->bc_diff_spole_fac;
+}
+
 void orc_set_f(orc *o, const double *f) {
   memcpy(o->f, f, sizeof(double) * (size_t)o->ntm * o->npm * o->nr);
 }

@@ -24,6 +24,16 @@ void orc_dims(const orc *o, int *out /* ntm npm nt np nr */);
 /* which: 0 t 1 p 2 th 3 ph 4 dt 5 dth 6 dp 7 dph 8 st 9 sth 10 ct 11 cth
  *        12 st_i 13 sth_i 14 dt_i 15 dth_i 16 dp_i 17 dph_i */
 int orc_mesh(const orc *o, int which, double *out);
+/* Operator data for other restatements of the step (the differentiable
+ * PyTorch transcription): WENO3 weights (which 0..6 theta CP MC M_T CP_T P_T
+ * MC_T eps, 7..13 phi; 1-based arrays shifted to 0-based, length nt or np),
+ * Lax-Friedrichs alphas (ntm,npm,nr), diffusion coefficients
+ * coef(2:ntm-1,2:npm-1,5,nr), and the pole factors (bc_flow n/s, bc_diff
+ * n/s). */
+int orc_weno(const orc *o, int which, double *out);
+void orc_alpha(const orc *o, double *alpha_t, double *alpha_p);
+void orc_coef(const orc *o, double *coef);
+void orc_pole_fac(const orc *o, double *out4);
 
 void orc_set_f(orc *o, const double *f);
 void orc_get_f(const orc *o, double *f);

@@ -47,6 +47,10 @@ def lib():
             getattr(L, name).argtypes = [ctypes.c_void_p]
         L.orc_dims.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]
         L.orc_mesh.argtypes = [ctypes.c_void_p, ctypes.c_int, _D]
+        L.orc_weno.argtypes = [ctypes.c_void_p, ctypes.c_int, _D]
+        L.orc_alpha.argtypes = [ctypes.c_void_p, _D, _D]
+        L.orc_coef.argtypes = [ctypes.c_void_p, _D]
+        L.orc_pole_fac.argtypes = [ctypes.c_void_p, _D]
         L.orc_set_f.argtypes = [ctypes.c_void_p, _D]
         L.orc_get_f.argtypes = [ctypes.c_void_p, _D]
         L.orc_ingest_pt.argtypes = [ctypes.c_void_p, _D, ctypes.c_double]
@@ -176,6 +180,31 @@ class Oracle:
         vp = np.empty((self.nr, self.np_, self.ntm))
         lib().orc_flow_get(self._h, _p(vt), _p(vp))
         return vt, vp
+
+    def weno_weights(self):
+        """WENO3 grid weights: dict theta/phi -> [CP, MC, M_T, CP_T, P_T, MC_T, eps]."""
+        out = {"theta": [], "phi": []}
+        for w in range(14):
+            buf = np.zeros(max(self.nt, self.np_) + 1)
+            n = lib().orc_weno(self._h, w, _p(buf))
+            out["theta" if w < 7 else "phi"].append(buf[:n].copy())
+        return out
+
+    def alphas(self):
+        at, ap = np.empty(self.shape), np.empty(self.shape)
+        lib().orc_alpha(self._h, _p(at), _p(ap))
+        return at, ap
+
+    def diffusion_coef(self):
+        """coef(2:ntm-1, 2:npm-1, 5, nr) as a C-order array (nr, 5, npm-2, ntm-2)."""
+        c = np.empty((self.nr, 5, self.npm - 2, self.ntm - 2))
+        lib().orc_coef(self._h, _p(c))
+        return c
+
+    def pole_factors(self):
+        out = np.zeros(4)
+        lib().orc_pole_fac(self._h, _p(out))
+        return dict(flow_n=out[0], flow_s=out[1], diff_n=out[2], diff_s=out[3])
 
     def flow_attenuate(self, values):
         lib().orc_flow_attenuate(self._h, _p(_vec(values, self.nr)))
