@@ -22,6 +22,8 @@ Prints one JSON line.  Extra objects:
                   committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary
   cpu_baseline -- the CPU oracle (1 core) on a bounded sample of the workload
   single       -- the same step for one realization per launch (plain path)
+  adjoint      -- cost of the adjoint gradient relative to the forward run
+                  (256x512, 16 realizations, 200 steps, checkpoints every 20)
 """
 import argparse
 import json
@@ -106,6 +108,60 @@ def cpu_baseline(wl, budget_s=15.0, max_steps=20):
                        f"512x1024 workload on one host core, CPU oracle (C restatement)")
 
 
+def adjoint_bench(nr=16, ntm=256, npm1=512, nsteps=200, every=20, seed=77):
+    """Gradient of a final-map misfit (256x512, 16 realizations, 200 steps of
+    1 h, checkpoint every 20 steps) with respect to the initial maps and the
+    per-realization eta, meridional amplitude and rotation scale.  The target
+    is a forward run with eta x 1.2 plus N(0, 2 G) noise.  Returns wall times
+    of the plain forward run and of forward + backward."""
+    import torch
+    from hipft_amd import workloads as W
+    from hipft_amd.adjoint import FlowModel, run
+    t, p = W.uniform_theta(ntm), W.uniform_phi(npm1)
+    rng = np.random.default_rng(seed)
+    eta = torch.tensor(np.exp(rng.uniform(math.log(100.0), math.log(1000.0), nr)),
+                       dtype=torch.float64)
+    v0 = torch.tensor(rng.uniform(10.0, 20.0, nr), dtype=torch.float64)
+    om = torch.tensor(rng.uniform(0.9, 1.1, nr), dtype=torch.float64)
+    vt_unit = np.stack([W.meridional_profile(t, 1.0) for _ in range(nr)])
+    dr_unit = np.array([W.dr_coefficients(1.0) for _ in range(nr)])
+    model = FlowModel(t, p, nr, vt_unit=vt_unit, dr_unit=dr_unit)
+    maps = np.stack([W.bipole_map(t, p, 200, seed + 1 + r) for r in range(nr)])
+    model.ft.set_state(maps, layout="pt")
+    model.bind_stream()
+    f0 = model.get()
+    with torch.no_grad():
+        tgt = run(f0, eta * 1.2, v0, om, model, nsteps, checkpoint_every=nsteps)
+        tgt = tgt + torch.as_tensor(rng.normal(0.0, 2.0, tuple(tgt.shape)), device="cuda")
+        run(f0, eta, v0, om, model, 2, checkpoint_every=nsteps)  # warm
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        run(f0, eta, v0, om, model, nsteps, checkpoint_every=nsteps)
+        torch.cuda.synchronize()
+        fwd = time.perf_counter() - t0
+    x = f0.clone().requires_grad_(True)
+    e, v, o = (a.clone().requires_grad_(True) for a in (eta, v0, om))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    y = run(x, e, v, o, model, nsteps, checkpoint_every=every)
+    loss = 0.5 * ((y - tgt) ** 2).sum()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    loss.backward()
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    s_r = model.ft.diffusion_stats()[0]
+    model.ft.close()
+    return dict(config=f"{ntm}x{npm1} uniform, {nr} realizations, {nsteps} steps of 1 h, "
+                       f"checkpoint every {every}, RKG2 stages {int(s_r.min())}-{int(s_r.max())}",
+                forward_s=round(fwd, 4), forward_ckpt_s=round(t1 - t0, 4),
+                backward_s=round(t2 - t1, 4),
+                adjoint_over_forward=round((t2 - t1) / fwd, 3),
+                gradient_over_forward=round((t2 - t0) / fwd, 3),
+                grad_norms=dict(f0=float(x.grad.norm()), eta=float(e.grad.norm()),
+                                v0=float(v.grad.norm()), omega=float(o.grad.norm())))
+
+
 def pmc_probe(m, nbytes=256 << 20):
     """Counter calibration copy of a known size, then one workload step."""
     import ctypes
@@ -146,6 +202,9 @@ def main():
     ap.add_argument("--nr", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-single", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="only the adjoint cost measurement (for profiling)")
     ap.add_argument("--retune", action="store_true", help="ignore the cached launch tuning")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="for rocprofv3 --pmc runs: a 256 MiB device-to-device copy (counter "
@@ -156,6 +215,9 @@ def main():
                                    "(realizations would shard across GPUs with no per-step "
                                    "communication in independent mode)"}))
         sys.exit(2)
+    if args.adjoint_only:
+        print(json.dumps({"adjoint": adjoint_bench()}), flush=True)
+        return
     nr = args.nr
     wl = make_workload(nr)
     m = setup_model(wl)
@@ -224,6 +286,9 @@ def main():
                              ms_per_step=round(1e3 * per_real, 3),
                              batched_speedup_per_realization=round(value * per_real, 2))
         m1.close()
+    if not args.no_adjoint:
+        m.close()
+        out["adjoint"] = adjoint_bench()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(wl)
     print(json.dumps(out), flush=True)
