@@ -4,13 +4,15 @@
 //
 // hft_step_adjoint recomputes the step from the context's state, storing what
 // the transposed sweeps need (the four SSPRK(4,3) stage inputs of each
-// advection step, the input of every diffusion cycle and, per cycle, RKG2
-// stage checkpoints), then runs the sweeps backwards:
+// advection step, the input of every diffusion round and the RKG2 stages),
+// then runs the sweeps backwards:
 //   Strang:  lam <- A'^T lam, D'^T lam, A'^T lam
-// The RKG2 recurrence is transposed stage by stage (k = s..1); its stage
-// states come from segment recomputation: pairs (u_{k0-1}, u_{k0-2}) are kept
-// at segment starts and each segment is recomputed before its adjoint stages
-// (a single segment holding every stage when memory allows).
+// The RKG2 recurrence is transposed stage by stage (k = s..1).  With fixed
+// sub-cycles whose stages fit in "adjoint_mem_mb" the recomputation keeps
+// every stage (one-stage launches); otherwise (PTL sub-cycling, or
+// "adjoint_segment" > 0) it keeps each round's input and, in the backward
+// sweep, pairs (u_{k0-1}, u_{k0-2}) at segment starts, recomputing each
+// segment before its adjoint stages.
 
 namespace {
 
@@ -111,10 +113,51 @@ struct CycleRec {
   std::vector<int> reals;
   std::vector<double> dt;     // [nr]
   std::vector<int64_t> s;     // [nr]
+  std::vector<double *> W;    // stored stages W[k] = u_k (k >= 1); empty: recompute
 };
 
+// Forward RKG2 super-step that keeps every stage u_1..u_{s-1} of the round
+// in rec.W (the realizations' final stages go into the state as usual).
+int superstep_store(hft_ctx *c, const std::vector<int> &reals, const std::vector<double> &dt2,
+                    const std::vector<int64_t> &s, bool use_af, CycleRec &rec, size_t &next) {
+  int nr = c->nr, rc;
+  std::vector<int> ord;
+  if ((rc = upload_rkg2(c, reals, dt2, s, ord))) return rc;
+  int n = (int)ord.size();
+  int64_t smax = s[ord[0]];
+  double *DTY = adj_buf(c, c->adj_fix, AB_DTY, &rc);
+  if (!DTY) return rc;
+  rec.W.assign(smax, nullptr);
+  for (int64_t k = 1; k < smax; k++) {
+    rec.W[k] = adj_buf(c, c->adj_st, next++, &rc);
+    if (!rec.W[k]) return rc;
+  }
+  Stage1Args a1;
+  a1.m = c->m; a1.g = tiles_for(c, n); a1.f = c->f; a1.dty0 = DTY; a1.u1 = rec.W[1];
+  a1.D = c->d_D; a1.dt = c->d_cdt; a1.ub1 = c->d_ub1; a1.order = c->d_order;
+  a1.Af = use_af ? c->dty0 : nullptr; a1.rstride = c->rstride;
+  if ((rc = launch(c, 0, k_rkg2_stage1, nblocks(a1.g), a1, 3.0 * n))) return rc;
+  StageArgs sa;
+  sa.m = c->m; sa.f = c->f; sa.dty0 = DTY; sa.D = c->d_D; sa.s = c->d_s; sa.order = c->d_order;
+  sa.rstride = c->rstride;
+  int active = n;
+  for (int64_t k = 2; k <= smax; k++) {
+    while (active > 0 && s[ord[active - 1]] < k) active--;
+    sa.g = tiles_for(c, active);
+    sa.k = (int)k;
+    sa.x = rec.W[k - 1];
+    sa.ukm2 = k == 2 ? c->f : rec.W[k - 2];
+    sa.out = k < smax ? rec.W[k] : c->buf0;  // stage smax finishes every active realization
+    sa.coef = c->d_coef + (size_t)k * nr;
+    if ((rc = launch(c, 0, k_rkg2_stage, nblocks(sa.g), sa, 5.0 * active))) return rc;
+  }
+  return 0;
+}
+
 // Adjoint of one round of RKG2 cycles from its input U0 (realizations of
-// the round only; the others pass through unchanged).
+// the round only; the others pass through unchanged).  Stage states come
+// from rec.W when the forward recomputation kept them, else from segment
+// recomputation.
 int cycle_adjoint(hft_ctx *c, const CycleRec &rec, const double *U0, double *lam) {
   int nr = c->nr, rc;
   std::vector<int> ord;
@@ -127,8 +170,16 @@ int cycle_adjoint(hft_ctx *c, const CycleRec &rec, const double *U0, double *lam
     while (q < n && s[ord[q]] >= kmin) q++;
     return q;
   };
-  // segment length
-  int64_t K = c->adj_seg;
+  double *LB[3], *R[3];
+  for (int i = 0; i < 3; i++)
+    if (!(LB[i] = adj_buf(c, c->adj_fix, AB_LB + i, &rc))) return rc;
+  double *L0 = adj_buf(c, c->adj_fix, AB_L0, &rc);
+  if (!L0) return rc;
+  double *M = adj_buf(c, c->adj_fix, AB_M, &rc);
+  if (!M) return rc;
+  bool stored = !rec.W.empty();
+  // segment length (one segment covering every stage when they are stored)
+  int64_t K = stored ? smax - 1 : c->adj_seg;
   if (K <= 0) {
     double field_mb = (double)c->nfield * 8.0 / 1048576.0;
     double all_mb = (double)(smax + 24) * field_mb;
@@ -137,27 +188,8 @@ int cycle_adjoint(hft_ctx *c, const CycleRec &rec, const double *U0, double *lam
   }
   if (K < 1) K = 1;
   int64_t nseg = (smax - 1 + K - 1) / K;
-  double *DTY = adj_buf(c, c->adj_fix, AB_DTY, &rc);
-  if (!DTY) return rc;
-  double *U1 = adj_buf(c, c->adj_fix, AB_U1, &rc);
-  if (!U1) return rc;
-  double *LB[3], *R[3];
-  for (int i = 0; i < 3; i++) {
-    if (!(LB[i] = adj_buf(c, c->adj_fix, AB_LB + i, &rc))) return rc;
-  }
-  double *L0 = adj_buf(c, c->adj_fix, AB_L0, &rc);
-  if (!L0) return rc;
-  double *M = adj_buf(c, c->adj_fix, AB_M, &rc);
-  if (!M) return rc;
-  // stage 1: dty0 = dt L u0, u1 = u0 + ub1 dty0
-  Stage1Args a1;
-  a1.m = c->m; a1.g = tiles_for(c, n); a1.f = U0; a1.dty0 = DTY; a1.u1 = U1;
-  a1.D = c->d_D; a1.dt = c->d_cdt; a1.ub1 = c->d_ub1; a1.order = c->d_order;
-  a1.Af = nullptr; a1.rstride = c->rstride;
-  if ((rc = launch(c, 0, k_rkg2_stage1, nblocks(a1.g), a1, 3.0 * n))) return rc;
+  std::vector<const double *> ck1(nseg), ck2(nseg);
   StageArgs sa;
-  sa.m = c->m; sa.f = const_cast<double *>(U0); sa.dty0 = DTY; sa.D = c->d_D; sa.s = c->d_s;
-  sa.order = c->d_order; sa.rstride = c->rstride;
   auto fwd_stage = [&](int64_t k, const double *x, const double *xm2, double *out) -> int {
     int na = nactive(k + 1);  // u_k is needed by realizations with s_r >= k+1
     if (na == 0) return 0;
@@ -169,43 +201,58 @@ int cycle_adjoint(hft_ctx *c, const CycleRec &rec, const double *U0, double *lam
     sa.coef = c->d_coef + (size_t)k * nr;
     return launch(c, 0, k_rkg2_stage, nblocks(sa.g), sa, 5.0 * na);
   };
-  // checkpoint pass: pairs (u_{k0-1}, u_{k0-2}) at segment starts k0 = 2 + i K
-  std::vector<const double *> ck1(nseg), ck2(nseg);
-  ck1[0] = U1;
   ck2[0] = U0;
-  if (nseg > 1) {
-    for (int i = 0; i < 3; i++)
-      if (!(R[i] = adj_buf(c, c->adj_fix, AB_R + i, &rc))) return rc;
-    const double *pm2 = U0, *pm1 = U1;
-    int ri = 0;
-    int64_t klast = 2 + (nseg - 1) * K;
-    for (int64_t k = 2; k <= klast - 1; k++) {
-      double *out = R[ri];
-      ri = (ri + 1) % 3;
-      if ((rc = fwd_stage(k, pm1, pm2, out))) return rc;
-      pm2 = pm1;
-      pm1 = out;
-      if ((k - 1) % K == 0) {  // k = k0 - 1 of segment i = (k-1)/K
-        int64_t i = (k - 1) / K;
-        double *b1 = adj_buf(c, c->adj_ck, 2 * i, &rc);
-        if (!b1) return rc;
-        double *b2 = adj_buf(c, c->adj_ck, 2 * i + 1, &rc);
-        if (!b2) return rc;
-        HIPCHK(hipMemcpyAsync(b1, pm1, c->nfield * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-        HIPCHK(hipMemcpyAsync(b2, pm2, c->nfield * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-        ck1[i] = b1;
-        ck2[i] = b2;
+  if (stored) {
+    ck1[0] = rec.W[1];
+  } else {
+    double *DTY = adj_buf(c, c->adj_fix, AB_DTY, &rc);
+    if (!DTY) return rc;
+    double *U1 = adj_buf(c, c->adj_fix, AB_U1, &rc);
+    if (!U1) return rc;
+    // stage 1: dty0 = dt L u0, u1 = u0 + ub1 dty0
+    Stage1Args a1;
+    a1.m = c->m; a1.g = tiles_for(c, n); a1.f = U0; a1.dty0 = DTY; a1.u1 = U1;
+    a1.D = c->d_D; a1.dt = c->d_cdt; a1.ub1 = c->d_ub1; a1.order = c->d_order;
+    a1.Af = nullptr; a1.rstride = c->rstride;
+    if ((rc = launch(c, 0, k_rkg2_stage1, nblocks(a1.g), a1, 3.0 * n))) return rc;
+    sa.m = c->m; sa.f = const_cast<double *>(U0); sa.dty0 = DTY; sa.D = c->d_D; sa.s = c->d_s;
+    sa.order = c->d_order; sa.rstride = c->rstride;
+    // checkpoint pass: pairs (u_{k0-1}, u_{k0-2}) at segment starts k0 = 2 + i K
+    ck1[0] = U1;
+    if (nseg > 1) {
+      for (int i = 0; i < 3; i++)
+        if (!(R[i] = adj_buf(c, c->adj_fix, AB_R + i, &rc))) return rc;
+      const double *pm2 = U0, *pm1 = U1;
+      int ri = 0;
+      int64_t klast = 2 + (nseg - 1) * K;
+      for (int64_t k = 2; k <= klast - 1; k++) {
+        double *out = R[ri];
+        ri = (ri + 1) % 3;
+        if ((rc = fwd_stage(k, pm1, pm2, out))) return rc;
+        pm2 = pm1;
+        pm1 = out;
+        if ((k - 1) % K == 0) {  // k = k0 - 1 of segment i = (k-1)/K
+          int64_t i = (k - 1) / K;
+          double *b1 = adj_buf(c, c->adj_ck, 2 * i, &rc);
+          if (!b1) return rc;
+          double *b2 = adj_buf(c, c->adj_ck, 2 * i + 1, &rc);
+          if (!b2) return rc;
+          HIPCHK(hipMemcpyAsync(b1, pm1, c->nfield * sizeof(double), hipMemcpyDeviceToDevice,
+                                c->stream));
+          HIPCHK(hipMemcpyAsync(b2, pm2, c->nfield * sizeof(double), hipMemcpyDeviceToDevice,
+                                c->stream));
+          ck1[i] = b1;
+          ck2[i] = b2;
+        }
       }
     }
   }
-  // adjoint initial values: lam_{s_r} = lam_r, partial lam_{s_r - 1} = 0
-  if ((rc = fill_dev(c, L0, c->nfield, 0.0))) return rc;
-  if ((rc = fill_dev(c, M, c->nfield, 0.0))) return rc;
-  for (int z = 0; z < n; z++) {
-    int r = ord[z];
-    if ((rc = copy_real(c, LB[s[r] % 3], lam, r))) return rc;
-    if ((rc = fill_dev(c, LB[(s[r] - 1) % 3] + (size_t)r * c->rstride, c->rstride, 0.0))) return rc;
-  }
+  // adjoint initial values: lam_{s_r} = lam_r, partial lam_{s_r - 1} = 0, L0 = M = 0
+  dim3 gi((unsigned)((c->rstride + kBlock - 1) / kBlock), (unsigned)n);
+  hipLaunchKernelGGL(k_adj_cycle_init, gi, dim3(kBlock), 0, c->stream, LB[0], LB[1], LB[2], L0,
+                     M, (const double *)lam, (const int *)c->d_order, (const int64_t *)c->d_s,
+                     c->rstride);
+  HIPCHK(hipGetLastError());
   AdjStageArgs ja;
   ja.m = c->m; ja.L0 = L0; ja.M = M; ja.D = c->d_D; ja.ub1 = c->d_ub1; ja.order = c->d_order;
   ja.rstride = c->rstride; ja.part = c->d_apD;
@@ -216,6 +263,10 @@ int cycle_adjoint(hft_ctx *c, const CycleRec &rec, const double *U0, double *lam
     W[0] = ck2[i];
     W[1] = ck1[i];
     for (int64_t k = k0; k <= k1 - 1; k++) {
+      if (stored) {
+        W[k - k0 + 2] = rec.W[k];
+        continue;
+      }
       double *out = adj_buf(c, c->adj_w, (size_t)(k - k0), &rc);
       if (!out) return rc;
       if ((rc = fwd_stage(k, W[k - k0 + 1], W[k - k0], out))) return rc;
@@ -238,8 +289,9 @@ int cycle_adjoint(hft_ctx *c, const CycleRec &rec, const double *U0, double *lam
   ca.m = c->m; ca.g = tiles_for(c, n); ca.M = M; ca.L0 = L0; ca.u0 = U0; ca.D = c->d_D;
   ca.dt = c->d_cdt; ca.order = c->d_order; ca.rstride = c->rstride; ca.part = c->d_apD;
   if ((rc = launch(c, kGroupAdjoint, k_rkg2_adj_close, nblocks(ca.g), ca, 4.0 * n))) return rc;
-  for (int z = 0; z < n; z++)
-    if ((rc = copy_real(c, lam, L0, ord[z]))) return rc;
+  hipLaunchKernelGGL(k_copy_reals, gi, dim3(kBlock), 0, c->stream, lam, (const double *)L0,
+                     (const int *)c->d_order, c->rstride);
+  HIPCHK(hipGetLastError());
   return 0;
 }
 
@@ -342,25 +394,36 @@ int hft_step_adjoint(hft_ctx *c, double dtl, int strang, int subcycles, double *
                             c->stream));
       return 0;
     };
-    struct Rec {
-      std::vector<CycleRec> *recs;
-      int nr;
-    } rr{&recs, nr};
-    auto cb = [](void *user, int round, const hft_cycle *rec) {
-      Rec *q = (Rec *)user;
+    // keep every RKG2 stage when the sub-cycles are fixed and they fit
+    // (adjoint_segment > 0 asks for segment recomputation instead)
+    bool store = false;
+    if (subcycles > 0 && c->adj_seg == 0) {
+      int64_t smax = 0;
+      for (int r = 0; r < nr; r++) {
+        double dte = c->mode == HFT_MODE_COUPLED ? c->dt_euler : c->dt_euler_r[r];
+        smax = std::max(smax, sts_stages(dtl / (double)subcycles, dte));
+      }
+      double field_mb = (double)c->nfield * 8.0 / 1048576.0;
+      store = (double)subcycles * (double)(smax + 2) * field_mb + 24.0 * field_mb <=
+              (double)c->adj_mem_mb;
+    }
+    size_t next = 0;
+    SuperstepFn ss = [&](const std::vector<int> &reals, const std::vector<double> &dt2,
+                         const std::vector<int64_t> &sv, bool use_af) -> int {
       CycleRec cr;
-      cr.dt.assign(q->nr, 0.0);
-      cr.s.assign(q->nr, 0);
-      for (int r = 0; r < q->nr; r++)
-        if (rec[r].dt > 0.0) {
-          cr.reals.push_back(r);
-          cr.dt[r] = rec[r].dt;
-          cr.s[r] = rec[r].sts_s;
-        }
-      (void)round;
-      q->recs->push_back(cr);
+      cr.reals = reals;
+      cr.dt.assign(nr, 0.0);
+      cr.s.assign(nr, 0);
+      for (int r : reals) {
+        cr.dt[r] = dt2[r];
+        cr.s[r] = sv[r];
+      }
+      int rc2 = store ? superstep_store(c, reals, dt2, sv, use_af, cr, next)
+                      : rkg2_superstep(c, reals, dt2, sv, use_af);
+      recs.push_back(std::move(cr));
+      return rc2;
     };
-    if ((rc = diffusion_impl(c, dtl, subcycles, cb, &rr, &pre))) return rc;
+    if ((rc = diffusion_impl(c, dtl, subcycles, nullptr, nullptr, &pre, &ss))) return rc;
   }
   if (strang && adv && (rc = advection_store(c, adt, XB))) return rc;
   // backward sweeps

@@ -15,7 +15,13 @@
 // (L = D_r * L1) and scales of the theta and phi velocities (the
 // Lax-Friedrichs alpha scales with them).  Discrete choices of the forward
 // run (stage counts, sub-cycle time steps, upwind signs) are held fixed.
+//
+// Unlike the forward kernels these need not round like the reference, so
+// they let the compiler contract multiply-adds (HFT_ADJ_FAST opens each body)
+// and use reciprocals.
 #include "hft_device.h"
+
+#define HFT_ADJ_FAST _Pragma("clang fp contract(fast)")
 
 namespace hft {
 
@@ -40,6 +46,7 @@ __device__ inline double block_sum(double *sh, double v) {
 //                y(ntm-1,:) = -bc_s sum_c 2D (x(ntm-1,c)-x(ntm-2,c)) dp(c).
 // ---------------------------------------------------------------------------
 __device__ inline void lap_c15(const Mesh &m, double D, int j, int c, double &c1, double &c5) {
+  HFT_ADJ_FAST
   double sti = m.st_i[j], dpi = m.dp_i[c + 1];
   c1 = D * m.dph_i[c + 1];
   c1 = c1 * dpi;
@@ -69,6 +76,7 @@ __device__ inline double lap_c4(const Mesh &m, double D, int j) {
 // (only read for rows 0,1 / ntm-2,ntm-1).
 __device__ inline double lap_T(const Mesh &m, double D, const double *lam, int j, int c,
                                double Ln, double Ls) {
+  HFT_ADJ_FAST
   const int ntm = m.ntm, nu = m.nu, pitch = m.pitch;
   const double *row = lam + (size_t)j * pitch;
   double y = 0.0;
@@ -134,6 +142,7 @@ struct AdjStageArgs {
 };
 
 __global__ __launch_bounds__(kBlock) void k_rkg2_adj_stage(AdjStageArgs a) {
+  HFT_ADJ_FAST
   __shared__ double sh[kBlock / 64];
   int t = xcd_remap(blockIdx.x, gridDim.x);
   int z, j, ct;
@@ -191,6 +200,7 @@ struct AdjCloseArgs {
 };
 
 __global__ __launch_bounds__(kBlock) void k_rkg2_adj_close(AdjCloseArgs a) {
+  HFT_ADJ_FAST
   __shared__ double sh[kBlock / 64];
   int t = xcd_remap(blockIdx.x, gridDim.x);
   int z, j, ct;
@@ -242,6 +252,7 @@ __global__ void k_sum_parts(const double *part, int n, int stride, int comp, int
 template <bool FROZEN>
 __device__ inline void weno_face_grad(double LNm2, double LNm1, double LN0, double LPm1, double LP0,
                                       double LPp1, const WenoW &w, double g[6]) {
+  HFT_ADJ_FAST
   const double one = 1.0, four = 4.0;
   double p0m = (one + w.mcm) * LNm1 - w.mcm * LNm2;
   double p1m = w.mc0 * LNm1 + w.cpm * LN0;
@@ -249,20 +260,20 @@ __device__ inline void weno_face_grad(double LNm2, double LNm1, double LN0, doub
   double p1p = w.cpm * LP0 + w.mc0 * LPm1;
   double t0 = w.mcm * (LNm1 - LNm2), t1 = w.cpm * (LN0 - LNm1);
   double t2 = w.cp0 * (LPp1 - LP0), t3 = w.mc0 * (LP0 - LPm1);
-  double e0 = w.em + four * (t0 * t0), e1 = w.em + four * (t1 * t1);
-  double e2 = w.e0 + four * (t2 * t2), e3 = w.e0 + four * (t3 * t3);
-  double w0m = w.ptm / (e0 * e0), w1m = w.mctm / (e1 * e1);
-  double w0p = w.mt0 / (e2 * e2), w1p = w.cpt0 / (e3 * e3);
-  double Sm = w0m + w1m, Sp = w0p + w1p;
-  double O0m = w0m / Sm, O1m = w1m / Sm, O0p = w0p / Sp, O1p = w1p / Sp;
+  double i0 = one / (w.em + four * (t0 * t0)), i1 = one / (w.em + four * (t1 * t1));
+  double i2 = one / (w.e0 + four * (t2 * t2)), i3 = one / (w.e0 + four * (t3 * t3));
+  double w0m = w.ptm * (i0 * i0), w1m = w.mctm * (i1 * i1);
+  double w0p = w.mt0 * (i2 * i2), w1p = w.cpt0 * (i3 * i3);
+  double iSm = one / (w0m + w1m), iSp = one / (w0p + w1p);
+  double O0m = w0m * iSm, O1m = w1m * iSm, O0p = w0p * iSp, O1p = w1p * iSp;
   double k0 = 0.0, k1 = 0.0, k2 = 0.0, k3 = 0.0;
   if (!FROZEN) {
     double um = O0m * p0m + O1m * p1m;
     double up = O0p * p0p + O1p * p1p;
-    k0 = (p0m - um) / Sm * (-16.0 * t0 * w0m / e0);
-    k1 = (p1m - um) / Sm * (-16.0 * t1 * w1m / e1);
-    k2 = (p0p - up) / Sp * (-16.0 * t2 * w0p / e2);
-    k3 = (p1p - up) / Sp * (-16.0 * t3 * w1p / e3);
+    k0 = (p0m - um) * (-16.0 * t0 * O0m * i0);
+    k1 = (p1m - um) * (-16.0 * t1 * O1m * i1);
+    k2 = (p0p - up) * (-16.0 * t2 * O0p * i2);
+    k3 = (p1p - up) * (-16.0 * t3 * O1p * i3);
   }
   // LN side: t0 = mcm (LNm1 - LNm2), t1 = cpm (LN0 - LNm1)
   g[0] = -w.mcm * (O0m + k0);
@@ -295,6 +306,7 @@ struct AdvVjpArgs {
 
 template <bool FROZEN, bool HAS_P>
 __global__ __launch_bounds__(1024) void k_advection_vjp_band(AdvVjpArgs a) {
+  HFT_ADJ_FAST
   extern __shared__ double sh[];  // 9 x ncol + 16
   const Mesh &m = a.m;
   const int ncol = a.ncol, nu = m.nu, ntm = m.ntm, nt = m.nt, pitch = m.pitch;
@@ -440,6 +452,33 @@ __global__ __launch_bounds__(1024) void k_advection_vjp_band(AdvVjpArgs a) {
     pp[0] = pp[0] - a.cdt * st;
     pp[1] = pp[1] - a.cdt * sp;
   }
+}
+
+// Start of a cycle adjoint for the realizations ord[0..n): lam_{s_r} = lam,
+// partial lam_{s_r - 1} = 0, adjoints of u0 and dty0 = 0.
+__global__ __launch_bounds__(kBlock) void k_adj_cycle_init(double *LB0, double *LB1, double *LB2,
+                                                           double *L0, double *M,
+                                                           const double *lam, const int *order,
+                                                           const int64_t *s, size_t rstride) {
+  size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= rstride) return;
+  int r = order[blockIdx.y];
+  size_t p = (size_t)r * rstride + i;
+  double *LB[3] = {LB0, LB1, LB2};
+  int64_t sr = s[r];
+  LB[sr % 3][p] = lam[p];
+  LB[(sr - 1) % 3][p] = 0.0;
+  L0[p] = 0.0;
+  M[p] = 0.0;
+}
+
+// dst = src for the realizations ord[0..n).
+__global__ __launch_bounds__(kBlock) void k_copy_reals(double *dst, const double *src,
+                                                       const int *order, size_t rstride) {
+  size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= rstride) return;
+  size_t p = (size_t)order[blockIdx.y] * rstride + i;
+  dst[p] = src[p];
 }
 
 // Pointwise helpers of the adjoint controller.

@@ -152,7 +152,7 @@ struct hft_ctx {
   double *d_apar = nullptr;
   int *d_ipar = nullptr;
   // adjoint (hft_step_adjoint): field buffers allocated on first use
-  std::vector<double *> adj_fix, adj_uc, adj_ck, adj_w;
+  std::vector<double *> adj_fix, adj_uc, adj_ck, adj_w, adj_st;
   double *d_apD = nullptr, *d_apA = nullptr, *d_gout = nullptr;
   size_t apA_cap = 0;
   int adj_band_rows = 16, adj_seg = 0;
@@ -917,7 +917,7 @@ int hft_destroy(hft_ctx *c) {
   }
   for (auto e : c->free_ev) hipEventDestroy(e);
   if (c->coef_ev) hipEventDestroy(c->coef_ev);
-  for (auto *vec : {&c->adj_fix, &c->adj_uc, &c->adj_ck, &c->adj_w})
+  for (auto *vec : {&c->adj_fix, &c->adj_uc, &c->adj_ck, &c->adj_w, &c->adj_st})
     for (double *q : *vec)
       if (q) hipFree(q);
   for (double *q : {c->d_apD, c->d_apA, c->d_gout})
@@ -1213,15 +1213,21 @@ int hft_advection_step(hft_ctx *c, double dtl) {
 
 // Controller of hft_diffusion_step; pre(round), when given, runs before each
 // round of cycles (the adjoint stores the cycle inputs there).
+// Controller of hft_diffusion_step.  Hooks for the adjoint: pre(round) runs
+// before each round of cycles; superstep, when given, replaces the batched
+// RKG2 super-step (same arguments as rkg2_superstep).
+typedef std::function<int(const std::vector<int> &, const std::vector<double> &,
+                          const std::vector<int64_t> &, bool)>
+    SuperstepFn;
 static int diffusion_impl(hft_ctx *c, double dtl, int subcycles, hft_cycle_fn cb, void *user,
-                          const std::function<int(int)> *pre);
+                          const std::function<int(int)> *pre, const SuperstepFn *superstep);
 
 int hft_diffusion_step(hft_ctx *c, double dtl, int subcycles, hft_cycle_fn cb, void *user) {
-  return diffusion_impl(c, dtl, subcycles, cb, user, nullptr);
+  return diffusion_impl(c, dtl, subcycles, cb, user, nullptr, nullptr);
 }
 
 static int diffusion_impl(hft_ctx *c, double dtl, int subcycles, hft_cycle_fn cb, void *user,
-                          const std::function<int(int)> *pre) {
+                          const std::function<int(int)> *pre, const SuperstepFn *superstep) {
   if (!c) return fail(HFT_E_ARG, "null context");
   if (!c->diff_set) return fail(HFT_E_STATE, "hft_diffusion_step: diffusion not set");
   if (subcycles < 0) return fail(HFT_E_ARG, "hft_diffusion_step: subcycles < 0");
@@ -1288,7 +1294,8 @@ static int diffusion_impl(hft_ctx *c, double dtl, int subcycles, hft_cycle_fn cb
         dt2r[g] = dt2[g];
       }
     }
-    int rc = rkg2_superstep(c, reals, dt2r, sr, auto_sc);
+    int rc = superstep ? (*superstep)(reals, dt2r, sr, auto_sc)
+                       : rkg2_superstep(c, reals, dt2r, sr, auto_sc);
     if (rc) return rc;
     for (auto &rr : recs) rr = hft_cycle{0, 0, 0, 0};
     for (int g : gs) {
