@@ -6,7 +6,8 @@ phi columns) by ``nsteps`` flux-transport steps on the GPU and returns the
 final maps.  Its backward is the hand-written discrete adjoint of libhft
 (``hft_step_adjoint``) with step checkpointing: the forward keeps the state
 every ``checkpoint_every`` steps; the backward recomputes each segment from
-its checkpoint and runs the step adjoints in reverse.
+its checkpoint (one forward pass keeping the stage data, when it fits in
+the library's adjoint memory budget) and runs the step adjoints in reverse.
 
 Per-realization parameters (tensors [nr], float64):
   eta    -- diffusivity (km^2/s),
@@ -29,8 +30,9 @@ class FlowModel:
     with no other flow terms, so that d/dlog(v0) and d/dlog(omega) are the
     library's theta- and phi-velocity scale sensitivities."""
 
-    def __init__(self, t, p, nr, vt_unit=None, dr_unit=None, mode="independent"):
+    def __init__(self, t, p, nr, vt_unit=None, dr_unit=None, mode="independent", autotune=True):
         self.ft = FluxTransport(t, p, nr, mode=mode)
+        self.autotune = autotune
         self.nr = nr
         self.nu = self.ft.npm - 2
         self.ntm = self.ft.ntm
@@ -53,6 +55,9 @@ class FlowModel:
                 ft.set_flow_analytic(p0=d[:, 0], p2=d[:, 1], p4=d[:, 2])
             elif self.vt_unit is not None:
                 ft.set_flow_analytic()
+        if self.autotune and self._params is None:
+            from .tune import autotune
+            autotune(ft)  # launch shapes for this grid and batch (cached)
         self._params = key
 
     def bind_stream(self):
@@ -104,15 +109,9 @@ class FluxTransportRun(torch.autograd.Function):
         for si in reversed(range(len(starts))):
             n0 = starts[si]
             n1 = starts[si + 1] if si + 1 < len(starts) else nsteps
-            states = [ctx.ckpts[n0]]
-            model.put(states[0])
-            for _ in range(n0 + 1, n1):
-                model.ft.flux_transport_step(dt, strang=strang, subcycles=subcycles)
-                states.append(model.get())
-            for n in reversed(range(n0, n1)):
-                model.put(states[n - n0])
-                grads += model.ft.step_adjoint(dt, lam.data_ptr(), model.nu, strang=strang,
-                                               subcycles=subcycles, frozen_weno=frozen)
+            model.put(ctx.ckpts[n0])
+            grads += model.ft.steps_adjoint(n1 - n0, dt, lam.data_ptr(), model.nu, strang=strang,
+                                            subcycles=subcycles, frozen_weno=frozen)
         dev = eta.device
         t = lambda a: torch.as_tensor(a, dtype=torch.float64, device=dev)
         ge = t(grads[:, 0]) / eta.detach()

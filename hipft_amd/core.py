@@ -89,6 +89,9 @@ _SIGS = [
     ("hft_step_adjoint", ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_int,
                                         ctypes.c_int, ctypes.c_void_p, ctypes.c_int64, _D,
                                         ctypes.c_int]),
+    ("hft_steps_adjoint", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_double,
+                                         ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                         ctypes.c_int64, _D, ctypes.c_int]),
 ]
 
 ADJ_FROZEN_WENO = 1
@@ -385,6 +388,17 @@ class FluxTransport:
         _check(lib().hft_step_adjoint(self._h, float(dt), int(bool(strang)), int(subcycles),
                                       ctypes.c_void_p(lam_ptr), int(ld), _p(g),
                                       ADJ_FROZEN_WENO if frozen_weno else 0), "hft_step_adjoint")
+        return g
+
+    def steps_adjoint(self, nsteps, dt, lam_ptr, ld, strang=True, subcycles=1,
+                      frozen_weno=False):
+        """Adjoint of nsteps steps from the current state (which ends at the
+        state after them); lam in place, returns [nr][3] as step_adjoint."""
+        g = np.zeros((self.nr, 3))
+        _check(lib().hft_steps_adjoint(self._h, int(nsteps), float(dt), int(bool(strang)),
+                                       int(subcycles), ctypes.c_void_p(lam_ptr), int(ld), _p(g),
+                                       ADJ_FROZEN_WENO if frozen_weno else 0),
+               "hft_steps_adjoint")
         return g
 
     def kernel_time(self, group):

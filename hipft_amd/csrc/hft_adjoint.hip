@@ -295,6 +295,166 @@ int cycle_adjoint(hft_ctx *c, const CycleRec &rec, const double *U0, double *lam
   return 0;
 }
 
+// What one step's forward recomputation keeps for its adjoint.
+struct StepStore {
+  double *XA[4] = {nullptr, nullptr, nullptr, nullptr};
+  double *XB[4] = {nullptr, nullptr, nullptr, nullptr};
+  std::vector<CycleRec> recs;
+  std::vector<double *> UC;  // input of every diffusion round
+};
+
+// Fixed sub-cycles: the largest RKG2 stage count of a cycle (0 for PTL).
+int64_t fixed_smax(hft_ctx *c, double dtl, int subcycles) {
+  if (subcycles <= 0 || !c->diff_set) return 0;
+  int64_t smax = 0;
+  for (int r = 0; r < c->nr; r++) {
+    double dte = c->mode == HFT_MODE_COUPLED ? c->dt_euler : c->dt_euler_r[r];
+    smax = std::max(smax, sts_stages(dtl / (double)subcycles, dte));
+  }
+  return smax;
+}
+
+// Fields one step keeps when its RKG2 stages are stored.
+double step_fields(hft_ctx *c, double dtl, int strang, int subcycles) {
+  double f = c->flow_set ? (strang ? 8.0 : 4.0) : 0.0;
+  if (c->diff_set) f += (double)subcycles * (double)(fixed_smax(c, dtl, subcycles) + 2);
+  return f;
+}
+
+// Forward step from the state, keeping in st (fields drawn from c->adj_st
+// starting at next) what backward_step needs.  store_stages: keep the RKG2
+// stages (fixed sub-cycles only), else only the round inputs.
+int forward_store_step(hft_ctx *c, double dtl, int strang, int subcycles, StepStore &st,
+                       size_t &next, bool store_stages) {
+  int rc;
+  int nr = c->nr;
+  bool adv = c->flow_set, dif = c->diff_set;
+  double adt = strang ? dtl * half : dtl;
+  if (adv) {
+    for (int i = 0; i < 4; i++)
+      if (!(st.XA[i] = adj_buf(c, c->adj_st, next++, &rc))) return rc;
+    if ((rc = advection_store(c, adt, st.XA))) return rc;
+  }
+  if (dif) {
+    std::function<int(int)> pre = [&](int round) -> int {
+      (void)round;
+      int rc2;
+      double *u = adj_buf(c, c->adj_st, next++, &rc2);
+      if (!u) return rc2;
+      st.UC.push_back(u);
+      HIPCHK(hipMemcpyAsync(u, c->f, c->nfield * sizeof(double), hipMemcpyDeviceToDevice,
+                            c->stream));
+      return 0;
+    };
+    bool store = store_stages && subcycles > 0;
+    SuperstepFn ss = [&](const std::vector<int> &reals, const std::vector<double> &dt2,
+                         const std::vector<int64_t> &sv, bool use_af) -> int {
+      CycleRec cr;
+      cr.reals = reals;
+      cr.dt.assign(nr, 0.0);
+      cr.s.assign(nr, 0);
+      for (int r : reals) {
+        cr.dt[r] = dt2[r];
+        cr.s[r] = sv[r];
+      }
+      int rc2 = store ? superstep_store(c, reals, dt2, sv, use_af, cr, next)
+                      : rkg2_superstep(c, reals, dt2, sv, use_af);
+      st.recs.push_back(std::move(cr));
+      return rc2;
+    };
+    if ((rc = diffusion_impl(c, dtl, subcycles, nullptr, nullptr, &pre, &ss))) return rc;
+  }
+  if (strang && adv) {
+    for (int i = 0; i < 4; i++)
+      if (!(st.XB[i] = adj_buf(c, c->adj_st, next++, &rc))) return rc;
+    if ((rc = advection_store(c, adt, st.XB))) return rc;
+  }
+  return 0;
+}
+
+// Transposed sweeps of one step over LAM (pitched layout).
+int backward_step(hft_ctx *c, double dtl, int strang, const StepStore &st, double *LAM,
+                  bool frozen) {
+  int rc;
+  bool adv = c->flow_set, dif = c->diff_set;
+  double adt = strang ? dtl * half : dtl;
+  double *XB[4] = {st.XB[0], st.XB[1], st.XB[2], st.XB[3]};
+  double *XA[4] = {st.XA[0], st.XA[1], st.XA[2], st.XA[3]};
+  if (strang && adv && (rc = advection_adjoint(c, adt, XB, LAM, frozen))) return rc;
+  if (dif) {
+    for (int q = (int)st.recs.size() - 1; q >= 0; q--) {
+      if (st.recs[q].reals.empty()) continue;
+      if ((rc = cycle_adjoint(c, st.recs[q], st.UC[q], LAM))) return rc;
+    }
+  }
+  if (adv && (rc = advection_adjoint(c, adt, XA, LAM, frozen))) return rc;
+  return 0;
+}
+
+// Common set-up of the adjoint entry points: partial sums zeroed, LAM
+// filled from the caller's device array.
+int adjoint_begin(hft_ctx *c, const double *lam_dev, int64_t ld, double **LAM, int *nbands_out) {
+  int nr = c->nr, rc;
+  int ctiles = (c->nu + kBlock - 1) / kBlock;
+  size_t nD = (size_t)nr * c->ntm * ctiles;
+  int nbands = (c->ntm + c->adj_band_rows - 1) / c->adj_band_rows;
+  size_t nA = (size_t)nr * nbands * 2;
+  if (!c->d_apD) {
+    if ((rc = dalloc(&c->d_apD, nD)) || (rc = dalloc(&c->d_gout, (size_t)nr * 3))) return rc;
+  }
+  if (nA > c->apA_cap) {
+    if (c->d_apA) hipFree(c->d_apA);
+    c->d_apA = nullptr;
+    if ((rc = dalloc(&c->d_apA, nA))) return rc;
+    c->apA_cap = nA;
+  }
+  if ((rc = fill_dev(c, c->d_apD, nD, 0.0)) || (rc = fill_dev(c, c->d_apA, nA, 0.0)) ||
+      (rc = fill_dev(c, c->d_gout, (size_t)nr * 3, 0.0)))
+    return rc;
+  if (!(*LAM = adj_buf(c, c->adj_fix, AB_LAM, &rc))) return rc;
+  HIPCHK(hipMemcpy2DAsync(*LAM, c->pitch * sizeof(double), lam_dev, (size_t)ld * sizeof(double),
+                          c->nu * sizeof(double), (size_t)nr * c->ntm, hipMemcpyDeviceToDevice,
+                          c->stream));
+  *nbands_out = nbands;
+  return 0;
+}
+
+// LAM back to the caller; parameter sensitivities added into grad (host).
+int adjoint_end(hft_ctx *c, double *lam_dev, int64_t ld, const double *LAM, int nbands,
+                double *grad) {
+  int nr = c->nr;
+  int ctiles = (c->nu + kBlock - 1) / kBlock;
+  HIPCHK(hipMemcpy2DAsync(lam_dev, (size_t)ld * sizeof(double), LAM, c->pitch * sizeof(double),
+                          c->nu * sizeof(double), (size_t)nr * c->ntm, hipMemcpyDeviceToDevice,
+                          c->stream));
+  hipLaunchKernelGGL(k_sum_parts, dim3(nr), dim3(256), 0, c->stream, (const double *)c->d_apD,
+                     c->ntm * ctiles, c->ntm * ctiles, 0, 1, c->d_gout, 3, 0);
+  hipLaunchKernelGGL(k_sum_parts, dim3(nr), dim3(256), 0, c->stream, (const double *)c->d_apA,
+                     nbands, nbands * 2, 0, 2, c->d_gout, 3, 1);
+  hipLaunchKernelGGL(k_sum_parts, dim3(nr), dim3(256), 0, c->stream, (const double *)c->d_apA,
+                     nbands, nbands * 2, 1, 2, c->d_gout, 3, 2);
+  HIPCHK(hipGetLastError());
+  if (grad) {
+    std::vector<double> g((size_t)nr * 3);
+    HIPCHK(hipMemcpyAsync(g.data(), c->d_gout, g.size() * sizeof(double), hipMemcpyDeviceToHost,
+                          c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (size_t i = 0; i < g.size(); i++) grad[i] += g[i];
+  }
+  return 0;
+}
+
+int adjoint_check(hft_ctx *c, const double *lam_dev, int64_t ld, int subcycles) {
+  if (!c || !lam_dev) return fail(HFT_E_ARG, "null argument");
+  if (ld < c->nu) return fail(HFT_E_ARG, "adjoint: ld < %d", c->nu);
+  if (subcycles < 0) return fail(HFT_E_ARG, "adjoint: subcycles < 0");
+  if (c->flow_set && (c->flow2d || c->atten))
+    return fail(HFT_E_STATE, "adjoint: attenuated (state-dependent) flows have no adjoint");
+  if (c->flow_set && c->nu > 1024)
+    return fail(HFT_E_ARG, "adjoint: more than 1024 unique phi columns");
+  return 0;
+}
+
 }  // namespace
 
 void set_adjoint_attributes() {
@@ -344,116 +504,71 @@ int hft_get_state_device(hft_ctx *c, double *dst, int64_t ld) {
 
 int hft_step_adjoint(hft_ctx *c, double dtl, int strang, int subcycles, double *lam_dev,
                      int64_t ld, double *grad, int flags) {
-  if (!c || !lam_dev) return fail(HFT_E_ARG, "null argument");
-  if (ld < c->nu) return fail(HFT_E_ARG, "hft_step_adjoint: ld < %d", c->nu);
-  if (subcycles < 0) return fail(HFT_E_ARG, "hft_step_adjoint: subcycles < 0");
-  bool adv = c->flow_set, dif = c->diff_set;
-  if (adv && (c->flow2d || c->atten))
-    return fail(HFT_E_STATE, "hft_step_adjoint: attenuated (state-dependent) flows have no adjoint");
-  if (adv && c->nu > 1024)
-    return fail(HFT_E_ARG, "hft_step_adjoint: more than 1024 unique phi columns");
+  int rc = adjoint_check(c, lam_dev, ld, subcycles);
+  if (rc) return rc;
   bool frozen = (flags & HFT_ADJ_FROZEN_WENO) != 0;
-  int nr = c->nr, rc;
-  // partial sums
-  int ctiles = (c->nu + kBlock - 1) / kBlock;
-  size_t nD = (size_t)nr * c->ntm * ctiles;
-  int nbands = (c->ntm + c->adj_band_rows - 1) / c->adj_band_rows;
-  size_t nA = (size_t)nr * nbands * 2;
-  if (!c->d_apD) {
-    if ((rc = dalloc(&c->d_apD, nD)) || (rc = dalloc(&c->d_gout, (size_t)nr * 3))) return rc;
-  }
-  if (nA > c->apA_cap) {
-    if (c->d_apA) hipFree(c->d_apA);
-    c->d_apA = nullptr;
-    if ((rc = dalloc(&c->d_apA, nA))) return rc;
-    c->apA_cap = nA;
-  }
-  if ((rc = fill_dev(c, c->d_apD, nD, 0.0)) || (rc = fill_dev(c, c->d_apA, nA, 0.0)) ||
-      (rc = fill_dev(c, c->d_gout, (size_t)nr * 3, 0.0)))
-    return rc;
-  double *LAM = adj_buf(c, c->adj_fix, AB_LAM, &rc);
-  if (!LAM) return rc;
-  HIPCHK(hipMemcpy2DAsync(LAM, c->pitch * sizeof(double), lam_dev, (size_t)ld * sizeof(double),
-                          c->nu * sizeof(double), (size_t)nr * c->ntm, hipMemcpyDeviceToDevice,
-                          c->stream));
-  double *XA[4], *XB[4];
-  for (int i = 0; i < 4; i++) {
-    if (!(XA[i] = adj_buf(c, c->adj_fix, AB_XA + i, &rc))) return rc;
-    if (strang && !(XB[i] = adj_buf(c, c->adj_fix, AB_XB + i, &rc))) return rc;
-  }
-  // forward recomputation with storage
-  std::vector<CycleRec> recs;
-  double adt = strang ? dtl * half : dtl;
-  if (adv && (rc = advection_store(c, adt, XA))) return rc;
-  if (dif) {
-    std::function<int(int)> pre = [&](int round) -> int {
-      int rc2;
-      double *u = adj_buf(c, c->adj_uc, (size_t)(round - 1), &rc2);
-      if (!u) return rc2;
-      HIPCHK(hipMemcpyAsync(u, c->f, c->nfield * sizeof(double), hipMemcpyDeviceToDevice,
+  double *LAM;
+  int nbands;
+  if ((rc = adjoint_begin(c, lam_dev, ld, &LAM, &nbands))) return rc;
+  double field_mb = (double)c->nfield * 8.0 / 1048576.0;
+  bool store = c->adj_seg == 0 &&
+               (step_fields(c, dtl, strang, subcycles) + 24.0) * field_mb <= (double)c->adj_mem_mb;
+  StepStore st;
+  size_t next = 0;
+  if ((rc = forward_store_step(c, dtl, strang, subcycles, st, next, store))) return rc;
+  if ((rc = backward_step(c, dtl, strang, st, LAM, frozen))) return rc;
+  return adjoint_end(c, lam_dev, ld, LAM, nbands, grad);
+}
+
+int hft_steps_adjoint(hft_ctx *c, int nsteps, double dtl, int strang, int subcycles,
+                      double *lam_dev, int64_t ld, double *grad, int flags) {
+  int rc = adjoint_check(c, lam_dev, ld, subcycles);
+  if (rc) return rc;
+  if (nsteps < 1) return fail(HFT_E_ARG, "hft_steps_adjoint: nsteps < 1");
+  bool frozen = (flags & HFT_ADJ_FROZEN_WENO) != 0;
+  double *LAM;
+  int nbands;
+  if ((rc = adjoint_begin(c, lam_dev, ld, &LAM, &nbands))) return rc;
+  double field_mb = (double)c->nfield * 8.0 / 1048576.0;
+  double per_step = step_fields(c, dtl, strang, subcycles);
+  bool all = c->adj_seg == 0 && subcycles > 0 &&
+             ((double)nsteps * per_step + 24.0) * field_mb <= (double)c->adj_mem_mb;
+  size_t next = 0;
+  if (all) {
+    // one forward pass keeping every step's data, then the reverse sweep
+    std::vector<StepStore> st(nsteps);
+    for (int n = 0; n < nsteps; n++)
+      if ((rc = forward_store_step(c, dtl, strang, subcycles, st[n], next, true))) return rc;
+    for (int n = nsteps - 1; n >= 0; n--)
+      if ((rc = backward_step(c, dtl, strang, st[n], LAM, frozen))) return rc;
+  } else {
+    // keep the step inputs, recompute each step before its adjoint
+    std::vector<double *> X(nsteps);
+    for (int n = 0; n < nsteps; n++) {
+      if (!(X[n] = adj_buf(c, c->adj_st, next++, &rc))) return rc;
+      HIPCHK(hipMemcpyAsync(X[n], c->f, c->nfield * sizeof(double), hipMemcpyDeviceToDevice,
                             c->stream));
-      return 0;
-    };
-    // keep every RKG2 stage when the sub-cycles are fixed and they fit
-    // (adjoint_segment > 0 asks for segment recomputation instead)
-    bool store = false;
-    if (subcycles > 0 && c->adj_seg == 0) {
-      int64_t smax = 0;
-      for (int r = 0; r < nr; r++) {
-        double dte = c->mode == HFT_MODE_COUPLED ? c->dt_euler : c->dt_euler_r[r];
-        smax = std::max(smax, sts_stages(dtl / (double)subcycles, dte));
-      }
-      double field_mb = (double)c->nfield * 8.0 / 1048576.0;
-      store = (double)subcycles * (double)(smax + 2) * field_mb + 24.0 * field_mb <=
-              (double)c->adj_mem_mb;
+      if ((rc = hft_flux_transport_step(c, dtl, strang, subcycles, nullptr, nullptr))) return rc;
     }
-    size_t next = 0;
-    SuperstepFn ss = [&](const std::vector<int> &reals, const std::vector<double> &dt2,
-                         const std::vector<int64_t> &sv, bool use_af) -> int {
-      CycleRec cr;
-      cr.reals = reals;
-      cr.dt.assign(nr, 0.0);
-      cr.s.assign(nr, 0);
-      for (int r : reals) {
-        cr.dt[r] = dt2[r];
-        cr.s[r] = sv[r];
-      }
-      int rc2 = store ? superstep_store(c, reals, dt2, sv, use_af, cr, next)
-                      : rkg2_superstep(c, reals, dt2, sv, use_af);
-      recs.push_back(std::move(cr));
-      return rc2;
-    };
-    if ((rc = diffusion_impl(c, dtl, subcycles, nullptr, nullptr, &pre, &ss))) return rc;
-  }
-  if (strang && adv && (rc = advection_store(c, adt, XB))) return rc;
-  // backward sweeps
-  if (strang && adv && (rc = advection_adjoint(c, adt, XB, LAM, frozen))) return rc;
-  if (dif) {
-    for (int q = (int)recs.size() - 1; q >= 0; q--) {
-      if (recs[q].reals.empty()) continue;
-      if ((rc = cycle_adjoint(c, recs[q], c->adj_uc[q], LAM))) return rc;
+    double *XN = nullptr;
+    if (!(XN = adj_buf(c, c->adj_st, next++, &rc))) return rc;
+    HIPCHK(hipMemcpyAsync(XN, c->f, c->nfield * sizeof(double), hipMemcpyDeviceToDevice,
+                          c->stream));
+    bool store = c->adj_seg == 0 && (per_step + (double)nsteps + 26.0) * field_mb <=
+                                        (double)c->adj_mem_mb;
+    size_t mark = next;
+    for (int n = nsteps - 1; n >= 0; n--) {
+      HIPCHK(hipMemcpyAsync(c->f, X[n], c->nfield * sizeof(double), hipMemcpyDeviceToDevice,
+                            c->stream));
+      StepStore st;
+      next = mark;
+      if ((rc = forward_store_step(c, dtl, strang, subcycles, st, next, store))) return rc;
+      if ((rc = backward_step(c, dtl, strang, st, LAM, frozen))) return rc;
     }
-  }
-  if (adv && (rc = advection_adjoint(c, adt, XA, LAM, frozen))) return rc;
-  HIPCHK(hipMemcpy2DAsync(lam_dev, (size_t)ld * sizeof(double), LAM, c->pitch * sizeof(double),
-                          c->nu * sizeof(double), (size_t)nr * c->ntm, hipMemcpyDeviceToDevice,
+    HIPCHK(hipMemcpyAsync(c->f, XN, c->nfield * sizeof(double), hipMemcpyDeviceToDevice,
                           c->stream));
-  // parameter sensitivities
-  hipLaunchKernelGGL(k_sum_parts, dim3(nr), dim3(256), 0, c->stream, (const double *)c->d_apD,
-                     c->ntm * ctiles, c->ntm * ctiles, 0, 1, c->d_gout, 3, 0);
-  hipLaunchKernelGGL(k_sum_parts, dim3(nr), dim3(256), 0, c->stream, (const double *)c->d_apA,
-                     nbands, nbands * 2, 0, 2, c->d_gout, 3, 1);
-  hipLaunchKernelGGL(k_sum_parts, dim3(nr), dim3(256), 0, c->stream, (const double *)c->d_apA,
-                     nbands, nbands * 2, 1, 2, c->d_gout, 3, 2);
-  HIPCHK(hipGetLastError());
-  if (grad) {
-    std::vector<double> g((size_t)nr * 3);
-    HIPCHK(hipMemcpyAsync(g.data(), c->d_gout, g.size() * sizeof(double), hipMemcpyDeviceToHost,
-                          c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
-    for (size_t i = 0; i < g.size(); i++) grad[i] += g[i];
   }
-  return 0;
+  return adjoint_end(c, lam_dev, ld, LAM, nbands, grad);
 }
 
 }  // extern "C"

@@ -109,6 +109,11 @@ struct hft_ctx {
   int fused = 1, fused_T = 4, band_rows = 32, fused_min = 8;
   int fused_target_blocks = 0;  // shrink bands while fewer blocks than this (0: off)
   int adv_band = 1, adv_band_rows = 16;  // banded WENO3 stage kernel
+  // hipGraph replay of RKG2 super-step launch sequences, keyed by
+  // (use_af, realization -> stage count in launch order)
+  int graphs_on = 0;
+  std::vector<std::pair<std::vector<int64_t>, hipGraphExec_t>> graphs;
+  int64_t graph_hits = 0, graph_builds = 0;
   bool have_fval = false;
   // flows
   bool flow_set = false, flow2d = false;
@@ -152,7 +157,7 @@ struct hft_ctx {
   double *d_apar = nullptr;
   int *d_ipar = nullptr;
   // adjoint (hft_step_adjoint): field buffers allocated on first use
-  std::vector<double *> adj_fix, adj_uc, adj_ck, adj_w, adj_st;
+  std::vector<double *> adj_fix, adj_ck, adj_w, adj_st;
   double *d_apD = nullptr, *d_apA = nullptr, *d_gout = nullptr;
   size_t apA_cap = 0;
   int adj_band_rows = 16, adj_seg = 0;
@@ -671,14 +676,69 @@ int upload_rkg2(hft_ctx *c, const std::vector<int> &reals, const std::vector<dou
 // time step dt2[r] and stage count s[r] (diffusion_step_sts_cd,
 // src/hipft.f90:5163-5242).  use_af: dty0 already holds L(f) from the PTL
 // pass (same operator, so identical to recomputing it).
+void clear_graphs(hft_ctx *c) {
+  for (auto &g : c->graphs) hipGraphExecDestroy(g.second);
+  c->graphs.clear();
+}
+
+bool fused_usable(const hft_ctx *c) {
+  return c->fused && c->nu <= 1024 && c->fused_T >= 2 && c->fused_T <= 4 &&
+         c->band_rows + 2 * c->fused_T <= kFusedMaxRows;
+}
+
+int superstep_launches(hft_ctx *c, int n, const std::vector<int> &ord,
+                       const std::vector<int64_t> &s, bool use_af);
+
 int rkg2_superstep(hft_ctx *c, const std::vector<int> &reals, const std::vector<double> &dt2,
                    const std::vector<int64_t> &s, bool use_af) {
   int n = (int)reals.size();
   if (n == 0) return 0;
-  int nr = c->nr;
   std::vector<int> ord;
-  int rc0 = upload_rkg2(c, reals, dt2, s, ord);
-  if (rc0) return rc0;
+  int rc = upload_rkg2(c, reals, dt2, s, ord);
+  if (rc) return rc;
+  if (fused_usable(c) && !c->buf2) {
+    if ((rc = dalloc(&c->buf2, c->nfield)) || (rc = dalloc(&c->buf3, c->nfield))) return rc;
+  }
+  // the launch sequence depends only on the stage counts in launch order
+  // (the coefficient tables are device data uploaded above)
+  if (!c->graphs_on || c->timing || c->stream == nullptr) return superstep_launches(c, n, ord, s, use_af);
+  std::vector<int64_t> key;
+  key.reserve(2 * n + 1);
+  key.push_back(use_af ? 1 : 0);
+  for (int z = 0; z < n; z++) {
+    key.push_back(ord[z]);
+    key.push_back(s[ord[z]]);
+  }
+  for (auto &g : c->graphs)
+    if (g.first == key) {
+      c->graph_hits++;
+      HIPCHK(hipGraphLaunch(g.second, c->stream));
+      return 0;
+    }
+  if (c->graphs.size() >= 64) clear_graphs(c);
+  HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  int rcl = superstep_launches(c, n, ord, s, use_af);
+  hipGraph_t graph = nullptr;
+  hipError_t e = hipStreamEndCapture(c->stream, &graph);
+  if (rcl) {
+    if (graph) hipGraphDestroy(graph);
+    return rcl;
+  }
+  if (e != hipSuccess) return fail(HFT_E_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+  hipGraphExec_t exec = nullptr;
+  e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  hipGraphDestroy(graph);
+  if (e != hipSuccess) return fail(HFT_E_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+  c->graphs.push_back({key, exec});
+  c->graph_builds++;
+  HIPCHK(hipGraphLaunch(exec, c->stream));
+  return 0;
+}
+
+// The launches of one RKG2 super-step (tables already uploaded).
+int superstep_launches(hft_ctx *c, int n, const std::vector<int> &ord,
+                       const std::vector<int64_t> &s, bool use_af) {
+  int nr = c->nr;
   int64_t smax = s[ord[0]];
   Stage1Args a1;
   a1.m = c->m; a1.g = tiles_for(c, n); a1.f = c->f; a1.dty0 = c->dty0; a1.u1 = c->buf1;
@@ -690,11 +750,7 @@ int rkg2_superstep(hft_ctx *c, const std::vector<int> &reals, const std::vector<
   // overwrite u_{k-2} in place (pointwise); fused launches write two fresh
   // buffers.  u_s of a realization that finishes in a fused launch is left
   // in that launch's outA and copied into f afterwards.
-  bool fused_ok = c->fused && c->nu <= 1024 && c->fused_T >= 2 && c->fused_T <= 4 &&
-                  c->band_rows + 2 * c->fused_T <= kFusedMaxRows;
-  if (fused_ok && !c->buf2) {
-    if ((rc = dalloc(&c->buf2, c->nfield)) || (rc = dalloc(&c->buf3, c->nfield))) return rc;
-  }
+  bool fused_ok = fused_usable(c) && c->buf2;
   double *pA = c->buf1, *pB = c->f;
   std::vector<double *> final_loc(nr, nullptr);
   auto pick_free = [&](double *x, double *y) {
@@ -917,7 +973,8 @@ int hft_destroy(hft_ctx *c) {
   }
   for (auto e : c->free_ev) hipEventDestroy(e);
   if (c->coef_ev) hipEventDestroy(c->coef_ev);
-  for (auto *vec : {&c->adj_fix, &c->adj_uc, &c->adj_ck, &c->adj_w, &c->adj_st})
+  for (auto &g : c->graphs) hipGraphExecDestroy(g.second);
+  for (auto *vec : {&c->adj_fix, &c->adj_ck, &c->adj_w, &c->adj_st})
     for (double *q : *vec)
       if (q) hipFree(q);
   for (double *q : {c->d_apD, c->d_apA, c->d_gout})
@@ -1545,6 +1602,7 @@ int hft_stream(hft_ctx *c, void **stream) {
 
 int hft_set_tuning(hft_ctx *c, const char *key, int value) {
   if (!c || !key) return fail(HFT_E_ARG, "hft_set_tuning: null argument");
+  clear_graphs(c);  // captured launch sequences depend on the tuning
   if (!strcmp(key, "fused")) c->fused = value != 0;
   else if (!strcmp(key, "fused_stages")) {
     if (value < 2 || value > 4) return fail(HFT_E_ARG, "fused_stages must be 2..4");
@@ -1558,6 +1616,8 @@ int hft_set_tuning(hft_ctx *c, const char *key, int value) {
   } else if (!strcmp(key, "fused_target_blocks")) {
     if (value < 0) return fail(HFT_E_ARG, "fused_target_blocks must be >= 0");
     c->fused_target_blocks = value;
+  } else if (!strcmp(key, "graphs")) {
+    c->graphs_on = value != 0;
   } else if (!strcmp(key, "adjoint_segment")) {
     if (value < 0) return fail(HFT_E_ARG, "adjoint_segment must be >= 0");
     c->adj_seg = value;
@@ -1585,6 +1645,9 @@ int hft_get_tuning(hft_ctx *c, const char *key, int *value) {
   else if (!strcmp(key, "band_rows")) *value = c->band_rows;
   else if (!strcmp(key, "fused_min_realizations")) *value = c->fused_min;
   else if (!strcmp(key, "fused_target_blocks")) *value = c->fused_target_blocks;
+  else if (!strcmp(key, "graphs")) *value = c->graphs_on;
+  else if (!strcmp(key, "graph_hits")) *value = (int)c->graph_hits;
+  else if (!strcmp(key, "graph_builds")) *value = (int)c->graph_builds;
   else if (!strcmp(key, "adjoint_segment")) *value = c->adj_seg;
   else if (!strcmp(key, "adjoint_mem_mb")) *value = (int)c->adj_mem_mb;
   else if (!strcmp(key, "adjoint_band_rows")) *value = c->adj_band_rows;

@@ -44,22 +44,10 @@ def candidates(nr):
     return out
 
 
-def device_name():
-    import ctypes
-    try:
-        hip = ctypes.CDLL("libamdhip64.so")
-        dev = ctypes.c_int()
-        hip.hipGetDevice(ctypes.byref(dev))
-        buf = ctypes.create_string_buffer(256)
-        if hip.hipDeviceGetName(buf, 256, dev.value) == 0:
-            return buf.value.decode()
-    except OSError:
-        pass
-    return "unknown"
-
-
 def cache_key(model):
-    return f"{device_name() or 'gfx950'}|{model.ntm}x{model.npm1}|nr={model.nr}"
+    # libhft is built for gfx950 only; the marketing name that HIP reports
+    # varies with how the runtime was initialised, so it is not part of the key
+    return f"gfx950|{model.ntm}x{model.npm1}|nr={model.nr}"
 
 
 def load_cache():

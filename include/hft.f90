@@ -224,5 +224,14 @@ module hft
       integer(c_int64_t), value :: ld
       real(c_double), intent(inout) :: grad(*)
     end function
+    integer(c_int) function hft_steps_adjoint(ctx, nsteps, dt, strang, subcycles, lam, ld, grad, &
+        flags) bind(c, name="hft_steps_adjoint")
+      import :: c_int, c_double, c_ptr, c_int64_t
+      type(c_ptr), value :: ctx, lam
+      integer(c_int), value :: nsteps, strang, subcycles, flags
+      real(c_double), value :: dt
+      integer(c_int64_t), value :: ld
+      real(c_double), intent(inout) :: grad(*)
+    end function
   end interface
 end module hft

@@ -190,6 +190,11 @@ int hft_stream(hft_ctx *ctx, void **stream);
  *   "advection_band"         1 = WENO3 stages by theta bands computing each
  *                            face once (default); 0 = one thread per point
  *   "advection_band_rows"    theta rows per block of the banded WENO3 (16)
+ *   "graphs"                 1 = capture each RKG2 super-step's launches in a
+ *                            hipGraph and replay it when the same
+ *                            realizations have the same stage counts (0);
+ *                            "graph_hits", "graph_builds" (read only) count
+ *                            replays and captures
  *   "adjoint_segment"        RKG2 stages per recomputed segment of the
  *                            adjoint (0 = all stages of a cycle when they fit
  *                            in "adjoint_mem_mb", else sqrt(s))
@@ -235,6 +240,14 @@ int hft_set_stream(hft_ctx *ctx, void *stream, int own);
 #define HFT_ADJ_FROZEN_WENO 1
 int hft_step_adjoint(hft_ctx *ctx, double dt, int strang, int subcycles, double *lam,
                      int64_t ld, double *grad, int flags);
+/* Adjoint of nsteps consecutive steps from the current state x_n: lam goes
+ * from dJ/dx_{n+nsteps} to dJ/dx_n, grad accumulates as above, and the state
+ * is x_{n+nsteps} on return.  With fixed sub-cycles and everything within
+ * "adjoint_mem_mb", one forward pass keeps every step's stage data and the
+ * reverse sweep follows (no recomputation); otherwise the step inputs are
+ * kept and each step is recomputed before its adjoint. */
+int hft_steps_adjoint(hft_ctx *ctx, int nsteps, double dt, int strang, int subcycles,
+                      double *lam, int64_t ld, double *grad, int flags);
 
 #ifdef __cplusplus
 }

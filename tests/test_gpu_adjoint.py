@@ -128,6 +128,35 @@ def test_segmented_rkg2_adjoint_matches_stored(gpu_available):
         assert np.allclose(grads, out[0][1], rtol=1e-12, atol=0)
 
 
+@pytest.mark.parametrize("mem_mb", [32768, 1])
+def test_multi_step_adjoint_matches_step_by_step(gpu_available, mem_mb):
+    """hft_steps_adjoint over 3 steps (one storing forward pass, or step
+    inputs with recomputation when the memory budget is tiny) equals three
+    hft_step_adjoint calls in reverse, and leaves the final state."""
+    o, g, ts = build(33, 65, 2, [250.0, 800.0])
+    nu = g.npm - 2
+    x0 = dev_state(g)
+    xs = [x0]
+    for _ in range(3):
+        g.flux_transport_step(1.0, strang=True, subcycles=2)
+        xs.append(dev_state(g))
+    rng = np.random.default_rng(3)
+    lam0 = torch.as_tensor(rng.standard_normal(tuple(x0.shape)), device="cuda")
+    lam = lam0.clone()
+    ref = np.zeros((2, 3))
+    for n in (2, 1, 0):
+        g.set_state_device(xs[n].data_ptr(), nu)
+        ref += g.step_adjoint(1.0, lam.data_ptr(), nu, strang=True, subcycles=2)
+    g.set_tuning(adjoint_mem_mb=mem_mb)
+    g.set_state_device(x0.data_ptr(), nu)
+    lam2 = lam0.clone()
+    got = g.steps_adjoint(3, 1.0, lam2.data_ptr(), nu, strang=True, subcycles=2)
+    torch.cuda.synchronize()
+    assert rel(lam2, lam) < 1e-13
+    assert np.allclose(got, ref, rtol=1e-12, atol=0)
+    assert rel(dev_state(g), xs[3]) == 0.0
+
+
 def test_gradient_vs_central_differences(gpu_available):
     """32 random map directions (h = 1e-6 relative) and the three parameters,
     against the forward HIP step (two steps, fixed sub-cycling).  The target
