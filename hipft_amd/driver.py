@@ -40,6 +40,8 @@ DEFAULTS = dict(
     diffusion_coef_factor=DIFF_KM2_S_TO_RS2_HR, diffusion_subcycles=0,
     output_history_time_cadence=0.0, output_map_time_cadence=0.0, output_map_idx_start=1,
     initial_map=None, initial_scales=None, initial_map_mult_fac=1.0,
+    output_map_directory=None, output_map_root_filename="hipft_brmap", output_map_2d=True,
+    output_single_precision=True,
     assimilate_data=False, assimilation_maps=None, assimilation_times=None,
     assimilate_data_custom_from_mu=False, assimilate_data_mu_power=4.0,
     assimilate_data_mu_powers=None, assimilate_data_mu_limit=0.1,
@@ -150,9 +152,27 @@ def run(cfg, max_steps=None, mode="coupled"):
         return cad > 0 and time >= c["time_start"] + (
             st["idx_out"] - (c["output_map_idx_start"] - 1)) * cad
 
+    def write_map(name):
+        # write_map (src/hipft.f90:2801-2917): PT maps, one-point phi overlap
+        d = c["output_map_directory"]
+        if not d:
+            return
+        import os
+        from . import h5io
+        os.makedirs(d, exist_ok=True)
+        maps = m.get_state("pt")
+        path = os.path.join(d, f"{c['output_map_root_filename']}{name}.h5")
+        if c["output_map_2d"] and nr == 1:
+            h5io.write_map(path, maps[0], [R.p, R.t], single=c["output_single_precision"])
+        else:
+            h5io.write_map(path, maps, [R.p, R.t, np.arange(1, nr + 1, dtype=np.float64)],
+                           single=c["output_single_precision"])
+
+    write_map("_initial")  # load_initial_condition, 2185
     emit(st["time"])
     if map_due(st["time"]):
         st["idx_out"] += 1
+        write_map(f"_idx{st['idx_out']:06d}")
     steps = 0
     while True:
         st["ntime"] += 1
@@ -236,9 +256,11 @@ def run(cfg, max_steps=None, mode="coupled"):
         emit(st["time"])
         if map_due(st["time"]):
             st["idx_out"] += 1
+            write_map(f"_idx{st['idx_out']:06d}")  # output_map_step, 2725-2760
         steps += 1
         if st["time"] >= c["time_end"] or (max_steps is not None and steps >= max_steps):
             break
+    write_map("_final")  # 1390
     return np.array(sol), np.array(num), R
 
 

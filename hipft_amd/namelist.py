@@ -88,11 +88,22 @@ LIST_KEYS = {
 }
 
 
-def to_run_config(nml):
-    """Map a parsed namelist onto driver.run's configuration dict."""
+def to_run_config(nml, base_dir="."):
+    """Map a parsed namelist onto driver.run's configuration dict.  An
+    ``initial_map_filename`` is read (HDF5, hipft_amd.h5io) into the initial
+    map and its scales, as load_initial_condition does (src/hipft.f90:1820-1870);
+    relative paths are taken from base_dir."""
+    import os
     cfg = {}
     for k, v in nml.items():
         if k in LIST_KEYS and not isinstance(v, list):
             v = [v]
         cfg[k] = v
+    fname = cfg.get("initial_map_filename")
+    if fname:
+        from . import h5io
+        path = fname if os.path.isabs(fname) else os.path.join(base_dir, fname)
+        data, scales = h5io.read_map(path)
+        cfg["initial_map"] = data
+        cfg["initial_scales"] = scales[:2]
     return cfg

@@ -5,6 +5,8 @@ Operators and single steps are compared bit for bit with the oracle (same
 inputs, same operation order, no FMA contraction, same polar-sum order).
 Whole runs are compared with the reference's committed histories.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -388,3 +390,23 @@ def test_graph_replay_matches_direct_launches(gpu_available, fused):
         g.close()
     for a, b in zip(out[1], out[0]):
         assert_same(a, b, f"graph replay (fused={fused})")
+
+
+def test_driver_writes_map_files(gpu_available, tmp_path):
+    """The driver's map output (write_map / output cadence): initial, indexed
+    and final HDF5 maps in the reference's layout; the final file holds the
+    final state (float32 by default, float64 on request)."""
+    from hipft_amd import h5io
+    cfg = dict(res_nt=31, res_np=61, n_realizations=2, validation_run=1, advance_diffusion=True,
+               diffusion_coef_constants=[500.0, 800.0], time_end=3.0, dt_max=1.0,
+               output_map_time_cadence=1.0, output_map_directory=str(tmp_path),
+               output_single_precision=False)
+    sol, num, R = driver.run(cfg)
+    names = sorted(os.listdir(tmp_path))
+    assert "hipft_brmap_initial.h5" in names and "hipft_brmap_final.h5" in names
+    assert "hipft_brmap_idx000001.h5" in names and "hipft_brmap_idx000004.h5" in names
+    d, sc = h5io.read_map(str(tmp_path / "hipft_brmap_final.h5"))
+    assert d.shape == (2, 31, 61) and np.array_equal(sc[2], [1.0, 2.0])
+    assert np.array_equal(d, R.model.get_state("pt"))
+    d0, _ = h5io.read_map(str(tmp_path / "hipft_brmap_initial.h5"))
+    assert not np.array_equal(d0, d)
