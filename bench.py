@@ -215,6 +215,11 @@ def main():
                                    "(realizations would shard across GPUs with no per-step "
                                    "communication in independent mode)"}))
         sys.exit(2)
+    if args.adjoint_only or not args.no_adjoint:
+        # the adjoint section shares the device with PyTorch: bring up torch's
+        # HIP context before libhft creates its own streams
+        import torch
+        torch.cuda.init()
     if args.adjoint_only:
         print(json.dumps({"adjoint": adjoint_bench()}), flush=True)
         return

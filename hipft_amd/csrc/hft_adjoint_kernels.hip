@@ -407,7 +407,7 @@ __global__ __launch_bounds__(1024) void k_advection_vjp_band(AdvVjpArgs a) {
       sLN[c] = lnc;
       sLP[c] = lpc;
       sLam[c] = col ? LAM(j) : 0.0;
-      __syncthreads();
+      lds_barrier();
       double sti = m.st_i[j];
       if (col) {
         int c2 = wrapc(cc - 2, nu), c1 = wrapc(cc - 1, nu), cp = wrapc(cc + 1, nu);
@@ -423,14 +423,14 @@ __global__ __launch_bounds__(1024) void k_advection_vjp_band(AdvVjpArgs a) {
       } else {
         for (int d = 0; d < 6; d++) sG[d * ncol + c] = 0.0;
       }
-      __syncthreads();
+      lds_barrier();
       if (col) {
         int cm = wrapc(cc - 1, nu), cp = wrapc(cc + 1, nu), cp2 = wrapc(cc + 2, nu);
         double sn = sG[2 * ncol + cc] + sG[1 * ncol + cp] + sG[0 * ncol + cp2];
         double sp = sG[5 * ncol + cm] + sG[4 * ncol + cc] + sG[3 * ncol + cp];
         yv = yv + (half * (vpj + apj) * sn + half * (vpj - apj) * sp);
       }
-      __syncthreads();
+      lds_barrier();
     }
     if (col) {
       size_t off = (size_t)j * pitch + c;

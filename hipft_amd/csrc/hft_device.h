@@ -53,6 +53,16 @@ struct Flow2D {
   size_t vt_rstride;  // nt*pitch
 };
 
+// Workgroup barrier that orders LDS only.  The band kernels exchange data
+// between waves through LDS alone; a full __syncthreads() would also wait
+// for every outstanding global load (vmcnt(0)) and so stall the loads they
+// prefetch rows ahead.
+__device__ inline void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 __host__ __device__ inline size_t fidx(int r, int j, int c, int ntm, int pitch) {
   return ((size_t)r * ntm + j) * (size_t)pitch + c;
 }

@@ -373,7 +373,7 @@ __device__ __forceinline__ void fused_step(const FusedArgs &a, FusedState<T> &S,
       if (act && R > 0) lds[(size_t)(t * 3 + (R % 3)) * ncol + c] = v;
     }
   }
-  __syncthreads();
+  lds_barrier();
 }
 
 template <int T>
@@ -466,7 +466,7 @@ __global__ __launch_bounds__(1024) void k_rkg2_fused(FusedArgs a) {
     S.pf[i][2] = v2;
     S.pf[i][3] = v3;
   }
-  __syncthreads();
+  lds_barrier();
   for (int i = 0; i < nsteps; i += 4) {
     fused_step<T, 0>(a, S, lds, rowc, i, nsteps, lo0, hi0, j0, j1, nst, final_here, col, c, cw,
                      ce, D2, P1, P5, A, B, U0, DT, OA, OB, u0_pole, dt_pole, cfs);
@@ -479,6 +479,292 @@ __global__ __launch_bounds__(1024) void k_rkg2_fused(FusedArgs a) {
     if (i + 3 < nsteps)
       fused_step<T, 3>(a, S, lds, rowc, i + 3, nsteps, lo0, hi0, j0, j1, nst, final_here, col,
                        c, cw, ce, D2, P1, P5, A, B, U0, DT, OA, OB, u0_pole, dt_pole, cfs);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Fused RKG2 stages, lean march (k_rkg2_march<T>): the same schedule and
+// arithmetic as k_rkg2_fused, with the per-step bookkeeping reduced to a few
+// scalar instructions.  The march is unrolled by 6, so every ring slot is a
+// compile-time constant (own-column windows: 3 slots by age; u0/dty0
+// histories: shifted by one each step; LDS rows: slot (R - lo0) mod 3 =
+// (i - t) mod 3); row
+// pointers advance by one pitch per step; and every row condition is a
+// comparison of the step index with a per-block constant.
+// ---------------------------------------------------------------------------
+template <int T>
+struct MarchState {
+  double w[T][3];       // level t own-column rows, slot (i - age) mod 3
+  double p0[T];         // level t north-pole row (formed late)
+  double hu[4], hd[4];  // u0 and dty0 at rows q-1-age, index age (shifted each step)
+  double pf[2][4];      // prefetched inputs of the next two steps
+};
+
+template <int T>
+struct MarchPlan {
+  int nsteps, iA, iB;
+  int ia[T + 1], ib[T + 1];  // level t active for ia <= i < ib
+  int b_one;                 // level t computes row 1 at step b_one + t (R >= 1 from there)
+  int north_tmax;            // levels t <= north_tmax form their north-pole row late
+  int b_south;               // level t computes row ntm-1 at step b_south + t
+  int b_o0, b_o1;            // level t rows inside the band: b_o0 + t <= i < b_o1 + t
+  int sS1, sS2;              // LDS ring offsets of rows ntm-1, ntm-2
+  int nst;                   // stages this launch advances for the realization
+  bool final_here, pole0_out, lo0zero;
+};
+
+template <int T, int U>
+__device__ __forceinline__ void march_step(MarchState<T> &S, double *lds, const double *rowc,
+                                           const MarchPlan<T> &P, const Mesh &m, int i,
+                                           bool col, int c, int cw, int ce, int ncol, double D2,
+                                           double P1, double P5, const double *A,
+                                           const double *B, const double *U0, const double *DT,
+                                           size_t pitch, size_t &rowq, double *OA, double *OB,
+                                           double u0_pole, double dt_pole,
+                                           const double (*cfs)[5]) {
+  const int nu = m.nu;
+  constexpr int s0 = U % 3, s1 = (U + 2) % 3, s2 = (U + 1) % 3;  // ages 0, 1, 2
+  constexpr int ps = U & 1;
+  double a0 = S.pf[ps][0], b1 = S.pf[ps][1], u1 = S.pf[ps][2], d1 = S.pf[ps][3];
+  {
+    // prefetch for step i+2: A row q+2, B/u0/dty0 row q+1 (rowq = q*pitch)
+    double v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0;
+    if (col) {
+      if (i + 2 < P.iA) v0 = A[rowq + 2 * pitch + c];
+      if (i + 2 < P.nsteps && i + 1 < P.iB) {
+        size_t o = rowq + pitch + c;
+        v1 = B[o];
+        v2 = U0[o];
+        v3 = DT[o];
+      }
+    }
+    S.pf[ps][0] = v0;
+    S.pf[ps][1] = v1;
+    S.pf[ps][2] = v2;
+    S.pf[ps][3] = v3;
+  }
+#pragma unroll
+  for (int d = 3; d > 0; d--) {
+    S.hu[d] = S.hu[d - 1];
+    S.hd[d] = S.hd[d - 1];
+  }
+  S.hu[0] = u1;
+  S.hd[0] = d1;
+  // level 0: row q of u_{k0-1}
+  S.w[0][s0] = a0;
+  if (i < P.iA) {
+    lds[(size_t)(U % 3) * ncol + c] = a0;
+    if (P.lo0zero && i == 0) S.p0[0] = a0;
+  }
+#pragma unroll
+  for (int t = 1; t <= T; t++) {
+    const int sl = ((U - t) % 3 + 3) % 3;  // LDS slot of row R = q - t
+    const double *lp = lds + (size_t)((t - 1) * 3) * ncol;  // level t-1 ring
+    const double *k = cfs[t - 1];
+    double u0v = S.hu[t - 1], dtv = S.hd[t - 1];  // u0/dty0 of row R: age t-1
+    const int ione = P.b_one + t;
+    bool act = t <= P.nst && i >= P.ia[t] && i < P.ib[t];
+    bool wantA = t == P.nst, wantB = !P.final_here && t == T - 1;
+    double v = 0.0;
+    if (act && t <= P.north_tmax && i == ione) {
+      // late north-pole row 0 of stage t, from level t-1 rows 0 and 1 (slots 0, 1)
+      const double *r0 = lp, *r1p = lp + ncol;
+      double fn = wave_pole_sum(nu, [&](int kk) { return (D2 * (r1p[kk] - r0[kk])) * m.dp[kk + 1]; });
+      double Lv = fn * m.bc_diff_n;
+      double xc = S.p0[t - 1];
+      double ukm2 = (t == 1) ? B[c < nu ? c : 0] : S.p0[t < 2 ? 0 : t - 2];
+      double vv = k[0] * xc;
+      vv = vv + k[1] * ukm2;
+      vv = vv + k[2] * u0_pole;
+      vv = vv + k[3] * Lv;
+      vv = vv + k[4] * dt_pole;
+      if (t < T) {
+        S.p0[t] = vv;
+        lds[(size_t)(t * 3 + 0) * ncol + c] = vv;
+      }
+      if (col && P.pole0_out) {
+        if (wantA) OA[c] = vv;
+        if (wantB) OB[c] = vv;
+      }
+    }
+    if (act && i >= ione) {
+      double Lv;
+      if (i == P.b_south + t) {
+        const double *r0 = lp + P.sS1, *r1p = lp + P.sS2;
+        double fs = wave_pole_sum(nu, [&](int kk) { return (D2 * (r0[kk] - r1p[kk])) * m.dp[kk + 1]; });
+        Lv = -fs * m.bc_diff_s;
+      } else {
+        const double *rc = lp + (size_t)sl * ncol;
+        double xc = S.w[t - 1][s1];
+        double xs = S.w[t - 1][s0];
+        double xn = (i == ione) ? S.p0[t - 1] : S.w[t - 1][s2];
+        double xw = rc[cw], xe = rc[ce];
+        const double *rr = rowc + 4 * (i - t);
+        double sti = rr[0];
+        double c1 = (P1 * sti) * sti;
+        double c2 = rr[1];
+        double c4 = rr[2];
+        double c5 = (P5 * sti) * sti;
+        double c3 = -(((c1 + c2) + c4) + c5);
+        Lv = c1 * xw;
+        Lv = Lv + c2 * xn;
+        Lv = Lv + c3 * xc;
+        Lv = Lv + c4 * xs;
+        Lv = Lv + c5 * xe;
+      }
+      double xc = S.w[t - 1][s1];
+      double ukm2 = (t == 1) ? b1 : S.w[t < 2 ? 0 : t - 2][s2];
+      v = k[0] * xc;
+      v = v + k[1] * ukm2;
+      v = v + k[2] * u0v;
+      v = v + k[3] * Lv;
+      v = v + k[4] * dtv;
+      if (col && i >= P.b_o0 + t && i < P.b_o1 + t) {
+        size_t o = rowq - (size_t)t * pitch + c;
+        if (wantA) OA[o] = v;
+        if (wantB) OB[o] = v;
+      }
+    }
+    if (t < T) {
+      S.w[t][s0] = v;
+      if (act && i >= ione) lds[(size_t)(t * 3 + sl) * ncol + c] = v;
+    }
+  }
+  rowq += pitch;
+  lds_barrier();
+}
+
+template <int T>
+__global__ __launch_bounds__(1024) void k_rkg2_march(FusedArgs a) {
+  extern __shared__ double lds[];  // [T][3][ncol]
+  __shared__ double cfs[4][5];
+  __shared__ double rowc[4 * kFusedMaxRows];
+  const Mesh &m = a.m;
+  const int nu = m.nu, ntm = m.ntm, ncol = a.ncol;
+  const size_t pitch = (size_t)m.pitch;
+  int L = xcd_remap(blockIdx.x, gridDim.x);
+  int z = L / a.nbands, band = L - z * a.nbands;
+  int r = a.order[z];
+  int c = threadIdx.x;
+  bool col = c < nu;
+  int cc = col ? c : 0;
+  int cw = wrapc(cc - 1, nu), ce = wrapc(cc + 1, nu);
+  int j0 = band * a.H, j1 = min(ntm, j0 + a.H);
+  int lo0 = max(0, j0 - T), hi0 = min(ntm, j1 + T);
+  int64_t sr = a.s[r];
+  int nst = (int)((sr - a.k0 + 1) < T ? (sr - a.k0 + 1) : T);
+  size_t base = (size_t)r * a.rstride;
+  const double *A = a.A + base, *B = a.B + base, *U0 = a.f + base, *DT = a.dty0 + base;
+  double *OA = a.outA + base, *OB = a.outB + base;
+  double D = a.D[r];
+  double D2 = D + D;
+  double P1 = (D * m.dph_i[cc + 1]) * m.dp_i[cc + 1];
+  double P5 = (D * m.dph_i[cc + 2]) * m.dp_i[cc + 1];
+  if (threadIdx.x < T * 5) {
+    int t = threadIdx.x / 5, e = threadIdx.x - 5 * t;
+    double v = 0.0;
+    if (t < nst) {
+      const StageCoef &k = a.coef[(size_t)(a.k0 + t) * a.nr_table + r];
+      v = e == 0 ? k.a : (e == 1 ? k.b : (e == 2 ? k.c : (e == 3 ? k.d : k.g)));
+    }
+    cfs[t][e] = v;
+  }
+  for (int R = lo0 + (int)threadIdx.x; R < hi0; R += blockDim.x) {
+    double *rr = rowc + 4 * (R - lo0);
+    if (R > 0 && R < ntm - 1) {
+      double sti = m.st_i[R];
+      double c2 = D * m.dth_i[R];
+      c2 = c2 * m.dt_i[R];
+      c2 = c2 * sti;
+      c2 = c2 * m.sth[R];
+      double c4 = D * m.dth_i[R + 1];
+      c4 = c4 * m.dt_i[R];
+      c4 = c4 * sti;
+      c4 = c4 * m.sth[R + 1];
+      rr[0] = sti;
+      rr[1] = c2;
+      rr[2] = c4;
+    } else {
+      rr[0] = rr[1] = rr[2] = 0.0;
+    }
+    rr[3] = 0.0;
+  }
+  MarchPlan<T> P;
+  P.nsteps = (hi0 - lo0) + T;
+  P.iA = hi0 - lo0;
+  P.iB = ntm - lo0;
+#pragma unroll
+  for (int t = 1; t <= T; t++) {
+    int rlo = max(0, j0 - T + t), rhi = min(ntm, j1 + T - t);
+    P.ia[t] = rlo - lo0 + t;
+    P.ib[t] = rhi - lo0 + t;
+  }
+  P.ia[0] = P.ib[0] = 0;
+  P.b_one = 1 - lo0;
+  P.north_tmax = T - j0;  // rlo_t = max(0, j0 - T + t) == 0
+  P.b_south = ntm - 1 - lo0;
+  P.b_o0 = j0 - lo0;
+  P.b_o1 = j1 - lo0;
+  P.sS1 = (((ntm - 1 - lo0) % 3 + 3) % 3) * ncol;
+  P.sS2 = (((ntm - 2 - lo0) % 3 + 3) % 3) * ncol;
+  P.nst = nst;
+  P.final_here = (a.k0 + nst - 1) == sr;
+  P.pole0_out = j0 == 0;
+  P.lo0zero = lo0 == 0;
+  MarchState<T> S;
+#pragma unroll
+  for (int t = 0; t < T; t++) {
+    S.p0[t] = 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; d++) S.w[t][d] = 0.0;
+  }
+#pragma unroll
+  for (int d = 0; d < 4; d++) S.hu[d] = S.hd[d] = 0.0;
+  double u0_pole = 0.0, dt_pole = 0.0;
+  if (lo0 == 0 && col) {
+    u0_pole = U0[c];
+    dt_pole = DT[c];
+  }
+  // inputs of steps 0 and 1: A rows lo0, lo0+1; B/u0/dty0 rows lo0-1, lo0
+#pragma unroll
+  for (int i = 0; i < 2; i++) {
+    int q = lo0 + i, rn = q - 1;
+    double v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0;
+    if (col && i < P.nsteps) {
+      if (q < hi0) v0 = A[(size_t)q * pitch + c];
+      if (rn >= 0 && rn < ntm) {
+        size_t o = (size_t)rn * pitch + c;
+        v1 = B[o];
+        v2 = U0[o];
+        v3 = DT[o];
+      }
+    }
+    S.pf[i][0] = v0;
+    S.pf[i][1] = v1;
+    S.pf[i][2] = v2;
+    S.pf[i][3] = v3;
+  }
+  size_t rowq = (size_t)lo0 * pitch;
+  lds_barrier();
+  const int nsteps = P.nsteps;
+  for (int i = 0; i < nsteps; i += 6) {
+    march_step<T, 0>(S, lds, rowc, P, m, i, col, c, cw, ce, ncol, D2, P1, P5, A, B, U0, DT, pitch,
+                     rowq, OA, OB, u0_pole, dt_pole, cfs);
+    if (i + 1 >= nsteps) break;
+    march_step<T, 1>(S, lds, rowc, P, m, i + 1, col, c, cw, ce, ncol, D2, P1, P5, A, B, U0, DT,
+                     pitch, rowq, OA, OB, u0_pole, dt_pole, cfs);
+    if (i + 2 >= nsteps) break;
+    march_step<T, 2>(S, lds, rowc, P, m, i + 2, col, c, cw, ce, ncol, D2, P1, P5, A, B, U0, DT,
+                     pitch, rowq, OA, OB, u0_pole, dt_pole, cfs);
+    if (i + 3 >= nsteps) break;
+    march_step<T, 3>(S, lds, rowc, P, m, i + 3, col, c, cw, ce, ncol, D2, P1, P5, A, B, U0, DT,
+                     pitch, rowq, OA, OB, u0_pole, dt_pole, cfs);
+    if (i + 4 >= nsteps) break;
+    march_step<T, 4>(S, lds, rowc, P, m, i + 4, col, c, cw, ce, ncol, D2, P1, P5, A, B, U0, DT,
+                     pitch, rowq, OA, OB, u0_pole, dt_pole, cfs);
+    if (i + 5 >= nsteps) break;
+    march_step<T, 5>(S, lds, rowc, P, m, i + 5, col, c, cw, ce, ncol, D2, P1, P5, A, B, U0, DT,
+                     pitch, rowq, OA, OB, u0_pole, dt_pole, cfs);
   }
 }
 
@@ -904,17 +1190,17 @@ __global__ __launch_bounds__(1024) void k_advection_band(AdvBandArgs a) {
       // upwind pole-face flux of the own column, summed over the row
       double fface = (j == 0) ? ft_dn : ft_up;
       if (col) sFP[c] = fface * m.dp[c + 1];
-      __syncthreads();
+      lds_barrier();
       double s = wave_pole_sum(nu, [&](int k) { return sFP[k]; });
       A = (j == 0) ? s * m.bc_flow_n : -s * m.bc_flow_s;
-      __syncthreads();
+      lds_barrier();
     } else {
       // phi split fluxes of row j (own cell), shared through LDS
       double xv = X[1];
       double av = fa.ap(j, cc);
       sLN[c] = half * xv * (fa.vp(j, cc) + av);
       sLP[c] = half * xv * (fa.vp(j, wrapc(cc + 1, nu)) - av);
-      __syncthreads();
+      lds_barrier();
       // west face of the own cell: face q = c
       double fpw = 0.0;
       if (col) {
@@ -923,7 +1209,7 @@ __global__ __launch_bounds__(1024) void k_advection_band(AdvBandArgs a) {
         fpw = weno_face(sLN[c2], sLN[c1], sLN[cc], sLP[c1], sLP[cc], sLP[cp], w);
       }
       sFP[c] = fpw;
-      __syncthreads();
+      lds_barrier();
       double fpe = sFP[wrapc(cc + 1, nu)];
       double sti = m.st_i[j];
       double at_ = (m.sth[j + 1] * ft_dn - m.sth[j] * ft_up) * sti;
@@ -931,7 +1217,7 @@ __global__ __launch_bounds__(1024) void k_advection_band(AdvBandArgs a) {
       double ap_ = (fpe - fpw) * sti;
       ap_ = ap_ * m.dp_i[cc + 1];
       A = at_ + ap_;
-      __syncthreads();
+      lds_barrier();
     }
     if (col) {
       size_t off = (size_t)j * pitch + c;

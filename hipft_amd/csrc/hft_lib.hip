@@ -108,6 +108,7 @@ struct hft_ctx {
   // tuning (hft_set_tuning): fused multi-stage RKG2 launches
   int fused = 1, fused_T = 4, band_rows = 32, fused_min = 8;
   int fused_target_blocks = 0;  // shrink bands while fewer blocks than this (0: off)
+  int fused_kernel = 1;         // 1: k_rkg2_fused, 2: k_rkg2_march (lean bookkeeping)
   int adv_band = 1, adv_band_rows = 16;  // banded WENO3 stage kernel
   // hipGraph replay of RKG2 super-step launch sequences, keyed by
   // (use_af, realization -> stage count in launch order)
@@ -793,10 +794,18 @@ int superstep_launches(hft_ctx *c, int n, const std::vector<int> &ord,
         if (s[ord[z]] <= k + T - 1) final_loc[ord[z]] = oA;
       }
       int blocks = active * fa.nbands;
-      switch (T) {
-        case 2: rc = launch_ex(c, kGroupFused, k_rkg2_fused<2>, blocks, fa.ncol, shmem, fa, units); break;
-        case 3: rc = launch_ex(c, kGroupFused, k_rkg2_fused<3>, blocks, fa.ncol, shmem, fa, units); break;
-        default: rc = launch_ex(c, kGroupFused, k_rkg2_fused<4>, blocks, fa.ncol, shmem, fa, units); break;
+      if (c->fused_kernel == 2) {
+        switch (T) {
+          case 2: rc = launch_ex(c, kGroupFused, k_rkg2_march<2>, blocks, fa.ncol, shmem, fa, units); break;
+          case 3: rc = launch_ex(c, kGroupFused, k_rkg2_march<3>, blocks, fa.ncol, shmem, fa, units); break;
+          default: rc = launch_ex(c, kGroupFused, k_rkg2_march<4>, blocks, fa.ncol, shmem, fa, units); break;
+        }
+      } else {
+        switch (T) {
+          case 2: rc = launch_ex(c, kGroupFused, k_rkg2_fused<2>, blocks, fa.ncol, shmem, fa, units); break;
+          case 3: rc = launch_ex(c, kGroupFused, k_rkg2_fused<3>, blocks, fa.ncol, shmem, fa, units); break;
+          default: rc = launch_ex(c, kGroupFused, k_rkg2_fused<4>, blocks, fa.ncol, shmem, fa, units); break;
+        }
       }
       if (rc) return rc;
       pA = oA;
@@ -946,6 +955,9 @@ int hft_create(int ntm, int npm1, const double *t, const double *p, int nr, int 
     hipFuncSetAttribute((const void *)k_rkg2_fused<2>, hipFuncAttributeMaxDynamicSharedMemorySize, 2 * row);
     hipFuncSetAttribute((const void *)k_rkg2_fused<3>, hipFuncAttributeMaxDynamicSharedMemorySize, 3 * row);
     hipFuncSetAttribute((const void *)k_rkg2_fused<4>, hipFuncAttributeMaxDynamicSharedMemorySize, 4 * row);
+    hipFuncSetAttribute((const void *)k_rkg2_march<2>, hipFuncAttributeMaxDynamicSharedMemorySize, 2 * row);
+    hipFuncSetAttribute((const void *)k_rkg2_march<3>, hipFuncAttributeMaxDynamicSharedMemorySize, 3 * row);
+    hipFuncSetAttribute((const void *)k_rkg2_march<4>, hipFuncAttributeMaxDynamicSharedMemorySize, 4 * row);
     set_adjoint_attributes();
   }
   *out = c;
@@ -1616,6 +1628,9 @@ int hft_set_tuning(hft_ctx *c, const char *key, int value) {
   } else if (!strcmp(key, "fused_target_blocks")) {
     if (value < 0) return fail(HFT_E_ARG, "fused_target_blocks must be >= 0");
     c->fused_target_blocks = value;
+  } else if (!strcmp(key, "fused_kernel")) {
+    if (value < 1 || value > 2) return fail(HFT_E_ARG, "fused_kernel must be 1 or 2");
+    c->fused_kernel = value;
   } else if (!strcmp(key, "graphs")) {
     c->graphs_on = value != 0;
   } else if (!strcmp(key, "adjoint_segment")) {
@@ -1645,6 +1660,7 @@ int hft_get_tuning(hft_ctx *c, const char *key, int *value) {
   else if (!strcmp(key, "band_rows")) *value = c->band_rows;
   else if (!strcmp(key, "fused_min_realizations")) *value = c->fused_min;
   else if (!strcmp(key, "fused_target_blocks")) *value = c->fused_target_blocks;
+  else if (!strcmp(key, "fused_kernel")) *value = c->fused_kernel;
   else if (!strcmp(key, "graphs")) *value = c->graphs_on;
   else if (!strcmp(key, "graph_hits")) *value = (int)c->graph_hits;
   else if (!strcmp(key, "graph_builds")) *value = (int)c->graph_builds;

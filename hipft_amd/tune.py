@@ -20,11 +20,11 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CACHE = os.path.join(HERE, "tuning_cache.json")
 
 DIFF_KEYS = ("fused", "fused_stages", "band_rows", "fused_min_realizations",
-             "fused_target_blocks")
+             "fused_target_blocks", "fused_kernel")
 ADV_KEYS = ("advection_band", "advection_band_rows")
 KEYS = DIFF_KEYS + ADV_KEYS
 DEFAULTS = dict(fused=1, fused_stages=4, band_rows=32, fused_min_realizations=8,
-                fused_target_blocks=0, advection_band=1, advection_band_rows=16)
+                fused_target_blocks=0, fused_kernel=1, advection_band=1, advection_band_rows=16)
 
 
 def adv_candidates():
@@ -34,13 +34,15 @@ def adv_candidates():
 
 def candidates(nr):
     out = [dict(fused=0, fused_stages=4, band_rows=32, fused_min_realizations=1,
-                fused_target_blocks=0)]
-    for T in (3, 4):
-        for H in (32, 64, 128):
-            for m in sorted({1, 4, 8, 16, 32} & set(range(1, max(nr, 1) + 1))):
-                for B in (0, 256, 512):
-                    out.append(dict(fused=1, fused_stages=T, band_rows=H,
-                                    fused_min_realizations=m, fused_target_blocks=B))
+                fused_target_blocks=0, fused_kernel=1)]
+    for K in (1, 2):
+        for T in (3, 4):
+            for H in (32, 64, 128):
+                for m in sorted({1, 4, 8, 16, 32} & set(range(1, max(nr, 1) + 1))):
+                    for B in (0, 256):
+                        out.append(dict(fused=1, fused_stages=T, band_rows=H,
+                                        fused_min_realizations=m, fused_target_blocks=B,
+                                        fused_kernel=K))
     return out
 
 

@@ -282,9 +282,10 @@ def test_atten_map_vs_reference(gpu_available):
     assert err <= 1e-10, cols
 
 
+@pytest.mark.parametrize("kernel", [1, 2])
 @pytest.mark.parametrize("T,H,B", [(2, 4, 0), (3, 7, 0), (4, 32, 0), (4, 5, 0), (4, 32, 40),
                                    (3, 64, 1000)])
-def test_fused_rkg2_bitexact(gpu_available, T, H, B):
+def test_fused_rkg2_bitexact(gpu_available, T, H, B, kernel):
     """Several RKG2 stages per launch (LDS row rings, band halos) give the
     oracle's result bit for bit, across band boundaries and the poles, also
     when the bands thin out as realizations finish (B = target blocks)."""
@@ -293,13 +294,14 @@ def test_fused_rkg2_bitexact(gpu_available, T, H, B):
     o.load_diffusion(eta)
     g.set_diffusion(eta)
     g.set_tuning(fused=1, fused_stages=T, band_rows=H, fused_min_realizations=1,
-                 fused_target_blocks=B)
+                 fused_target_blocks=B, fused_kernel=kernel)
     o.diffusion_step(6.0, subcycles=0)
     g.diffusion_step(6.0, subcycles=0)
-    assert_same(g.get_state("tp"), o.get_f(), f"fused T={T} H={H} B={B}")
+    assert_same(g.get_state("tp"), o.get_f(), f"fused kernel {kernel} T={T} H={H} B={B}")
 
 
-def test_fused_matches_single_stage_full_size(gpu_available):
+@pytest.mark.parametrize("kernel", [1, 2])
+def test_fused_matches_single_stage_full_size(gpu_available, kernel):
     """512x1024 batch, independent stage counts: fused and one-stage-per-launch
     paths agree bit for bit."""
     t, p = grid(512, 1024)
@@ -311,7 +313,8 @@ def test_fused_matches_single_stage_full_size(gpu_available):
         g = FluxTransport(t, p, nr, mode="independent")
         g.set_state(maps, layout="pt")
         g.set_diffusion(eta)
-        g.set_tuning(fused=fused, fused_min_realizations=1, fused_target_blocks=512 * fused)
+        g.set_tuning(fused=fused, fused_min_realizations=1, fused_target_blocks=512 * fused,
+                     fused_kernel=kernel)
         g.diffusion_step(1.0, subcycles=0)
         out.append(g.get_state("tp"))
         s, _ = g.diffusion_stats()
