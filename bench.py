@@ -80,32 +80,54 @@ def time_steps(m, k):
     return time.perf_counter() - t0
 
 
-def cpu_baseline(wl, budget_s=15.0, max_steps=20):
-    """The oracle (CPU restatement, 1 core) on realization 0 of the workload."""
-    from oracle.oracle import Oracle
-    o = Oracle(wl["t"], wl["p"], 1)
-    o.ingest_pt(wl["maps"][0])
-    nt = len(wl["t"]) + 1
-    vt = np.ascontiguousarray(np.broadcast_to(wl["vt"][0][None, None, :], (1, o.npm, nt)))
-    o.flow_reset()
-    o.flow_set(vt, np.zeros((1, o.np_, o.ntm)))
-    o.flow_add_dr(*[[c] for c in wl["dr"][0]])
-    o.set_lf_alpha()
-    o.load_diffusion([wl["eta"][0]])
-    n = 0
-    t0 = time.perf_counter()
-    while n < max_steps:
-        o.advection_step(0.5)
-        o.diffusion_step(1.0, 0)
-        o.advection_step(0.5)
-        o.analysis()
-        n += 1
-        if time.perf_counter() - t0 > budget_s:
-            break
-    el = time.perf_counter() - t0
-    return dict(value=n / el, unit="realization-steps/s", cores=1, kind="port",
-                sample=f"{n} steps of realization 0 (eta={wl['eta'][0]:.1f} km^2/s) of the "
-                       f"512x1024 workload on one host core, CPU oracle (C restatement)")
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(wl, budget_s=12.0, max_steps=20):
+    """The oracle (CPU restatement, OpenMP over all host threads, and one
+    thread) on realization 0 of the workload."""
+    from oracle.oracle import Oracle, get_threads, set_threads
+    nthreads = get_threads()
+
+    def run(threads):
+        set_threads(threads)
+        o = Oracle(wl["t"], wl["p"], 1)
+        o.ingest_pt(wl["maps"][0])
+        nt = len(wl["t"]) + 1
+        vt = np.ascontiguousarray(np.broadcast_to(wl["vt"][0][None, None, :], (1, o.npm, nt)))
+        o.flow_reset()
+        o.flow_set(vt, np.zeros((1, o.np_, o.ntm)))
+        o.flow_add_dr(*[[c] for c in wl["dr"][0]])
+        o.set_lf_alpha()
+        o.load_diffusion([wl["eta"][0]])
+        n = 0
+        t0 = time.perf_counter()
+        while n < max_steps:
+            o.advection_step(0.5)
+            o.diffusion_step(1.0, 0)
+            o.advection_step(0.5)
+            o.analysis()
+            n += 1
+            if time.perf_counter() - t0 > budget_s:
+                break
+        return n, time.perf_counter() - t0
+
+    n_all, el_all = run(nthreads)
+    n_one, el_one = run(1)
+    set_threads(nthreads)
+    return dict(value=n_all / el_all, unit="realization-steps/s", cores=nthreads, kind="port",
+                sample=f"{n_all} steps of realization 0 (eta={wl['eta'][0]:.1f} km^2/s) of the "
+                       f"512x1024 workload, CPU oracle (C restatement, OpenMP over points) on "
+                       f"{nthreads} threads of {os.cpu_count()} CPUs ({_cpu_model()})",
+                single_core=dict(value=n_one / el_one, cores=1, steps=n_one))
 
 
 def adjoint_bench(nr=16, ntm=256, npm1=512, nsteps=200, every=20, seed=77):

@@ -21,6 +21,9 @@
 #include "hft_oracle.h"
 
 #include <math.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <stdlib.h>
 #include <string.h>
 
@@ -243,6 +246,22 @@ int orc_mesh(const orc *o, int which, double *out) {
   if (which < 0 || which > 17) return -1;
   memcpy(out, src[which] + 1, sizeof(double) * (size_t)len[which]);
   return len[which];
+}
+
+void orc_set_threads(int n) {
+#ifdef _OPENMP
+  omp_set_num_threads(n > 0 ? n : 1);
+#else
+  (void)n;
+#endif
+}
+
+int orc_get_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
 }
 
 int orc_weno(const orc *o, int which, double *out) {
@@ -483,6 +502,7 @@ void orc_advection_operator(orc *o, const double *x, double *aop) {
 #define L2(a, j, k) (a)[((j) - 1) + (size_t)ntm * (k)] /* k = 0..npm */
   for (int i = 1; i <= nr; i++) {
     /* theta: LF split 7766-7769 */
+    #pragma omp parallel for schedule(static)
     for (int k = 1; k <= npm; k++)
       for (int j = 1; j <= ntm; j++) {
         double f = FF(o, x, j, k, i), a = FF(o, o->alpha_t, j, k, i);
@@ -490,6 +510,7 @@ void orc_advection_operator(orc *o, const double *x, double *aop) {
         L2(LN, j, k) = half * f * (VT(o, j, k, i) + a);
       }
     const double *CP = o->D_C_CPt, *MC = o->D_C_MCt;
+    #pragma omp parallel for schedule(static)
     for (int k = 1; k <= npm; k++)
       for (int j = 3; j <= nt - 2; j++) { /* 7773-7803 */
         double LNm2 = L2(LN, j - 2, k), LNm1 = L2(LN, j - 1, k), LN0 = L2(LN, j, k);
@@ -527,6 +548,7 @@ void orc_advection_operator(orc *o, const double *x, double *aop) {
                                   (one + cct) * FF(o, x, ntm - 1, k, i));
     }
     /* phi: LF split 7833-7845 */
+    #pragma omp parallel for schedule(static)
     for (int k = 1; k <= npm - 1; k++)
       for (int j = 2; j <= ntm - 1; j++) {
         double f = FF(o, x, j, k, i), a = FF(o, o->alpha_p, j, k, i);
@@ -540,6 +562,7 @@ void orc_advection_operator(orc *o, const double *x, double *aop) {
       L2(LN, j, npm) = L2(LN, j, 2);
     }
     const double *CPp = o->D_C_CPp, *MCp = o->D_C_MCp;
+    #pragma omp parallel for schedule(static)
     for (int k = 2; k <= o->npm1; k++)
       for (int j = 2; j <= ntm - 1; j++) { /* 7850-7880 */
         double LNm2 = L2(LN, j, k - 2), LNm1 = L2(LN, j, k - 1), LN0 = L2(LN, j, k);
@@ -569,6 +592,7 @@ void orc_advection_operator(orc *o, const double *x, double *aop) {
       FPX(j, 1) = FPX(j, np - 1);
       FPX(j, np) = FPX(j, 2);
     }
+    #pragma omp parallel for schedule(static)
     for (int k = 2; k <= npm - 1; k++) /* 7890-7898 */
       for (int j = 2; j <= ntm - 1; j++) {
         double a = (o->sth[j + 1] * FT(j + 1, k) - o->sth[j] * FT(j, k)) * o->st_i[j] * o->dt_i[j];
@@ -599,12 +623,16 @@ void orc_advection_step(orc *o, double dtl) {
   double *aop = (double *)malloc(sizeof(double) * n);
   double *f = o->f;
   orc_advection_operator(o, f, aop);
+  #pragma omp parallel for schedule(static)
   for (size_t q = 0; q < n; q++) f1[q] = f[q] - half * dtl * aop[q];
   orc_advection_operator(o, f1, aop);
+  #pragma omp parallel for schedule(static)
   for (size_t q = 0; q < n; q++) f1[q] = f1[q] - half * dtl * aop[q];
   orc_advection_operator(o, f1, aop);
+  #pragma omp parallel for schedule(static)
   for (size_t q = 0; q < n; q++) f1[q] = two_third * f[q] + third * f1[q] - sixth * dtl * aop[q];
   orc_advection_operator(o, f1, aop);
+  #pragma omp parallel for schedule(static)
   for (size_t q = 0; q < n; q++) f[q] = f1[q] - half * dtl * aop[q];
   free(f1);
   free(aop);
@@ -664,6 +692,7 @@ void orc_diffusion_operator(orc *o, const double *x, double *y) {
   int ntm = o->ntm, npm = o->npm, nt = o->nt, nr = o->nr;
   double *pv = (double *)malloc(sizeof(double) * (size_t)npm);
   for (int i = 1; i <= nr; i++) {
+    #pragma omp parallel for schedule(static)
     for (int k = 2; k <= npm - 1; k++)
       for (int j = 2; j <= ntm - 1; j++)
         FF(o, y, j, k, i) = COEF(o, j, k, 1, i) * FF(o, x, j, k - 1, i) +
@@ -766,6 +795,7 @@ long orc_diffusion_sts(orc *o, double dtl) {
   double *ukm2 = malloc(sizeof(double) * n);
   double *f = o->f;
   orc_diffusion_operator(o, f, dty0);
+  #pragma omp parallel for schedule(static)
   for (size_t q = 0; q < n; q++) {
     u0[q] = f[q];
     ukm2[q] = f[q];
@@ -775,6 +805,7 @@ long orc_diffusion_sts(orc *o, double dtl) {
   for (long k = 2; k <= s; k++) {
     orc_diffusion_operator(o, ukm1, ykm1);
     double a = uj[k - 1], b = vj[k - 1], c = ubj[k - 1], g = gj[k - 1];
+    #pragma omp parallel for schedule(static)
     for (size_t q = 0; q < n; q++) {
       f[q] = a * ukm1[q] + b * ukm2[q] + (one - a - b) * u0[q] + c * dtl * ykm1[q] + g * dty0[q];
       ukm2[q] = ukm1[q];

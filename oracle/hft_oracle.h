@@ -31,6 +31,9 @@ int orc_mesh(const orc *o, int which, double *out);
  * coef(2:ntm-1,2:npm-1,5,nr), and the pole factors (bc_flow n/s, bc_diff
  * n/s). */
 int orc_weno(const orc *o, int which, double *out);
+/* OpenMP threads of the loops over points (results do not depend on it). */
+void orc_set_threads(int n);
+int orc_get_threads(void);
 void orc_alpha(const orc *o, double *alpha_t, double *alpha_p);
 void orc_coef(const orc *o, double *coef);
 void orc_pole_fac(const orc *o, double *out4);

@@ -48,6 +48,8 @@ def lib():
         L.orc_dims.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]
         L.orc_mesh.argtypes = [ctypes.c_void_p, ctypes.c_int, _D]
         L.orc_weno.argtypes = [ctypes.c_void_p, ctypes.c_int, _D]
+        L.orc_set_threads.argtypes = [ctypes.c_int]
+        L.orc_get_threads.restype = ctypes.c_int
         L.orc_alpha.argtypes = [ctypes.c_void_p, _D, _D]
         L.orc_coef.argtypes = [ctypes.c_void_p, _D]
         L.orc_pole_fac.argtypes = [ctypes.c_void_p, _D]
@@ -276,6 +278,15 @@ class Oracle:
         lib().orc_assimilate(self._h, _p(d), int(custom_from_mu), _p(_vec(mu_power, n)),
                              _p(_vec(mu_limit, n)), _p(_vec(lat_limit, n)), float(mult_fac),
                              int(add_mode), int(balance))
+
+
+def set_threads(n):
+    """OpenMP threads of the oracle's loops over points."""
+    lib().orc_set_threads(int(n))
+
+
+def get_threads():
+    return lib().orc_get_threads()
 
 
 def sts_rkg2(dt, dt_euler):

@@ -319,3 +319,27 @@ def test_tesseral_harmonic_closed_form():
     # fully normalised real harmonic: coefficient sqrt(2)*sqrt(13/(4pi)*1/(11!))*(10395)
     norm = math.sqrt(2 * 13 / (4 * PI) / math.factorial(11)) * 10395.0
     assert abs(abs(ratio[0]) - norm) < 1e-12 * norm
+
+
+def test_thread_count_does_not_change_results():
+    """The oracle's OpenMP loops run over independent points only: one and
+    several threads give identical bits for a Strang step with PTL."""
+    from oracle import oracle as O
+    from hipft_amd import workloads as W
+    t, p = W.clustered_theta(41, 2.0), W.uniform_phi(81)
+    out = []
+    n0 = O.get_threads()
+    for n in (1, 4):
+        O.set_threads(n)
+        o = O.Oracle(t, p, 2)
+        o.ingest_pt(np.stack([W.bipole_map(t, p, 10, 5 + r) for r in range(2)]))
+        o.flow_reset()
+        o.flow_add_dr(*[[c, c * 1.05] for c in W.dr_coefficients()])
+        o.set_lf_alpha()
+        o.load_diffusion([300.0, 700.0])
+        o.advection_step(0.5)
+        o.diffusion_step(1.0, subcycles=0)
+        o.advection_step(0.5)
+        out.append(o.get_f())
+    O.set_threads(n0)
+    assert np.array_equal(out[0], out[1])
