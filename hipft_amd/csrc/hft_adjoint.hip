@@ -36,10 +36,9 @@ enum {
   AB_DTY = 12,   // dty0 of the cycle
   AB_LB = 13,    // 3: rolling RKG2 stage adjoints
   AB_L0 = 16,
-  AB_M = 17,
-  AB_R = 18,     // 3: rolling stage states of the checkpoint pass
-  AB_U1 = 21,
-  AB_COUNT = 22
+  AB_R = 17,     // 3: rolling stage states of the checkpoint pass
+  AB_U1 = 20,
+  AB_COUNT = 21
 };
 
 int fill_dev(hft_ctx *c, double *p, size_t n, double v) {
@@ -175,8 +174,6 @@ int cycle_adjoint(hft_ctx *c, const CycleRec &rec, const double *U0, double *lam
     if (!(LB[i] = adj_buf(c, c->adj_fix, AB_LB + i, &rc))) return rc;
   double *L0 = adj_buf(c, c->adj_fix, AB_L0, &rc);
   if (!L0) return rc;
-  double *M = adj_buf(c, c->adj_fix, AB_M, &rc);
-  if (!M) return rc;
   bool stored = !rec.W.empty();
   // segment length (one segment covering every stage when they are stored)
   int64_t K = stored ? smax - 1 : c->adj_seg;
@@ -247,15 +244,15 @@ int cycle_adjoint(hft_ctx *c, const CycleRec &rec, const double *U0, double *lam
       }
     }
   }
-  // adjoint initial values: lam_{s_r} = lam_r, partial lam_{s_r - 1} = 0, L0 = M = 0
+  // adjoint initial values: lam_{s_r} = lam_r, adjoint of u0 = 0
   dim3 gi((unsigned)((c->rstride + kBlock - 1) / kBlock), (unsigned)n);
   hipLaunchKernelGGL(k_adj_cycle_init, gi, dim3(kBlock), 0, c->stream, LB[0], LB[1], LB[2], L0,
-                     M, (const double *)lam, (const int *)c->d_order, (const int64_t *)c->d_s,
+                     (const double *)lam, (const int *)c->d_order, (const int64_t *)c->d_s,
                      c->rstride);
   HIPCHK(hipGetLastError());
   AdjStageArgs ja;
-  ja.m = c->m; ja.L0 = L0; ja.M = M; ja.D = c->d_D; ja.ub1 = c->d_ub1; ja.order = c->d_order;
-  ja.rstride = c->rstride; ja.part = c->d_apD;
+  ja.m = c->m; ja.L0 = L0; ja.u0 = U0; ja.D = c->d_D; ja.s = c->d_s; ja.dt = c->d_cdt;
+  ja.order = c->d_order; ja.rstride = c->rstride; ja.part = c->d_apD;
   for (int64_t i = nseg - 1; i >= 0; i--) {
     int64_t k0 = 2 + i * K, k1 = std::min<int64_t>(k0 + K - 1, smax);
     // W[t] = u_{k0-2+t}
@@ -278,17 +275,19 @@ int cycle_adjoint(hft_ctx *c, const CycleRec &rec, const double *U0, double *lam
       ja.g = tiles_for(c, na);
       ja.k = (int)k;
       ja.lam = LB[k % 3];
-      ja.P = LB[(k - 1) % 3];
-      ja.Pn = LB[(k - 2) % 3];
+      ja.lamn = LB[(k + 1) % 3];
+      ja.out = LB[(k - 1) % 3];
       ja.u = W[k - k0 + 1];
       ja.coef = c->d_coef + (size_t)k * nr;
-      if ((rc = launch(c, kGroupAdjoint, k_rkg2_adj_stage, nblocks(ja.g), ja, 9.0 * na))) return rc;
+      ja.coefn = c->d_coef + (size_t)(k + 1) * nr;
+      if ((rc = launch(c, kGroupAdjoint, k_rkg2_adj_stage, nblocks(ja.g), ja, 7.0 * na))) return rc;
     }
   }
   AdjCloseArgs ca;
-  ca.m = c->m; ca.g = tiles_for(c, n); ca.M = M; ca.L0 = L0; ca.u0 = U0; ca.D = c->d_D;
-  ca.dt = c->d_cdt; ca.order = c->d_order; ca.rstride = c->rstride; ca.part = c->d_apD;
-  if ((rc = launch(c, kGroupAdjoint, k_rkg2_adj_close, nblocks(ca.g), ca, 4.0 * n))) return rc;
+  ca.m = c->m; ca.g = tiles_for(c, n); ca.lam1 = LB[1]; ca.L0 = L0; ca.u0 = U0; ca.D = c->d_D;
+  ca.dt = c->d_cdt; ca.ub1 = c->d_ub1; ca.order = c->d_order; ca.rstride = c->rstride;
+  ca.part = c->d_apD;
+  if ((rc = launch(c, kGroupAdjoint, k_rkg2_adj_close, nblocks(ca.g), ca, 3.0 * n))) return rc;
   hipLaunchKernelGGL(k_copy_reals, gi, dim3(kBlock), 0, c->stream, lam, (const double *)L0,
                      (const int *)c->d_order, c->rstride);
   HIPCHK(hipGetLastError());

@@ -117,24 +117,31 @@ __device__ inline void pole_sums(const Mesh &m, const double *lam, int j, double
   }
 }
 
-// RKG2 adjoint of stage k >= 2 (forward: u_k = a u_{k-1} + b u_{k-2} +
-// c u0 + d L u_{k-1} + g dty0).  With lam = adjoint of u_k:
-//   P   <- P + a lam + d L^T lam       (adjoint of u_{k-1}, now complete)
-//   Pn  <- b lam                        (first part of the adjoint of u_{k-2})
-//   L0  += c lam ; M += g lam           (adjoints of u0 and dty0)
-// k == 2 (u_{k-2} = u0) also closes stage 1 (u1 = u0 + ub1 dty0):
-//   L0 += b lam + lam_1 ; M += ub1 lam_1.
-// part[slot] += d <L^T lam, u_{k-1}>, the diffusivity log-derivative, one
-// slot per tile (no atomics, fixed order).
+// RKG2 adjoint of stage k >= 2.  Forward: u_k = a_k u_{k-1} + b_k u_{k-2} +
+// c_k u0 + d_k L u_{k-1} + g_k dty0, u_1 = u0 + ub_1 dty0, dty0 = dt L u0.
+// With lam_j the adjoint of u_j, stage k (descending) forms
+//   lam_{k-1} = b_{k+1} lam_{k+1} + a_k lam_k + d_k L^T lam_k
+// (b_{k+1} lam_{k+1} is absent at k = s), and adds to the adjoint of u0
+//   c_k lam_k + g_k dt L^T lam_k
+// (the dty0 adjoint sum_k g_k lam_k + ub_1 lam_1 goes through dt L^T at
+// once, since L^T lam_k is formed here anyway).  k == 2 also adds b_2 lam_2
+// and lam_1 (u_0 = u0 and u_1 = u0 + ...); k_rkg2_adj_close adds
+// ub_1 dt L^T lam_1.  part[slot] += <L^T lam_k, d_k u_{k-1} + g_k dt u0>,
+// the diffusivity log-derivative, one slot per tile (fixed order, no atomics).
 struct AdjStageArgs {
   Mesh m;
   Tiles g;
-  const double *lam;
-  double *P, *Pn, *L0, *M;
-  const double *u;  // u_{k-1}
+  const double *lam;   // lam_k
+  const double *lamn;  // lam_{k+1}
+  double *out;         // lam_{k-1}
+  double *L0;          // adjoint of u0
+  const double *u;     // u_{k-1}
+  const double *u0;
   const double *D;
-  const StageCoef *coef;  // stage k, [nr]
-  const double *ub1;      // [nr], k == 2 only
+  const StageCoef *coef;   // stage k, [nr]
+  const StageCoef *coefn;  // stage k+1, [nr] (read where k < s_r)
+  const int64_t *s;
+  const double *dt;        // [nr] cycle time step
   const int *order;
   int k;
   size_t rstride;
@@ -163,19 +170,14 @@ __global__ __launch_bounds__(kBlock) void k_rkg2_adj_stage(AdjStageArgs a) {
     size_t p = base + (size_t)j * m.pitch + c;
     const StageCoef cf = a.coef[r];
     double lk = lam[(size_t)j * m.pitch + c];
-    double l1 = a.P[p] + cf.a * lk + cf.d * lt;
-    a.P[p] = l1;
-    double l0 = a.L0[p] + cf.c * lk;
-    double mm = a.M[p] + cf.g * lk;
-    if (a.k == 2) {
-      l0 = l0 + cf.b * lk + l1;
-      mm = mm + a.ub1[r] * l1;
-    } else {
-      a.Pn[p] = cf.b * lk;
-    }
+    double lp = cf.a * lk + cf.d * lt;
+    if (a.k < a.s[r]) lp = lp + a.coefn[r].b * a.lamn[p];
+    a.out[p] = lp;
+    double gdt = cf.g * a.dt[r];
+    double l0 = a.L0[p] + cf.c * lk + gdt * lt;
+    if (a.k == 2) l0 = l0 + cf.b * lk + lp;
     a.L0[p] = l0;
-    a.M[p] = mm;
-    e = cf.d * lt * a.u[p];
+    e = lt * (cf.d * a.u[p] + gdt * a.u0[p]);
   }
   double s = block_sum(sh, e);
   if (threadIdx.x == 0) {
@@ -184,16 +186,17 @@ __global__ __launch_bounds__(kBlock) void k_rkg2_adj_stage(AdjStageArgs a) {
   }
 }
 
-// Closing step of the stage-1 adjoint: dty0 = dt L u0, so
-//   L0 += dt L^T M,  part += dt <L^T M, u0>.
+// Closing step: the dty0 adjoint's ub_1 lam_1 term, L0 += ub_1 dt L^T lam_1,
+// part += ub_1 dt <L^T lam_1, u0>.
 struct AdjCloseArgs {
   Mesh m;
   Tiles g;
-  const double *M;
+  const double *lam1;
   double *L0;
   const double *u0;
   const double *D;
-  const double *dt;  // [nr] cycle time step
+  const double *dt;   // [nr] cycle time step
+  const double *ub1;  // [nr]
   const int *order;
   size_t rstride;
   double *part;
@@ -211,16 +214,16 @@ __global__ __launch_bounds__(kBlock) void k_rkg2_adj_close(AdjCloseArgs a) {
   bool col = c < m.nu;
   int cc = col ? c : 0;
   size_t base = (size_t)r * a.rstride;
-  const double *Mr = a.M + base;
+  const double *l1 = a.lam1 + base;
   double Ln, Ls;
-  pole_sums(m, Mr, j, Ln, Ls);
-  double lt = lap_T(m, a.D[r], Mr, j, cc, Ln, Ls);
+  pole_sums(m, l1, j, Ln, Ls);
+  double lt = lap_T(m, a.D[r], l1, j, cc, Ln, Ls);
   double e = 0.0;
   if (col) {
     size_t p = base + (size_t)j * m.pitch + c;
-    double dt = a.dt[r];
-    a.L0[p] = a.L0[p] + dt * lt;
-    e = dt * lt * a.u0[p];
+    double w = a.ub1[r] * a.dt[r];
+    a.L0[p] = a.L0[p] + w * lt;
+    e = w * lt * a.u0[p];
   }
   double s = block_sum(sh, e);
   if (threadIdx.x == 0) {
@@ -455,21 +458,18 @@ __global__ __launch_bounds__(1024) void k_advection_vjp_band(AdvVjpArgs a) {
 }
 
 // Start of a cycle adjoint for the realizations ord[0..n): lam_{s_r} = lam,
-// partial lam_{s_r - 1} = 0, adjoints of u0 and dty0 = 0.
+// adjoint of u0 = 0.
 __global__ __launch_bounds__(kBlock) void k_adj_cycle_init(double *LB0, double *LB1, double *LB2,
-                                                           double *L0, double *M,
-                                                           const double *lam, const int *order,
-                                                           const int64_t *s, size_t rstride) {
+                                                           double *L0, const double *lam,
+                                                           const int *order, const int64_t *s,
+                                                           size_t rstride) {
   size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
   if (i >= rstride) return;
   int r = order[blockIdx.y];
   size_t p = (size_t)r * rstride + i;
   double *LB[3] = {LB0, LB1, LB2};
-  int64_t sr = s[r];
-  LB[sr % 3][p] = lam[p];
-  LB[(sr - 1) % 3][p] = 0.0;
+  LB[s[r] % 3][p] = lam[p];
   L0[p] = 0.0;
-  M[p] = 0.0;
 }
 
 // dst = src for the realizations ord[0..n).
