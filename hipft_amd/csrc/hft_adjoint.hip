@@ -136,20 +136,45 @@ int superstep_store(hft_ctx *c, const std::vector<int> &reals, const std::vector
   a1.D = c->d_D; a1.dt = c->d_cdt; a1.ub1 = c->d_ub1; a1.order = c->d_order;
   a1.Af = use_af ? c->dty0 : nullptr; a1.rstride = c->rstride;
   if ((rc = launch(c, 0, k_rkg2_stage1, nblocks(a1.g), a1, 3.0 * n))) return rc;
+  if (fused_usable(c) && !c->buf2) {
+    if ((rc = dalloc(&c->buf2, c->nfield)) || (rc = dalloc(&c->buf3, c->nfield))) return rc;
+  }
+  bool fused_ok = fused_usable(c) && c->buf2;
+  std::vector<double *> final_loc(nr, nullptr);
   StageArgs sa;
   sa.m = c->m; sa.f = c->f; sa.dty0 = DTY; sa.D = c->d_D; sa.s = c->d_s; sa.order = c->d_order;
   sa.rstride = c->rstride;
   int active = n;
-  for (int64_t k = 2; k <= smax; k++) {
+  int64_t k = 2;
+  while (k <= smax) {
     while (active > 0 && s[ord[active - 1]] < k) active--;
-    sa.g = tiles_for(c, active);
-    sa.k = (int)k;
-    sa.x = rec.W[k - 1];
-    sa.ukm2 = k == 2 ? c->f : rec.W[k - 2];
-    sa.out = k < smax ? rec.W[k] : c->buf0;  // stage smax finishes every active realization
-    sa.coef = c->d_coef + (size_t)k * nr;
-    if ((rc = launch(c, 0, k_rkg2_stage, nblocks(sa.g), sa, 5.0 * active))) return rc;
+    const double *pA = rec.W[k - 1];
+    const double *pB = k == 2 ? c->f : rec.W[k - 2];
+    int T = c->fused_T;
+    if (fused_ok && active >= c->fused_min) {
+      // several stages per launch, each stage's rows also kept in rec.W
+      double *Wt[4] = {nullptr, nullptr, nullptr, nullptr};
+      for (int t = 0; t < T; t++)
+        if (k + t < smax) Wt[t] = rec.W[k + t];
+      if ((rc = launch_fused(c, k, active, ord, s, pA, pB, DTY, c->buf0, c->buf2, Wt))) return rc;
+      for (int z = 0; z < active; z++)
+        if (s[ord[z]] <= k + T - 1) final_loc[ord[z]] = c->buf0;
+      k += T;
+    } else {
+      sa.g = tiles_for(c, active);
+      sa.k = (int)k;
+      sa.x = pA;
+      sa.ukm2 = pB;
+      sa.out = k < smax ? rec.W[k] : c->buf1;  // stage smax finishes every active realization
+      sa.coef = c->d_coef + (size_t)k * nr;
+      if ((rc = launch(c, 0, k_rkg2_stage, nblocks(sa.g), sa, 5.0 * active))) return rc;
+      k += 1;
+    }
   }
+  for (int r = 0; r < nr; r++)
+    if (final_loc[r])
+      HIPCHK(hipMemcpyAsync(c->f + (size_t)r * c->rstride, final_loc[r] + (size_t)r * c->rstride,
+                            c->rstride * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
   return 0;
 }
 

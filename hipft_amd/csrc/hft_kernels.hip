@@ -240,6 +240,7 @@ struct FusedArgs {
   const int64_t *s;
   const int *order;
   size_t rstride;
+  double *W[4];        // optional: stage k0+t-1 rows into W[t-1] (null: not kept)
 };
 
 constexpr int kFusedMaxRows = 160;  // H + 2T rows of per-row coefficients
@@ -261,7 +262,7 @@ __device__ __forceinline__ void fused_step(const FusedArgs &a, FusedState<T> &S,
                                            const double *B, const double *U0,
                                            const double *DT, double *OA, double *OB,
                                            double u0_pole, double dt_pole,
-                                           const double (*cfs)[5]) {
+                                           const double (*cfs)[5], double *const *Wst) {
   const Mesh &m = a.m;
   const int ncol = a.ncol, nu = m.nu, ntm = m.ntm, pitch = m.pitch;
   // slot of age d at this sub-step: (U - d) & 3
@@ -325,6 +326,7 @@ __device__ __forceinline__ void fused_step(const FusedArgs &a, FusedState<T> &S,
       if (col && 0 >= j0 && 0 < j1) {
         if (t == nst) OA[c] = vv;
         if (!final_here && t == T - 1) OB[c] = vv;
+        if (Wst[t - 1]) Wst[t - 1][c] = vv;
       }
     }
     if (act && R > 0) {
@@ -366,6 +368,7 @@ __device__ __forceinline__ void fused_step(const FusedArgs &a, FusedState<T> &S,
         size_t o = (size_t)R * pitch + c;
         if (t == nst) OA[o] = v;
         if (!final_here && t == T - 1) OB[o] = v;
+        if (Wst[t - 1]) Wst[t - 1][o] = v;
       }
     }
     if (t < T) {
@@ -398,6 +401,9 @@ __global__ __launch_bounds__(1024) void k_rkg2_fused(FusedArgs a) {
   size_t base = (size_t)r * a.rstride;
   const double *A = a.A + base, *B = a.B + base, *U0 = a.f + base, *DT = a.dty0 + base;
   double *OA = a.outA + base, *OB = a.outB + base;
+  double *Wst[T];
+#pragma unroll
+  for (int t = 0; t < T; t++) Wst[t] = a.W[t] ? a.W[t] + base : nullptr;
   double D = a.D[r];
   double D2 = D + D;
   double P1 = (D * m.dph_i[cc + 1]) * m.dp_i[cc + 1];
@@ -469,16 +475,16 @@ __global__ __launch_bounds__(1024) void k_rkg2_fused(FusedArgs a) {
   lds_barrier();
   for (int i = 0; i < nsteps; i += 4) {
     fused_step<T, 0>(a, S, lds, rowc, i, nsteps, lo0, hi0, j0, j1, nst, final_here, col, c, cw,
-                     ce, D2, P1, P5, A, B, U0, DT, OA, OB, u0_pole, dt_pole, cfs);
+                     ce, D2, P1, P5, A, B, U0, DT, OA, OB, u0_pole, dt_pole, cfs, Wst);
     if (i + 1 < nsteps)
       fused_step<T, 1>(a, S, lds, rowc, i + 1, nsteps, lo0, hi0, j0, j1, nst, final_here, col,
-                       c, cw, ce, D2, P1, P5, A, B, U0, DT, OA, OB, u0_pole, dt_pole, cfs);
+                       c, cw, ce, D2, P1, P5, A, B, U0, DT, OA, OB, u0_pole, dt_pole, cfs, Wst);
     if (i + 2 < nsteps)
       fused_step<T, 2>(a, S, lds, rowc, i + 2, nsteps, lo0, hi0, j0, j1, nst, final_here, col,
-                       c, cw, ce, D2, P1, P5, A, B, U0, DT, OA, OB, u0_pole, dt_pole, cfs);
+                       c, cw, ce, D2, P1, P5, A, B, U0, DT, OA, OB, u0_pole, dt_pole, cfs, Wst);
     if (i + 3 < nsteps)
       fused_step<T, 3>(a, S, lds, rowc, i + 3, nsteps, lo0, hi0, j0, j1, nst, final_here, col,
-                       c, cw, ce, D2, P1, P5, A, B, U0, DT, OA, OB, u0_pole, dt_pole, cfs);
+                       c, cw, ce, D2, P1, P5, A, B, U0, DT, OA, OB, u0_pole, dt_pole, cfs, Wst);
   }
 }
 

@@ -736,6 +736,52 @@ int rkg2_superstep(hft_ctx *c, const std::vector<int> &reals, const std::vector<
   return 0;
 }
 
+// One fused launch: stages k..k+T-1 of the first `active` realizations of
+// ord, reading u_{k-1} (pA) and u_{k-2} (pB); the last stage goes to oA (or
+// each realization's final stage, if earlier), the one before to oB, and
+// stage k+t-1 also to W[t] where W[t] is not null.
+int launch_fused(hft_ctx *c, int64_t k, int active, const std::vector<int> &ord,
+                 const std::vector<int64_t> &s, const double *pA, const double *pB,
+                 const double *dty0, double *oA, double *oB, double *const W[4]) {
+  int nr = c->nr, T = c->fused_T;
+  FusedArgs fa;
+  fa.m = c->m;
+  fa.H = c->band_rows;
+  if (c->fused_target_blocks > 0 &&
+      (int64_t)active * ((c->ntm + fa.H - 1) / fa.H) < c->fused_target_blocks) {
+    // few realizations left: thinner bands so the launch still fills the GPU
+    int64_t h = (int64_t)c->ntm * active / c->fused_target_blocks;
+    fa.H = (int)std::max<int64_t>(4, std::min<int64_t>(fa.H, h));
+  }
+  fa.nbands = (c->ntm + fa.H - 1) / fa.H;
+  fa.k0 = (int)k;
+  fa.ncol = ((c->nu + 63) / 64) * 64;
+  fa.A = pA; fa.B = pB; fa.f = c->f; fa.dty0 = dty0; fa.outA = oA; fa.outB = oB;
+  fa.D = c->d_D; fa.coef = c->d_coef; fa.nr_table = nr; fa.s = c->d_s; fa.order = c->d_order;
+  fa.rstride = c->rstride;
+  for (int t = 0; t < 4; t++) fa.W[t] = W[t];
+  size_t shmem = (size_t)T * 3 * fa.ncol * sizeof(double);
+  double units = 0;
+  for (int z = 0; z < active; z++) {
+    int64_t st = std::min<int64_t>(T, s[ord[z]] - k + 1);
+    units += 5.0 * st;  // algorithmic passes: as if stage by stage
+  }
+  int blocks = active * fa.nbands;
+  bool store = W[0] || W[1] || W[2] || W[3];
+  if (c->fused_kernel == 2 && !store) {
+    switch (T) {
+      case 2: return launch_ex(c, kGroupFused, k_rkg2_march<2>, blocks, fa.ncol, shmem, fa, units);
+      case 3: return launch_ex(c, kGroupFused, k_rkg2_march<3>, blocks, fa.ncol, shmem, fa, units);
+      default: return launch_ex(c, kGroupFused, k_rkg2_march<4>, blocks, fa.ncol, shmem, fa, units);
+    }
+  }
+  switch (T) {
+    case 2: return launch_ex(c, kGroupFused, k_rkg2_fused<2>, blocks, fa.ncol, shmem, fa, units);
+    case 3: return launch_ex(c, kGroupFused, k_rkg2_fused<3>, blocks, fa.ncol, shmem, fa, units);
+    default: return launch_ex(c, kGroupFused, k_rkg2_fused<4>, blocks, fa.ncol, shmem, fa, units);
+  }
+}
+
 // The launches of one RKG2 super-step (tables already uploaded).
 int superstep_launches(hft_ctx *c, int n, const std::vector<int> &ord,
                        const std::vector<int64_t> &s, bool use_af) {
@@ -771,43 +817,11 @@ int superstep_launches(hft_ctx *c, int n, const std::vector<int> &ord,
     if (fused_ok && active >= c->fused_min) {
       double *oA = pick_free(nullptr, nullptr);
       double *oB = pick_free(oA, nullptr);
-      FusedArgs fa;
-      fa.m = c->m;
-      fa.H = c->band_rows;
-      if (c->fused_target_blocks > 0 &&
-          (int64_t)active * ((c->ntm + fa.H - 1) / fa.H) < c->fused_target_blocks) {
-        // few realizations left: thinner bands so the launch still fills the GPU
-        int64_t h = (int64_t)c->ntm * active / c->fused_target_blocks;
-        fa.H = (int)std::max<int64_t>(4, std::min<int64_t>(fa.H, h));
-      }
-      fa.nbands = (c->ntm + fa.H - 1) / fa.H;
-      fa.k0 = (int)k;
-      fa.ncol = ((c->nu + 63) / 64) * 64;
-      fa.A = pA; fa.B = pB; fa.f = c->f; fa.dty0 = c->dty0; fa.outA = oA; fa.outB = oB;
-      fa.D = c->d_D; fa.coef = c->d_coef; fa.nr_table = nr; fa.s = c->d_s; fa.order = c->d_order;
-      fa.rstride = c->rstride;
-      size_t shmem = (size_t)T * 3 * fa.ncol * sizeof(double);
-      double units = 0;
-      for (int z = 0; z < active; z++) {
-        int64_t st = std::min<int64_t>(T, s[ord[z]] - k + 1);
-        units += 5.0 * st;  // algorithmic passes: as if stage by stage
-        if (s[ord[z]] <= k + T - 1) final_loc[ord[z]] = oA;
-      }
-      int blocks = active * fa.nbands;
-      if (c->fused_kernel == 2) {
-        switch (T) {
-          case 2: rc = launch_ex(c, kGroupFused, k_rkg2_march<2>, blocks, fa.ncol, shmem, fa, units); break;
-          case 3: rc = launch_ex(c, kGroupFused, k_rkg2_march<3>, blocks, fa.ncol, shmem, fa, units); break;
-          default: rc = launch_ex(c, kGroupFused, k_rkg2_march<4>, blocks, fa.ncol, shmem, fa, units); break;
-        }
-      } else {
-        switch (T) {
-          case 2: rc = launch_ex(c, kGroupFused, k_rkg2_fused<2>, blocks, fa.ncol, shmem, fa, units); break;
-          case 3: rc = launch_ex(c, kGroupFused, k_rkg2_fused<3>, blocks, fa.ncol, shmem, fa, units); break;
-          default: rc = launch_ex(c, kGroupFused, k_rkg2_fused<4>, blocks, fa.ncol, shmem, fa, units); break;
-        }
-      }
+      double *none[4] = {nullptr, nullptr, nullptr, nullptr};
+      rc = launch_fused(c, k, active, ord, s, pA, pB, c->dty0, oA, oB, none);
       if (rc) return rc;
+      for (int z = 0; z < active; z++)
+        if (s[ord[z]] <= k + T - 1) final_loc[ord[z]] = oA;
       pA = oA;
       pB = oB;
       k += T;

@@ -1,0 +1,85 @@
+"""Host-side tools without a GPU: the launch autotuner's search and cache,
+and bench.py's one-GPU contract."""
+import json
+import os
+import subprocess
+import sys
+
+import pytest
+
+from hipft_amd import tune
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+class FakeModel:
+    """Stands in for FluxTransport: records tunings, times by a cost table."""
+
+    def __init__(self, nr=4, cost=None):
+        self.nr, self.ntm, self.npm1 = nr, 33, 65
+        self.cfg = {}
+        self.cost = cost or (lambda c: 1.0)
+        self.calls = 0
+
+    def set_tuning(self, **kw):
+        self.cfg.update(kw)
+
+    def get_state(self, layout):
+        return "state"
+
+    def set_state(self, s, layout):
+        assert s == "state"
+
+    def synchronize(self):
+        pass
+
+    def has_flow(self):
+        return True
+
+    def diffusion_step(self, dt, subcycles):
+        import time
+        self.calls += 1
+        time.sleep(1e-4 * self.cost(self.cfg))
+
+    def advection_step(self, dt):
+        import time
+        time.sleep(1e-4 * (2.0 if self.cfg["advection_band"] == 0 else 1.0))
+
+
+def test_candidates_cover_knobs():
+    c1 = tune.candidates(1)
+    c64 = tune.candidates(64)
+    assert any(c["fused"] == 0 for c in c1)
+    assert all(c["fused_min_realizations"] <= 1 for c in c1 if c["fused"])
+    assert {c["fused_kernel"] for c in c64 if c["fused"]} == {1, 2}
+    assert all(set(tune.DIFF_KEYS) <= set(c) for c in c64)
+
+
+def test_apply_sets_every_key_with_defaults():
+    m = FakeModel()
+    tune.apply(m, {"fused": 0})
+    assert set(m.cfg) == set(tune.KEYS)
+    assert m.cfg["fused"] == 0 and m.cfg["band_rows"] == tune.DEFAULTS["band_rows"]
+
+
+def test_autotune_picks_fastest_and_caches(tmp_path, monkeypatch):
+    monkeypatch.setattr(tune, "CACHE", str(tmp_path / "cache.json"))
+    best_cfg = dict(fused=1, fused_stages=3, band_rows=64, fused_min_realizations=4,
+                    fused_target_blocks=0, fused_kernel=1)
+    cost = lambda c: 0.2 if all(c.get(k) == v for k, v in best_cfg.items()) else 3.0
+    m = FakeModel(nr=4, cost=cost)
+    got = tune.autotune(m, reps=1, use_cache=False)
+    assert all(got[k] == v for k, v in best_cfg.items())
+    assert got["advection_band"] == 1
+    cache = json.load(open(tmp_path / "cache.json"))
+    assert tune.cache_key(m) in cache
+    m2 = FakeModel(nr=4, cost=cost)
+    again = tune.autotune(m2, use_cache=True)
+    assert again == cache[tune.cache_key(m)] and m2.calls == 0
+
+
+def test_bench_refuses_more_than_one_gpu():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in json.loads(r.stdout.strip().splitlines()[-1])["error"]
