@@ -46,6 +46,39 @@ def build_lib(verbose=False):
     return LIB
 
 
+FORTRAN_DRIVER = os.path.join(HERE, "fortran", "hft_run")
+
+
+def amdflang():
+    for cand in (os.environ.get("HFT_FC"), "/opt/rocm/lib/llvm/bin/amdflang",
+                 shutil.which("amdflang")):
+        if cand and os.path.exists(cand):
+            return cand
+    return None
+
+
+def build_fortran(verbose=False):
+    """The Fortran driver (include/hft.f90 + fortran/hft_run.f90) linked to
+    libhft.so; skipped when no Fortran compiler is installed."""
+    fc = amdflang()
+    if fc is None:
+        if verbose:
+            print("no Fortran compiler: hft_run not built", flush=True)
+        return None
+    import tempfile
+    with tempfile.TemporaryDirectory() as mod_dir:
+        cmd = [fc, "-O2", "-J", mod_dir, os.path.join(ROOT, "include", "hft.f90"),
+               os.path.join(HERE, "fortran", "hft_run.f90"), "-L", HERE, "-lhft",
+               "-Wl,-rpath,$ORIGIN/..", "-o", FORTRAN_DRIVER]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        res = subprocess.run(cmd, capture_output=True, text=True, cwd=mod_dir)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError("Fortran driver build failed")
+    return FORTRAN_DRIVER
+
+
 def build_oracle(verbose=False):
     """Compile the CPU oracle (test infrastructure) with its Makefile."""
     odir = os.path.join(ROOT, "oracle")
@@ -58,5 +91,6 @@ def build_oracle(verbose=False):
 
 if __name__ == "__main__":
     build_lib(verbose=True)
+    build_fortran(verbose=True)
     build_oracle(verbose=True)
     print("built", LIB)

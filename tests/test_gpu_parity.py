@@ -413,3 +413,50 @@ def test_driver_writes_map_files(gpu_available, tmp_path):
     assert np.array_equal(d, R.model.get_state("pt"))
     d0, _ = h5io.read_map(str(tmp_path / "hipft_brmap_initial.h5"))
     assert not np.array_equal(d0, d)
+
+
+def test_fortran_driver_matches_python(gpu_available, tmp_path):
+    """The Fortran time loop (hipft_amd/fortran/hft_run.f90, through the
+    iso_c_binding module include/hft.f90) gives the same history rows and
+    final state, bit for bit, as the same run driven from Python."""
+    import subprocess
+    from hipft_amd import build as B
+    exe = B.FORTRAN_DRIVER
+    if not os.path.exists(exe):
+        pytest.skip("Fortran driver not built (no Fortran compiler)")
+    ntm, npm1, nr, nsteps = 41, 81, 3, 4
+    t, p = grid(ntm, npm1)
+    maps = np.stack([W.bipole_map(t, p, 12, 70 + r) for r in range(nr)])
+    eta = np.array([200.0, 500.0, 900.0])
+    dr = np.array([W.dr_coefficients(1.0 + 0.05 * r) for r in range(nr)])
+    mf = np.array([[22.0, 11.0, -28.0]] * nr)
+    for name, a in (("t", t), ("p", p), ("map", maps)):
+        np.ascontiguousarray(a, dtype=np.float64).tofile(str(tmp_path / f"{name}.bin"))
+    fl = lambda v: ", ".join(repr(float(x)) for x in v)
+    ctl = f"""&hft_run
+  ntm = {ntm}, npm1 = {npm1}, nr = {nr}, mode = 1, nsteps = {nsteps}, strang = 1,
+  subcycles = 0, dt = 1.0, advect = .true.,
+  eta = {fl(eta)},
+  dr_p0 = {fl(dr[:, 0])}, dr_p2 = {fl(dr[:, 1])}, dr_p4 = {fl(dr[:, 2])},
+  mf_p1 = {fl(mf[:, 0])}, mf_p3 = {fl(mf[:, 1])}, mf_p5 = {fl(mf[:, 2])},
+  t_file = 't.bin', p_file = 'p.bin', map_file = 'map.bin',
+  hist_file = 'hist.bin', final_file = 'final.bin'
+/
+"""
+    (tmp_path / "run.nml").write_text(ctl)
+    r = subprocess.run([exe, "run.nml"], cwd=str(tmp_path), capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    rows = np.fromfile(str(tmp_path / "hist.bin")).reshape(nsteps, nr, 16)
+    final = np.fromfile(str(tmp_path / "final.bin")).reshape(nr, npm1 + 1, ntm)
+    g = FluxTransport(t, p, nr, mode="independent")
+    g.set_state(maps, layout="pt")
+    g.set_flow_analytic(p0=dr[:, 0], p2=dr[:, 1], p4=dr[:, 2], p1=mf[:, 0], p3=mf[:, 1],
+                        p5=mf[:, 2])
+    g.set_diffusion(eta)
+    for n in range(nsteps):
+        g.flux_transport_step(1.0, strang=True, subcycles=0)
+        h = g.analysis()
+        assert_same(rows[n, :, 2:], h, f"history row {n + 1}")
+    assert_same(final, g.get_state("tp"), "final state")
+    g.close()
