@@ -468,7 +468,8 @@ int adjoint_end(hft_ctx *c, double *lam_dev, int64_t ld, const double *LAM, int 
   return 0;
 }
 
-int adjoint_check(hft_ctx *c, const double *lam_dev, int64_t ld, int subcycles) {
+int adjoint_check(hft_ctx *c, const double *lam_dev, int64_t ld, int subcycles, double dtl,
+                  int strang) {
   if (!c || !lam_dev) return fail(HFT_E_ARG, "null argument");
   if (ld < c->nu) return fail(HFT_E_ARG, "adjoint: ld < %d", c->nu);
   if (subcycles < 0) return fail(HFT_E_ARG, "adjoint: subcycles < 0");
@@ -476,6 +477,15 @@ int adjoint_check(hft_ctx *c, const double *lam_dev, int64_t ld, int subcycles) 
     return fail(HFT_E_STATE, "adjoint: attenuated (state-dependent) flows have no adjoint");
   if (c->flow_set && c->nu > 1024)
     return fail(HFT_E_ARG, "adjoint: more than 1024 unique phi columns");
+  if (c->flow_set) {
+    std::vector<int> nsub;
+    int nmax, rc = advection_substeps(c, strang ? half * dtl : dtl, nsub, nmax);
+    if (rc) return rc;
+    if (nmax > 1)
+      return fail(HFT_E_ARG,
+                  "adjoint: the advection step exceeds the CFL limit of a realization "
+                  "(%d sub-steps); shorten dt", nmax);
+  }
   return 0;
 }
 
@@ -528,7 +538,7 @@ int hft_get_state_device(hft_ctx *c, double *dst, int64_t ld) {
 
 int hft_step_adjoint(hft_ctx *c, double dtl, int strang, int subcycles, double *lam_dev,
                      int64_t ld, double *grad, int flags) {
-  int rc = adjoint_check(c, lam_dev, ld, subcycles);
+  int rc = adjoint_check(c, lam_dev, ld, subcycles, dtl, strang);
   if (rc) return rc;
   bool frozen = (flags & HFT_ADJ_FROZEN_WENO) != 0;
   double *LAM;
@@ -546,7 +556,7 @@ int hft_step_adjoint(hft_ctx *c, double dtl, int strang, int subcycles, double *
 
 int hft_steps_adjoint(hft_ctx *c, int nsteps, double dtl, int strang, int subcycles,
                       double *lam_dev, int64_t ld, double *grad, int flags) {
-  int rc = adjoint_check(c, lam_dev, ld, subcycles);
+  int rc = adjoint_check(c, lam_dev, ld, subcycles, dtl, strang);
   if (rc) return rc;
   if (nsteps < 1) return fail(HFT_E_ARG, "hft_steps_adjoint: nsteps < 1");
   bool frozen = (flags & HFT_ADJ_FROZEN_WENO) != 0;

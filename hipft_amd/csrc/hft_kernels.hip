@@ -964,6 +964,8 @@ struct AdvArgs {
   const double *y;   // second input of stage 3 (may be null)
   double *out;
   double ca, cb, cdt;  // out = ca*y + cb*x - cdt*A(x)   (or x - cdt*A(x))
+  const double *dtr;   // per-realization sub-step, when set: cdt = cmul * dtr[r]
+  double cmul;
   const int *order;
   size_t rstride;
   int write_aop;       // operator-only mode: out = A(x)
@@ -1086,14 +1088,15 @@ __global__ __launch_bounds__(kBlock) void k_advection_stage(AdvArgs a) {
   double A = advection_op<F2D>(m, fa, x, j, c < m.nu ? c : 0);
   if (c >= m.nu) return;
   size_t off = (size_t)j * m.pitch + c;
+  double cdt = a.dtr ? a.cmul * a.dtr[r] : a.cdt;
   double v;
   if (a.write_aop) {
     v = A;
   } else if (HAS_Y) {
     v = a.ca * a.y[base + off] + a.cb * x[off];
-    v = v - a.cdt * A;
+    v = v - cdt * A;
   } else {
-    v = x[off] - a.cdt * A;
+    v = x[off] - cdt * A;
   }
   a.out[base + off] = v;
 }
@@ -1117,6 +1120,8 @@ struct AdvBandArgs {
   const double *x, *y;
   double *out;
   double ca, cb, cdt;
+  const double *dtr;   // per-realization sub-step, when set: cdt = cmul * dtr[r]
+  double cmul;
   int write_aop;
   int nbands, H, ncol;
   const int *order;
@@ -1147,6 +1152,7 @@ __global__ __launch_bounds__(1024) void k_advection_band(AdvBandArgs a) {
   size_t base = (size_t)r * a.rstride;
   const double *x = a.x + base;
   const double half = 0.5;
+  const double cdt = a.dtr ? a.cmul * a.dtr[r] : a.cdt;
   int j0 = band * a.H, j1 = min(ntm, j0 + a.H);
   // theta split fluxes of the own column: LN/LP of reference cell jr
   auto ln_t = [&](int j0r, double xv) { return half * xv * (fa.vt(j0r, cc) + fa.at(j0r, cc)); };
@@ -1232,9 +1238,9 @@ __global__ __launch_bounds__(1024) void k_advection_band(AdvBandArgs a) {
         v = A;
       } else if (HAS_Y) {
         v = a.ca * a.y[base + off] + a.cb * X[1];
-        v = v - a.cdt * A;
+        v = v - cdt * A;
       } else {
-        v = X[1] - a.cdt * A;
+        v = X[1] - cdt * A;
       }
       a.out[base + off] = v;
     }

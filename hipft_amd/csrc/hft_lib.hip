@@ -110,6 +110,13 @@ struct hft_ctx {
   int fused_target_blocks = 0;  // shrink bands while fewer blocks than this (0: off)
   int fused_kernel = 1;         // 1: k_rkg2_fused, 2: k_rkg2_march (lean bookkeeping)
   int adv_band = 1, adv_band_rows = 16;  // banded WENO3 stage kernel
+  // independent mode: per-realization CFL sub-steps of the advection step
+  int adv_substeps = 1;
+  int64_t flow_gen = 0, adv_lim_gen = -1;
+  std::vector<double> adv_lim;
+  int *d_adv_order = nullptr, *h_adv_order = nullptr;
+  double *d_adv_dt = nullptr, *h_adv_dt = nullptr;
+  hipEvent_t adv_ev = nullptr;
   // hipGraph replay of RKG2 super-step launch sequences, keyed by
   // (use_af, realization -> stage count in launch order)
   int graphs_on = 0;
@@ -540,6 +547,7 @@ int rebuild_flow(hft_ctx *c) {
       (rc = up(c->ap1, &c->d_ap1)))
     return rc;
   c->flow_set = true;
+  c->flow_gen++;
   return 0;
 }
 
@@ -573,6 +581,7 @@ int update_flow_2d(hft_ctx *c) {
                      c->rstride, c->vt_rstride);
   HIPCHK(hipGetLastError());
   c->flow2d = true;
+  c->flow_gen++;
   return 0;
 }
 
@@ -1003,8 +1012,12 @@ int hft_destroy(hft_ctx *c) {
   for (auto *vec : {&c->adj_fix, &c->adj_ck, &c->adj_w, &c->adj_st})
     for (double *q : *vec)
       if (q) hipFree(q);
-  for (double *q : {c->d_apD, c->d_apA, c->d_gout})
+  for (double *q : {c->d_apD, c->d_apA, c->d_gout, c->d_adv_dt})
     if (q) hipFree(q);
+  if (c->d_adv_order) hipFree(c->d_adv_order);
+  if (c->h_adv_order) hipHostFree(c->h_adv_order);
+  if (c->h_adv_dt) hipHostFree(c->h_adv_dt);
+  if (c->adv_ev) hipEventDestroy(c->adv_ev);
   if (c->own_stream) {
     hipStreamSynchronize(c->own_stream);
     hipStreamDestroy(c->own_stream);  // c->stream belongs to the caller
@@ -1119,6 +1132,7 @@ int hft_set_flow_attenuation(hft_ctx *c, const double *value) {
   if (!value) {
     c->atten = false;
     c->flow2d = false;
+    c->flow_gen++;
     return 0;
   }
   c->atten_inv.resize(c->nr);
@@ -1241,9 +1255,22 @@ int hft_dt_euler(hft_ctx *c, double *dt, double *dt_r) {
   return 0;
 }
 
+// Realizations of one advection launch: the first n of order, each with its
+// own sub-step dtr[r] (cdt = cmul * dtr[r]).  Null plan: every realization,
+// one dt.
+struct AdvPlan {
+  const int *order;
+  const double *dtr;
+  int n;
+};
+
 static int advection_stage(hft_ctx *c, const double *x, const double *y, double *out, double ca,
-                           double cb, double cdt, bool aop_only) {
-  double u = (y ? 3.0 : 2.0) * c->nr;
+                           double cb, double cdt, bool aop_only, const AdvPlan *plan = nullptr,
+                           double cmul = 0.0) {
+  int n = plan ? plan->n : c->nr;
+  const int *order = plan ? plan->order : c->d_order_id;
+  const double *dtr = plan ? plan->dtr : nullptr;
+  double u = (y ? 3.0 : 2.0) * n;
   if (c->adv_band && c->nu <= 1024) {
     AdvBandArgs a;
     a.m = c->m;
@@ -1251,12 +1278,13 @@ static int advection_stage(hft_ctx *c, const double *x, const double *y, double 
     a.f2.vt = c->d_vt2; a.f2.vp = c->d_vp2; a.f2.at = c->d_at2; a.f2.ap = c->d_ap2;
     a.f2.vt_rstride = c->vt_rstride;
     a.x = x; a.y = y; a.out = out; a.ca = ca; a.cb = cb; a.cdt = cdt;
+    a.dtr = dtr; a.cmul = cmul;
     a.write_aop = aop_only ? 1 : 0;
     a.H = c->adv_band_rows;
     a.nbands = (c->ntm + a.H - 1) / a.H;
     a.ncol = ((c->nu + 63) / 64) * 64;
-    a.order = c->d_order_id; a.rstride = c->rstride;
-    int blocks = c->nr * a.nbands;
+    a.order = order; a.rstride = c->rstride;
+    int blocks = n * a.nbands;
     size_t shmem = (size_t)3 * a.ncol * sizeof(double);
     if (c->flow2d) {
       if (y) return launch_ex(c, 1, k_advection_band<true, true>, blocks, a.ncol, shmem, a, u);
@@ -1267,12 +1295,13 @@ static int advection_stage(hft_ctx *c, const double *x, const double *y, double 
   }
   AdvArgs a;
   a.m = c->m;
-  a.g = tiles_for(c, c->nr);
+  a.g = tiles_for(c, n);
   a.f1.vt = c->d_vt1; a.f1.at = c->d_at1; a.f1.vp = c->d_vp1; a.f1.ap = c->d_ap1;
   a.f2.vt = c->d_vt2; a.f2.vp = c->d_vp2; a.f2.at = c->d_at2; a.f2.ap = c->d_ap2;
   a.f2.vt_rstride = c->vt_rstride;
   a.x = x; a.y = y; a.out = out; a.ca = ca; a.cb = cb; a.cdt = cdt;
-  a.order = c->d_order_id; a.rstride = c->rstride; a.write_aop = aop_only ? 1 : 0;
+  a.dtr = dtr; a.cmul = cmul;
+  a.order = order; a.rstride = c->rstride; a.write_aop = aop_only ? 1 : 0;
   int nb = nblocks(a.g);
   if (c->flow2d) {
     if (y) return launch(c, 1, k_advection_stage<true, true>, nb, a, u);
@@ -1282,20 +1311,79 @@ static int advection_stage(hft_ctx *c, const double *x, const double *y, double 
   return launch(c, 1, k_advection_stage<false, false>, nb, a, u);
 }
 
-int hft_advection_step(hft_ctx *c, double dtl) {
-  if (!c) return fail(HFT_E_ARG, "null context");
-  if (!c->flow_set) return fail(HFT_E_STATE, "hft_advection_step: no flow set");
-  // advection_step_ssprk43, src/hipft.f90:7551-7574
-  int rc;
-  if ((rc = advection_stage(c, c->f, nullptr, c->buf0, 0, 0, half * dtl, false))) return rc;
-  if ((rc = advection_stage(c, c->buf0, nullptr, c->buf1, 0, 0, half * dtl, false))) return rc;
-  if ((rc = advection_stage(c, c->buf1, c->f, c->buf0, two_third, third, sixth * dtl, false))) return rc;
-  if ((rc = advection_stage(c, c->buf0, nullptr, c->f, 0, 0, half * dtl, false))) return rc;
+// CFL sub-steps per realization of an advection step of length dtl
+// (independent mode): n_r = ceil(dtl / limit_r), limit_r from the flow CFL
+// rule of get_flow_dtmax (src/hipft.f90:7287-7350) for an unsplit step.
+static int advection_substeps(hft_ctx *c, double dtl, std::vector<int> &nsub, int &nmax) {
+  int nr = c->nr;
+  nsub.assign(nr, 1);
+  nmax = 1;
+  if (c->mode != HFT_MODE_INDEPENDENT || !c->adv_substeps) return 0;
+  if (c->adv_lim_gen != c->flow_gen) {
+    double mn;
+    c->adv_lim.assign(nr, kHuge);
+    int rc = hft_flow_dtmax(c, 0, &mn, c->adv_lim.data());
+    if (rc) return rc;
+    c->adv_lim_gen = c->flow_gen;
+  }
+  for (int r = 0; r < nr; r++) {
+    double q = dtl / c->adv_lim[r];
+    int n = q > 1.0 ? (int)std::min(std::ceil(q), 1.0e6) : 1;
+    nsub[r] = n;
+    nmax = std::max(nmax, n);
+  }
   return 0;
 }
 
-// Controller of hft_diffusion_step; pre(round), when given, runs before each
-// round of cycles (the adjoint stores the cycle inputs there).
+int hft_advection_step(hft_ctx *c, double dtl) {
+  if (!c) return fail(HFT_E_ARG, "null context");
+  if (!c->flow_set) return fail(HFT_E_STATE, "hft_advection_step: no flow set");
+  int rc, nmax;
+  std::vector<int> nsub;
+  if ((rc = advection_substeps(c, dtl, nsub, nmax))) return rc;
+  if (nmax == 1) {
+    // advection_step_ssprk43, src/hipft.f90:7551-7574
+    if ((rc = advection_stage(c, c->f, nullptr, c->buf0, 0, 0, half * dtl, false))) return rc;
+    if ((rc = advection_stage(c, c->buf0, nullptr, c->buf1, 0, 0, half * dtl, false))) return rc;
+    if ((rc = advection_stage(c, c->buf1, c->f, c->buf0, two_third, third, sixth * dtl, false)))
+      return rc;
+    return advection_stage(c, c->buf0, nullptr, c->f, 0, 0, half * dtl, false);
+  }
+  // realization r takes nsub[r] SSPRK(4,3) steps of dtl / nsub[r]; sub-step
+  // i advances the realizations with nsub > i (a prefix of the sorted order)
+  int nr = c->nr;
+  std::vector<int> ord(nr);
+  for (int r = 0; r < nr; r++) ord[r] = r;
+  std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return nsub[a] > nsub[b]; });
+  if (!c->d_adv_order) {
+    int rc2;
+    if ((rc2 = dalloc(&c->d_adv_order, nr)) || (rc2 = dalloc(&c->d_adv_dt, nr))) return rc2;
+    HIPCHK(hipHostMalloc((void **)&c->h_adv_order, nr * sizeof(int)));
+    HIPCHK(hipHostMalloc((void **)&c->h_adv_dt, nr * sizeof(double)));
+    HIPCHK(hipEventCreateWithFlags(&c->adv_ev, hipEventDisableTiming));
+  } else {
+    HIPCHK(hipEventSynchronize(c->adv_ev));  // the previous upload has been read
+  }
+  for (int z = 0; z < nr; z++) c->h_adv_order[z] = ord[z];
+  for (int r = 0; r < nr; r++) c->h_adv_dt[r] = dtl / (double)nsub[r];
+  HIPCHK(hipMemcpyAsync(c->d_adv_order, c->h_adv_order, nr * sizeof(int), hipMemcpyHostToDevice,
+                        c->stream));
+  HIPCHK(hipMemcpyAsync(c->d_adv_dt, c->h_adv_dt, nr * sizeof(double), hipMemcpyHostToDevice,
+                        c->stream));
+  HIPCHK(hipEventRecord(c->adv_ev, c->stream));
+  int active = nr;
+  for (int i = 0; i < nmax; i++) {
+    while (active > 0 && nsub[ord[active - 1]] <= i) active--;
+    AdvPlan plan{c->d_adv_order, c->d_adv_dt, active};
+    if ((rc = advection_stage(c, c->f, nullptr, c->buf0, 0, 0, 0, false, &plan, half))) return rc;
+    if ((rc = advection_stage(c, c->buf0, nullptr, c->buf1, 0, 0, 0, false, &plan, half))) return rc;
+    if ((rc = advection_stage(c, c->buf1, c->f, c->buf0, two_third, third, 0, false, &plan, sixth)))
+      return rc;
+    if ((rc = advection_stage(c, c->buf0, nullptr, c->f, 0, 0, 0, false, &plan, half))) return rc;
+  }
+  return 0;
+}
+
 // Controller of hft_diffusion_step.  Hooks for the adjoint: pre(round) runs
 // before each round of cycles; superstep, when given, replaces the batched
 // RKG2 super-step (same arguments as rkg2_superstep).
@@ -1656,6 +1744,8 @@ int hft_set_tuning(hft_ctx *c, const char *key, int value) {
   } else if (!strcmp(key, "adjoint_band_rows")) {
     if (value < 1 || value > 4096) return fail(HFT_E_ARG, "adjoint_band_rows must be 1..4096");
     c->adj_band_rows = value;
+  } else if (!strcmp(key, "advection_substeps")) {
+    c->adv_substeps = value != 0;
   } else if (!strcmp(key, "advection_band")) {
     c->adv_band = value != 0;
   } else if (!strcmp(key, "advection_band_rows")) {
@@ -1681,6 +1771,7 @@ int hft_get_tuning(hft_ctx *c, const char *key, int *value) {
   else if (!strcmp(key, "adjoint_segment")) *value = c->adj_seg;
   else if (!strcmp(key, "adjoint_mem_mb")) *value = (int)c->adj_mem_mb;
   else if (!strcmp(key, "adjoint_band_rows")) *value = c->adj_band_rows;
+  else if (!strcmp(key, "advection_substeps")) *value = c->adv_substeps;
   else if (!strcmp(key, "advection_band")) *value = c->adv_band;
   else if (!strcmp(key, "advection_band_rows")) *value = c->adv_band_rows;
   else return fail(HFT_E_ARG, "unknown tuning key '%s'", key);

@@ -134,7 +134,10 @@ int hft_dt_euler(hft_ctx *ctx, double *dt, double *dt_r);
  * (0 = automatic PTL sub-cycling).  cb/user may be NULL. */
 int hft_flux_transport_step(hft_ctx *ctx, double dt, int strang, int subcycles,
                             hft_cycle_fn cb, void *user);
-/* advection_step (SSPRK(4,3) + WENO3, src/hipft.f90:4509-4549, 7515-7582). */
+/* advection_step (SSPRK(4,3) + WENO3, src/hipft.f90:4509-4549, 7515-7582).
+ * Independent mode: a realization whose flow CFL limit (hft_flow_dtmax with
+ * strang = 0) is below dt takes n_r = ceil(dt / limit_r) SSPRK(4,3) steps of
+ * dt / n_r, the others one step of dt (tuning key "advection_substeps"). */
 int hft_advection_step(hft_ctx *ctx, double dt);
 /* diffusion_step (RKG2 with PTL sub-cycles, src/hipft.f90:4551-4695). */
 int hft_diffusion_step(hft_ctx *ctx, double dt, int subcycles, hft_cycle_fn cb, void *user);
@@ -190,6 +193,9 @@ int hft_stream(hft_ctx *ctx, void **stream);
  *   "advection_band"         1 = WENO3 stages by theta bands computing each
  *                            face once (default); 0 = one thread per point
  *   "advection_band_rows"    theta rows per block of the banded WENO3 (16)
+ *   "advection_substeps"     1 = per-realization CFL sub-steps of the
+ *                            advection step in independent mode (default);
+ *                            0 = one step of dt for every realization
  *   "graphs"                 1 = capture each RKG2 super-step's launches in a
  *                            hipGraph and replay it when the same
  *                            realizations have the same stage counts (0);

@@ -122,6 +122,65 @@ def test_advection_step_bitexact(gpu_available):
     assert_same(g.get_state("tp"), o.get_f(), "3 SSPRK(4,3) steps")
 
 
+@pytest.mark.parametrize("band", [0, 1])
+def test_independent_advection_cfl_substeps(gpu_available, band):
+    """Independent mode, an advection step longer than some realizations' flow
+    CFL limits: realization r takes n_r = ceil(dt / limit_r) SSPRK(4,3) steps
+    of dt / n_r (limit_r = the single-realization oracle's flow_dtmax(0)),
+    bit for bit; with the sub-steps switched off every realization takes one
+    step of dt."""
+    t, p = grid(61, 121)
+    nr = 4
+    scale = [1.0, 2.2, 3.5, 1.4]  # differential rotation sets the limits here
+    vt = np.stack([W.meridional_profile(t, 10.0 + 4.0 * r) for r in range(nr)])
+    p0, p2, p4 = (np.array([c * scale[r] for r in range(nr)]) for c in W.dr_coefficients())
+    singles, maps = [], []
+    for r in range(nr):
+        o = Oracle(t, p, 1)
+        maps.append(random_state(o, 30 + r)[0])
+        o.flow_reset()
+        o.flow_set(np.ascontiguousarray(np.broadcast_to(vt[r][None, None, :], (1, o.npm, o.nt))),
+                   np.zeros((1, o.np_, o.ntm)))
+        o.flow_add_dr([p0[r]], [p2[r]], [p4[r]])
+        o.set_lf_alpha()
+        singles.append(o)
+    lim = np.array([o.flow_dtmax(0) for o in singles])
+    dt = 0.98 * lim.max()
+    n = np.maximum(1, np.ceil(dt / lim)).astype(int)
+    assert n.min() == 1 and len(set(n.tolist())) == 4, n
+
+    def gpu(substeps):
+        g = FluxTransport(t, p, nr, mode="independent")
+        g.set_state(np.stack(maps), layout="pt")
+        g.set_flow_profiles(vt, None)
+        g.set_flow_analytic(p0=p0, p2=p2, p4=p4)
+        g.set_tuning(advection_band=band, advection_substeps=substeps)
+        assert np.array_equal(g.flow_dtmax(0)[1], lim)
+        for _ in range(2):
+            g.advection_step(dt)
+        f = g.get_state("tp")
+        g.close()
+        return f
+
+    f = gpu(1)
+    for r, o in enumerate(singles):
+        for _ in range(2 * n[r]):
+            o.advection_step(dt / n[r])
+        assert_same(f[r:r + 1], o.get_f(), f"realization {r} with {n[r]} sub-steps")
+    f0 = gpu(0)
+    for r in range(nr):
+        o = Oracle(t, p, 1)
+        o.ingest_pt(maps[r][None])
+        o.flow_reset()
+        o.flow_set(np.ascontiguousarray(np.broadcast_to(vt[r][None, None, :], (1, o.npm, o.nt))),
+                   np.zeros((1, o.np_, o.ntm)))
+        o.flow_add_dr([p0[r]], [p2[r]], [p4[r]])
+        o.set_lf_alpha()
+        for _ in range(2):
+            o.advection_step(dt)
+        assert_same(f0[r:r + 1], o.get_f(), f"realization {r} without sub-steps")
+
+
 def test_attenuated_flow_advection(gpu_available):
     o, g = pair(45, 91, nr=2)
     p0, p2, p4 = W.dr_coefficients()
