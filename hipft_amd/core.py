@@ -54,6 +54,7 @@ _SIGS = [
     ("hft_set_flow_analytic", ctypes.c_int, [ctypes.c_void_p, _D, _D, _D, _D, _D, _D,
                                              ctypes.c_double, ctypes.c_double]),
     ("hft_set_flow_profiles", ctypes.c_int, [ctypes.c_void_p, _D, _D]),
+    ("hft_set_flow_arrays", ctypes.c_int, [ctypes.c_void_p, _D, _D, ctypes.c_int]),
     ("hft_set_flow_attenuation", ctypes.c_int, [ctypes.c_void_p, _D]),
     ("hft_update_flow", ctypes.c_int, [ctypes.c_void_p]),
     ("hft_flow_dtmax", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _D, _D]),
@@ -214,6 +215,32 @@ class FluxTransport:
         b = None if vp is None else np.ascontiguousarray(
             np.broadcast_to(np.asarray(vp, dtype=np.float64), (self.nr, self.ntm)))
         _check(lib().hft_set_flow_profiles(self._h, _p(a), _p(b)), "hft_set_flow_profiles")
+        self._flow = True
+
+    def set_flow_arrays(self, vt=None, vp=None):
+        """2-D base flows (flows from files) in code units, in the reference's
+        storage order as numpy arrays: vt [nrf][npm][nt] and vp [nrf][npm][ntm]
+        (C order, i.e. Fortran vt(nt,npm,nrf)), nrf = 1 (shared by every
+        realization) or nr.  None removes them."""
+        if vt is None and vp is None:
+            _check(lib().hft_set_flow_arrays(self._h, None, None, 0), "hft_set_flow_arrays")
+            return
+        vt = np.asarray(vt, dtype=np.float64)
+        vp = np.asarray(vp, dtype=np.float64)
+        if vt.ndim == 2:
+            vt = vt[None]
+        if vp.ndim == 2:
+            vp = vp[None]
+        nrf = vt.shape[0]
+        if nrf not in (1, self.nr) or vp.shape[0] != nrf:
+            raise ValueError(f"set_flow_arrays: {nrf} flows for {self.nr} realizations")
+        if vt.shape[1:] != (self.npm, self.nt) or vp.shape[1:] != (self.npm, self.ntm):
+            raise ValueError(f"set_flow_arrays: vt {vt.shape}, vp {vp.shape}; expected "
+                             f"[nrf][{self.npm}][{self.nt}] and [nrf][{self.npm}][{self.ntm}]")
+        a = np.ascontiguousarray(vt)
+        b = np.ascontiguousarray(vp)
+        _check(lib().hft_set_flow_arrays(self._h, _p(a), _p(b), int(nrf == self.nr and self.nr > 1)),
+               "hft_set_flow_arrays")
         self._flow = True
 
     def has_flow(self):

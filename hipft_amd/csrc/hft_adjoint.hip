@@ -474,7 +474,7 @@ int adjoint_check(hft_ctx *c, const double *lam_dev, int64_t ld, int subcycles, 
   if (ld < c->nu) return fail(HFT_E_ARG, "adjoint: ld < %d", c->nu);
   if (subcycles < 0) return fail(HFT_E_ARG, "adjoint: subcycles < 0");
   if (c->flow_set && (c->flow2d || c->atten))
-    return fail(HFT_E_STATE, "adjoint: attenuated (state-dependent) flows have no adjoint");
+    return fail(HFT_E_STATE, "adjoint: 2-D flows (from files or attenuated) have no adjoint");
   if (c->flow_set && c->nu > 1024)
     return fail(HFT_E_ARG, "adjoint: more than 1024 unique phi columns");
   if (c->flow_set) {

@@ -1468,40 +1468,66 @@ __global__ __launch_bounds__(kBlock) void k_apply_balanced(double *f, const doub
   }
 }
 
-// Attenuated flows (update_flow, src/hipft.f90:3489-3507): from base 1-D
-// profiles and the state, build 2-D vt (nt rows) and vp (ntm rows, faces q).
+// 2-D flows: vt (nt rows) and vp (ntm rows, faces q) per realization.
 struct AttenArgs {
   const double *f;
-  const double *vt1, *vp1;   // [nr][nt], [nr][ntm] unattenuated profiles
+  const double *vt1, *vp1;   // [nr][nt], [nr][ntm] assembled 1-D flows
+  // 2-D base flows (flows from files; null: the 1-D flows above are used):
+  // vtf [rf][nt][pitch], vpf [rf][ntm][pitch], rf = r or 0 (shared, stride 0)
+  const double *vtf, *vpf;
+  size_t vtf_rstride, vpf_rstride;
+  // analytic terms added to the base flows in update_flow's order
+  // (src/hipft.f90:3427-3445): vt - mft[r][j] + vtc, vp + drp[r][j] + omp[j]
+  const double *mft, *drp, *omp;
+  double vtc;
+  int has_vtc;
   double *vt, *vp;           // 2-D outputs
-  const double *inv;         // [nr] 1/flow_attenuate_value
+  const double *inv;         // [nr] 1/flow_attenuate_value, or null (no attenuation)
   int ntm, nt, nu, pitch;
   size_t rstride, vt_rstride;
 };
 
+// 2-D flows: update_flow (src/hipft.f90:3411-3510) from 2-D base flows and/or
+// 1-D flows, attenuated by |Br| (3489-3507) when a.inv is set.
 __global__ __launch_bounds__(kBlock) void k_attenuate(AttenArgs a) {
   int r = blockIdx.z, j = blockIdx.y;
   int c = blockIdx.x * kBlock + threadIdx.x;
   if (c >= a.nu) return;
   const double one = 1.0;
   const double *f = a.f + (size_t)r * a.rstride;
-  double inv = a.inv[r];
   // vt(jf) for jf = 1..nt (0-based j = jf-1 < nt): attenuated for jf = 2..nt-1
   if (j < a.nt) {
-    double v = a.vt1[(size_t)r * a.nt + j];
-    if (j >= 1 && j <= a.nt - 2) {
+    double v;
+    if (a.vtf) {
+      v = a.vtf[(size_t)r * a.vtf_rstride + (size_t)j * a.pitch + c];
+      if (a.mft) v = v - a.mft[(size_t)r * a.nt + j];
+      if (a.has_vtc) v = v + a.vtc;
+    } else {
+      v = a.vt1[(size_t)r * a.nt + j];
+    }
+    if (a.inv && j >= 1 && j <= a.nt - 2) {
       double b0 = fabs(f[(size_t)(j - 1) * a.pitch + c]), b1 = fabs(f[(size_t)j * a.pitch + c]);
       double m = b0 > b1 ? b0 : b1;
-      v = v * (one - tanh(m * inv));
+      v = v * (one - tanh(m * a.inv[r]));
     }
     a.vt[(size_t)r * a.vt_rstride + (size_t)j * a.pitch + c] = v;
   }
   // vp at face q = c (reference k = c+2, between columns c-1 and c)
   if (j < a.ntm) {
-    double v = a.vp1[(size_t)r * a.ntm + j];
-    double b0 = fabs(f[(size_t)j * a.pitch + wrapc(c - 1, a.nu)]), b1 = fabs(f[(size_t)j * a.pitch + c]);
-    double m = b0 > b1 ? b0 : b1;
-    v = v * (one - tanh(m * inv));
+    double v;
+    if (a.vpf) {
+      v = a.vpf[(size_t)r * a.vpf_rstride + (size_t)j * a.pitch + c];
+      if (a.drp) v = v + a.drp[(size_t)r * a.ntm + j];
+      if (a.omp) v = v + a.omp[j];
+    } else {
+      v = a.vp1[(size_t)r * a.ntm + j];
+    }
+    if (a.inv) {
+      double b0 = fabs(f[(size_t)j * a.pitch + wrapc(c - 1, a.nu)]);
+      double b1 = fabs(f[(size_t)j * a.pitch + c]);
+      double m = b0 > b1 ? b0 : b1;
+      v = v * (one - tanh(m * a.inv[r]));
+    }
     a.vp[(size_t)r * a.rstride + (size_t)j * a.pitch + c] = v;
   }
 }

@@ -135,6 +135,13 @@ struct hft_ctx {
   double *d_atten_inv = nullptr, *d_vt2 = nullptr, *d_vp2 = nullptr, *d_at2 = nullptr,
          *d_ap2 = nullptr;
   size_t vt_rstride = 0;
+  // 2-D base flows (flows from files): one shared or one per realization,
+  // and the analytic terms update_flow adds to them
+  bool ff = false;
+  size_t ff_vt_rstride = 0, ff_vp_rstride = 0;
+  double *d_vtf = nullptr, *d_vpf = nullptr, *d_mft = nullptr, *d_drp = nullptr,
+         *d_omp = nullptr;
+  bool ff_mf = false, ff_dr = false, ff_om = false;
   // diffusion
   bool diff_set = false;
   std::vector<double> D, dt_euler_r;
@@ -546,6 +553,40 @@ int rebuild_flow(hft_ctx *c) {
   if ((rc = up(c->vt1, &c->d_vt1)) || (rc = up(c->vp1, &c->d_vp1)) || (rc = up(c->at1, &c->d_at1)) ||
       (rc = up(c->ap1, &c->d_ap1)))
     return rc;
+  if (c->ff) {
+    // the analytic terms alone, added to the 2-D base flows on the device
+    c->ff_mf = !c->an_p1.empty();
+    c->ff_dr = !c->an_p0.empty();
+    c->ff_om = c->an_omega != 0.0;
+    std::vector<double> mft, drp, omp;
+    if (c->ff_mf) {
+      mft.assign((size_t)nr * nt, 0.0);
+      for (int r = 0; r < nr; r++)
+        for (int jr = 1; jr <= nt; jr++) {
+          double cc = c->cth[jr];
+          mft[(size_t)r * nt + jr - 1] =
+              m_s_to_rs_hr * c->sth[jr] *
+              (c->an_p1[r] * cc + c->an_p3[r] * ipow(cc, 3) + c->an_p5[r] * ipow(cc, 5));
+        }
+      if ((rc = up(mft, &c->d_mft))) return rc;
+    }
+    if (c->ff_dr) {
+      drp.assign((size_t)nr * ntm, 0.0);
+      for (int r = 0; r < nr; r++)
+        for (int jr = 1; jr <= ntm; jr++) {
+          double cc = c->ct[jr];
+          drp[(size_t)r * ntm + jr - 1] =
+              m_s_to_rs_hr * c->st[jr] *
+              (c->an_p0[r] + c->an_p2[r] * ipow(cc, 2) + c->an_p4[r] * ipow(cc, 4));
+        }
+      if ((rc = up(drp, &c->d_drp))) return rc;
+    }
+    if (c->ff_om) {
+      omp.assign(ntm, 0.0);
+      for (int jr = 1; jr <= ntm; jr++) omp[jr - 1] = c->an_omega * km_s_to_rs_hr * c->st[jr];
+      if ((rc = up(omp, &c->d_omp))) return rc;
+    }
+  }
   c->flow_set = true;
   c->flow_gen++;
   return 0;
@@ -570,7 +611,13 @@ int update_flow_2d(hft_ctx *c) {
   if (rc) return rc;
   AttenArgs a;
   a.f = c->f; a.vt1 = c->d_vt1; a.vp1 = c->d_vp1; a.vt = c->d_vt2; a.vp = c->d_vp2;
-  a.inv = c->d_atten_inv; a.ntm = c->ntm; a.nt = c->nt; a.nu = c->nu; a.pitch = c->pitch;
+  a.vtf = c->ff ? c->d_vtf : nullptr; a.vpf = c->ff ? c->d_vpf : nullptr;
+  a.vtf_rstride = c->ff_vt_rstride; a.vpf_rstride = c->ff_vp_rstride;
+  a.mft = c->ff_mf ? c->d_mft : nullptr; a.drp = c->ff_dr ? c->d_drp : nullptr;
+  a.omp = c->ff_om ? c->d_omp : nullptr;
+  a.vtc = c->an_vtc * km_s_to_rs_hr; a.has_vtc = c->an_vtc != 0.0;
+  a.inv = c->atten ? c->d_atten_inv : nullptr;
+  a.ntm = c->ntm; a.nt = c->nt; a.nu = c->nu; a.pitch = c->pitch;
   a.rstride = c->rstride; a.vt_rstride = c->vt_rstride;
   dim3 g((c->nu + kBlock - 1) / kBlock, c->nt, c->nr);
   hipLaunchKernelGGL(k_attenuate, g, dim3(kBlock), 0, c->stream, a);
@@ -996,7 +1043,8 @@ int hft_destroy(hft_ctx *c) {
                   c->d_coef, c->d_s, c->d_order, c->d_order_id, c->d_cdt, c->d_ub1,
                   c->d_absmax, c->d_dtmin, c->d_target, c->d_part, c->d_hist, c->d_da_t,
                   c->d_sn_t, c->d_cs_t, c->d_tt, c->d_da_p, c->d_cs_p, c->d_sn_p, c->d_pcol,
-                  c->d_map, c->d_apar, c->d_ipar};
+                  c->d_map, c->d_apar, c->d_ipar, c->d_vtf, c->d_vpf, c->d_mft, c->d_drp,
+                  c->d_omp};
   for (void *q : ptrs)
     if (q) hipFree(q);
   if (c->h_coef) hipHostFree(c->h_coef);
@@ -1113,7 +1161,7 @@ int hft_set_flow_analytic(hft_ctx *c, const double *p0, const double *p2, const 
   c->an_vtc = vt_const;
   int rc = rebuild_flow(c);
   if (rc) return rc;
-  if (c->atten) return update_flow_2d(c);
+  if (c->atten || c->ff) return update_flow_2d(c);
   return 0;
 }
 
@@ -1121,10 +1169,57 @@ int hft_set_flow_profiles(hft_ctx *c, const double *vt, const double *vp) {
   if (!c) return fail(HFT_E_ARG, "null context");
   if (vt) c->prof_vt.assign(vt, vt + (size_t)c->nr * c->nt); else c->prof_vt.clear();
   if (vp) c->prof_vp.assign(vp, vp + (size_t)c->nr * c->ntm); else c->prof_vp.clear();
+  c->ff = false;  // profiles replace 2-D base flows
+  c->flow2d = false;
   int rc = rebuild_flow(c);
   if (rc) return rc;
   if (c->atten) return update_flow_2d(c);
   return 0;
+}
+
+// Flows from files (add_flow_from_files, src/hipft.f90:3525-3660): 2-D base
+// flows in the reference's storage, vt(nt,npm,nrf) and vp(ntm,np,nrf) in code
+// units (rs/hr), theta fastest; nrf = nr when per_realization, else 1 (the
+// reference's files are shared by all realizations).  The unique phi columns
+// k = 2..npm-1 are taken; the seam columns follow the periodic rule
+// (set_periodic_bc_3d).  Analytic flows set before or after are added in
+// update_flow's order.  vt = vp = NULL removes the base flows.
+int hft_set_flow_arrays(hft_ctx *c, const double *vt, const double *vp, int per_realization) {
+  if (!c) return fail(HFT_E_ARG, "null context");
+  if ((vt == nullptr) != (vp == nullptr))
+    return fail(HFT_E_ARG, "hft_set_flow_arrays: vt and vp must be both set or both NULL");
+  int rc;
+  if (!vt) {
+    c->ff = false;
+    c->flow2d = false;
+    if ((rc = rebuild_flow(c))) return rc;
+    return c->atten ? update_flow_2d(c) : 0;
+  }
+  int nrf = per_realization ? c->nr : 1;
+  int nt = c->nt, ntm = c->ntm, npm = c->npm, nu = c->nu, pitch = c->pitch;
+  size_t rt = (size_t)nt * pitch, rp = (size_t)ntm * pitch;
+  if (!c->d_vtf) {
+    // room for per-realization flows, used shared or not
+    if ((rc = dalloc(&c->d_vtf, rt * c->nr)) || (rc = dalloc(&c->d_vpf, rp * c->nr))) return rc;
+  }
+  std::vector<double> ht(rt * nrf, 0.0), hp(rp * nrf, 0.0);
+  for (int r = 0; r < nrf; r++)
+    for (int q = 0; q < nu; q++) {
+      int k = q + 2;  // reference column
+      for (int j = 0; j < nt; j++)
+        ht[r * rt + (size_t)j * pitch + q] = vt[((size_t)r * npm + (k - 1)) * nt + j];
+      for (int j = 0; j < ntm; j++)
+        hp[r * rp + (size_t)j * pitch + q] = vp[((size_t)r * npm + (k - 1)) * ntm + j];
+    }
+  HIPCHK(hipMemcpy(c->d_vtf, ht.data(), ht.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_vpf, hp.data(), hp.size() * sizeof(double), hipMemcpyHostToDevice));
+  c->ff_vt_rstride = per_realization ? rt : 0;
+  c->ff_vp_rstride = per_realization ? rp : 0;
+  c->ff = true;
+  c->prof_vt.clear();  // 2-D base flows replace profiles
+  c->prof_vp.clear();
+  if ((rc = rebuild_flow(c))) return rc;
+  return update_flow_2d(c);
 }
 
 int hft_set_flow_attenuation(hft_ctx *c, const double *value) {
@@ -1133,7 +1228,7 @@ int hft_set_flow_attenuation(hft_ctx *c, const double *value) {
     c->atten = false;
     c->flow2d = false;
     c->flow_gen++;
-    return 0;
+    return c->ff ? update_flow_2d(c) : 0;
   }
   c->atten_inv.resize(c->nr);
   for (int r = 0; r < c->nr; r++) c->atten_inv[r] = value[r] > zero ? one / value[r] : kHuge;

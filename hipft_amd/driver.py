@@ -17,6 +17,7 @@ DTIME_ADV_STB, DTIME_ADV_USED, DTIME_DIFF_STB, N_DIFF_PER_STEP,
 N_DIFF_CYCLES] (num), per realization.
 """
 import math
+import os
 
 import numpy as np
 
@@ -35,7 +36,7 @@ DEFAULTS = dict(
     flow_dr_coef_p0_values=None, flow_dr_coef_p2_values=None, flow_dr_coef_p4_values=None,
     flow_mf_coef_p1_value=22.0, flow_mf_coef_p3_value=11.0, flow_mf_coef_p5_value=-28.0,
     flow_mf_coef_p1_values=None, flow_mf_coef_p3_values=None, flow_mf_coef_p5_values=None,
-    flow_profiles=None,
+    flow_profiles=None, use_flow_from_files=False, flow_list_filename=None, flow_root_dir=".",
     advance_diffusion=False, diffusion_coef_constant=0.0, diffusion_coef_constants=None,
     diffusion_coef_factor=DIFF_KM2_S_TO_RS2_HR, diffusion_subcycles=0,
     output_history_time_cadence=0.0, output_map_time_cadence=0.0, output_map_idx_start=1,
@@ -47,7 +48,8 @@ DEFAULTS = dict(
     assimilate_data_mu_powers=None, assimilate_data_mu_limit=0.1,
     assimilate_data_mu_limits=None, assimilate_data_lat_limit=0.0,
     assimilate_data_lat_limits=None, assimilate_data_mult_fac=1.0, assimilate_data_add=False,
-    assimilate_data_balance_flux=False,
+    assimilate_data_balance_flux=False, assimilate_data_map_list_filename=None,
+    assimilate_data_map_root_dir=".", base_dir=".",
 )
 
 
@@ -127,7 +129,21 @@ def run(cfg, max_steps=None, mode="coupled"):
         "flow_dr_coef_p0_value", "flow_dr_coef_p2_value", "flow_dr_coef_p4_value",
         "flow_mf_coef_p1_value", "flow_mf_coef_p3_value", "flow_mf_coef_p5_value",
         "flow_attenuate_value")}
+    def resolve(name):
+        return name if os.path.isabs(name) else os.path.join(c["base_dir"], name)
+
+    flow_files = None
+    if c["advance_flow"] and c["use_flow_from_files"]:
+        from .inputs import FlowFiles
+        flow_files = FlowFiles(resolve(c["flow_list_filename"]), resolve(c["flow_root_dir"]),
+                               c["time_start"], len(R.t), len(R.p))
     assim_times = list(c["assimilation_times"] or [])
+    if c["assimilate_data"] and c["assimilate_data_map_list_filename"]:
+        from .inputs import AssimilationList
+        al = AssimilationList(resolve(c["assimilate_data_map_list_filename"]),
+                              resolve(c["assimilate_data_map_root_dir"]), c["time_start"])
+        assim_times = al.times
+        c["assimilation_maps"] = al.map
     assim_idx = 0
     assimilate = bool(c["assimilate_data"]) and len(assim_times) > 0
 
@@ -157,7 +173,6 @@ def run(cfg, max_steps=None, mode="coupled"):
         d = c["output_map_directory"]
         if not d:
             return
-        import os
         from . import h5io
         os.makedirs(d, exist_ok=True)
         maps = m.get_state("pt")
@@ -185,6 +200,11 @@ def run(cfg, max_steps=None, mode="coupled"):
             mf = c["flow_mf_model"] == 1
             if c["flow_profiles"] is not None and steps == 0:
                 m.set_flow_profiles(*c["flow_profiles"])
+            if flow_files is not None:
+                # add_flow_from_files (3525-3673): the flow is re-added every step
+                if flow_files.update(time):
+                    m.set_flow_arrays(*flow_files.arrays())
+                st["flow_needs"] = True
             m.set_flow_analytic(
                 p0=coef["flow_dr_coef_p0_value"] if dr else None,
                 p2=coef["flow_dr_coef_p2_value"] if dr else None,
@@ -267,7 +287,6 @@ def run(cfg, max_steps=None, mode="coupled"):
 def write_histories(sol, num, directory, first_index=1):
     """Write hipft_history_{sol,num}_r######.out in the reference's layout
     (headers src/hipft.f90:1616-1646, formats 2633-2634)."""
-    import os
     sol_hdr = ["STEP", "TIME", "BR_MIN", "BR_MAX", "BR_ABS_MIN", "FLUX_POSITIVE",
                "FLUX_NEGATIVE", "NPOLE_FLUX_POSITIVE", "NPOLE_FLUX_NEGATIVE",
                "SPOLE_FLUX_POSITIVE", "SPOLE_FLUX_NEGATIVE", "NPOLE_AREA", "SPOLE_AREA",

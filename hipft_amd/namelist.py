@@ -99,6 +99,7 @@ def to_run_config(nml, base_dir="."):
         if k in LIST_KEYS and not isinstance(v, list):
             v = [v]
         cfg[k] = v
+    cfg["base_dir"] = base_dir  # flow and assimilation lists are read relative to it
     fname = cfg.get("initial_map_filename")
     if fname:
         from . import h5io

@@ -65,6 +65,12 @@ module hft
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx, vt, vp
     end function
+    integer(c_int) function hft_set_flow_arrays(ctx, vt, vp, per_realization) &
+        bind(c, name="hft_set_flow_arrays")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, vt, vp
+      integer(c_int), value :: per_realization
+    end function
     integer(c_int) function hft_set_flow_attenuation(ctx, value) bind(c, name="hft_set_flow_attenuation")
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx, value

@@ -107,6 +107,17 @@ int hft_set_flow_analytic(hft_ctx *ctx, const double *p0, const double *p2, cons
  * the analytic terms of hft_set_flow_analytic are added on top of it
  * (src/hipft.f90:3421-3448).  NULL vt or vp = zero. */
 int hft_set_flow_profiles(hft_ctx *ctx, const double *vt, const double *vp);
+/* Flows from files (add_flow_from_files, src/hipft.f90:3525-3660; the
+ * driver reads the files and converts m/s to code units as 3596-3598,
+ * 3625-3627 do): 2-D base flows in the reference's storage,
+ * vt(nt,npm,nrf) and vp(ntm,np,nrf) with np = npm, theta fastest,
+ * R_sun/hr; nrf = nr when per_realization != 0, else 1 (one flow for every
+ * realization, as the reference's files are).  Only the unique phi columns
+ * 2..npm-1 are read; the seam columns are periodic.  The analytic terms of
+ * hft_set_flow_analytic are added in update_flow's order (3421-3448) and
+ * attenuation applies on top.  Replaces profiles set by
+ * hft_set_flow_profiles; vt = vp = NULL removes the base flows. */
+int hft_set_flow_arrays(hft_ctx *ctx, const double *vt, const double *vp, int per_realization);
 /* Flow attenuation by |Br| (src/hipft.f90:3489-3515): value[nr] in Gauss
  * (<= 0 disables for that realization).  When enabled, the flow becomes a
  * per-step 2-D field recomputed from the state at every hft_update_flow. */

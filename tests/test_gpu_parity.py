@@ -200,6 +200,95 @@ def test_attenuated_flow_advection(gpu_available):
     assert abs(dt_g - dt_o) <= 1e-13 * dt_o
 
 
+def _flow_2d(o, nrf, seed):
+    """phi-dependent flows in code units, periodic in the reference's seam
+    columns: vt [nrf][npm][nt], vp [nrf][np][ntm]."""
+    rng = np.random.default_rng(seed)
+    t, p = o.mesh["t"][: o.ntm], o.mesh["p"]
+    vt = np.empty((nrf, o.npm, o.nt))
+    vp = np.empty((nrf, o.np_, o.ntm))
+    for r in range(nrf):
+        a, b, ph = rng.uniform(0.5, 1.5), rng.uniform(0.1, 0.4), rng.uniform(0, 2 * np.pi)
+        mf = W.meridional_profile(t, 15.0 * a)  # on the nt theta faces
+        vt[r] = mf[None, :] * (1.0 + b * np.cos(p[: o.npm, None] + ph))
+        vp[r] = 2e-4 * np.sin(t)[None, :] * np.sin(2 * p[: o.np_, None] + ph)
+    for v in (vt, vp):
+        v[:, 0] = v[:, -2]
+        v[:, -1] = v[:, 1]
+    return vt, vp
+
+
+@pytest.mark.parametrize("shared", [True, False])
+def test_flows_from_arrays_bitexact(gpu_available, shared):
+    """2-D base flows (flows from files) plus analytic meridional flow and
+    differential rotation: flows, Lax-Friedrichs alphas, the WENO3 operator,
+    the CFL limit and SSPRK(4,3) steps match the oracle bit for bit."""
+    nr = 3
+    o, g = pair(45, 91, nr=nr)
+    vt, vp = _flow_2d(o, 1 if shared else nr, 7)
+    p0, p2, p4 = (np.array([c * (1.0 + 0.1 * r) for r in range(nr)]) for c in W.dr_coefficients())
+    p1 = [22.0, 18.0, 26.0]
+    o.flow_reset()
+    o.flow_set(np.broadcast_to(vt, (nr,) + vt.shape[1:]), np.broadcast_to(vp, (nr,) + vp.shape[1:]))
+    o.flow_add_mf(p1, [11.0] * nr, [-28.0] * nr)
+    o.flow_add_dr(p0, p2, p4)
+    o.flow_add_rigid(0.2)
+    o.flow_add_vt_const(-0.03)
+    o.set_lf_alpha()
+    g.set_flow_analytic(p0=p0, p2=p2, p4=p4, p1=p1, p3=11.0, p5=-28.0, vp_rigid_omega=0.2,
+                        vt_const=-0.03)
+    g.set_flow_arrays(vt, vp)
+    x = o.get_f()
+    assert_same(g.apply_advection_operator(x), o.advection_operator(x), "A x (2-D flows)")
+    dt_o = o.flow_dtmax(0)
+    assert g.flow_dtmax(0)[0] == dt_o
+    for _ in range(2):
+        o.advection_step(0.4 * dt_o)
+        g.advection_step(0.4 * dt_o)
+    assert_same(g.get_state("tp"), o.get_f(), "SSPRK(4,3) with 2-D flows")
+    # removing the base flows leaves the analytic flows
+    g.set_flow_arrays(None, None)
+    o.flow_reset()
+    o.flow_add_mf(p1, [11.0] * nr, [-28.0] * nr)
+    o.flow_add_dr(p0, p2, p4)
+    o.flow_add_rigid(0.2)
+    o.flow_add_vt_const(-0.03)
+    o.set_lf_alpha()
+    x = o.get_f()
+    assert_same(g.apply_advection_operator(x), o.advection_operator(x), "A x (base removed)")
+
+
+def test_flows_from_arrays_attenuated(gpu_available):
+    """2-D base flows attenuated by |Br| every step (update_flow with
+    flow_attenuate): within 1e-12 of the oracle (device tanh)."""
+    nr = 2
+    o, g = pair(45, 91, nr=nr)
+    vt, vp = _flow_2d(o, 1, 11)
+    p0, p2, p4 = W.dr_coefficients()
+    att = [250.0, 120.0]
+
+    def oracle_flow():
+        o.flow_reset()
+        o.flow_set(np.broadcast_to(vt, (nr,) + vt.shape[1:]), np.broadcast_to(vp, (nr,) + vp.shape[1:]))
+        o.flow_add_dr([p0] * nr, [p2] * nr, [p4] * nr)
+        o.flow_attenuate(att)
+        o.set_lf_alpha()
+
+    g.set_flow_arrays(vt, vp)
+    g.set_flow_analytic(p0=p0, p2=p2, p4=p4)
+    g.set_flow_attenuation(att)
+    oracle_flow()
+    x = o.get_f()
+    assert_same(g.apply_advection_operator(x), o.advection_operator(x), "A x (2-D, attenuated)",
+                1e-12)
+    for _ in range(3):
+        o.advection_step(2.0)
+        g.advection_step(2.0)
+        oracle_flow()
+        g.update_flow()
+    assert_same(g.get_state("tp"), o.get_f(), "advection with attenuated 2-D flows", 1e-11)
+
+
 def test_rkg2_fixed_cycles_bitexact(gpu_available):
     o, g = pair(47, 97, nr=2)
     eta = [300.0, 700.0]
@@ -472,6 +561,65 @@ def test_driver_writes_map_files(gpu_available, tmp_path):
     assert np.array_equal(d, R.model.get_state("pt"))
     d0, _ = h5io.read_map(str(tmp_path / "hipft_brmap_initial.h5"))
     assert not np.array_equal(d0, d)
+
+
+def test_driver_flows_from_files(gpu_available, tmp_path):
+    """The driver with use_flow_from_files: a flow list of three phi-dependent
+    flow pairs (HDF5, m/s) switched at 3 h and 6 h and wrapped at 9 h, plus
+    analytic differential rotation, Strang splitting and diffusion.  Replayed
+    on the oracle step by step (same dt sequence, same active flow): the CFL
+    limit of every step and the final state agree bit for bit."""
+    from hipft_amd import h5io, validation as V
+    from hipft_amd.inputs import FlowFiles
+    s1, s2 = V.uniform_scales(31, 61)
+    ntm, npm1 = len(s2), len(s1)
+    nt, npm = ntm + 1, npm1 + 1
+    rng = np.random.default_rng(5)
+    th = np.linspace(0.0, np.pi, nt)
+    names = []
+    for i in range(3):
+        a, ph = rng.uniform(10.0, 20.0), rng.uniform(0, 2 * np.pi)
+        vt = -a * np.sin(2 * th)[:, None] * (1.0 + 0.3 * np.cos(s1[None, :] + ph))
+        pp = np.linspace(0, 2 * np.pi, npm)
+        vp = 30.0 * np.sin(s2)[:, None] * np.sin(2 * pp[None, :] + ph)
+        vt[:, 0] = vt[:, -1]
+        vp[:, 0], vp[:, -1] = vp[:, -2], vp[:, 1]
+        h5io.write_map(str(tmp_path / f"vt{i}.h5"), vt, [s1, th], single=False)
+        h5io.write_map(str(tmp_path / f"vp{i}.h5"), vp, [pp, s2], single=False)
+        names.append((f"vt{i}.h5", f"vp{i}.h5"))
+    with open(tmp_path / "flows.csv", "w") as fh:
+        fh.write("DAY        VT          VP\n")
+        for d, (a, b) in zip([0.0, 0.125, 0.25], names):
+            fh.write(f"{d:11.5f} {a:11s} {b:11s}\n")
+    nr = 2
+    cfg = dict(res_nt=31, res_np=61, n_realizations=nr, validation_run=1, advance_flow=True,
+               flow_dr_model=1, use_flow_from_files=True, flow_list_filename="flows.csv",
+               flow_root_dir=".", base_dir=str(tmp_path), advance_diffusion=True,
+               diffusion_coef_constants=[300.0, 600.0], strang_splitting=True, time_end=11.0,
+               dt_max=1.5)
+    sol, num, R = driver.run(cfg)
+    o = Oracle(R.t, R.p, nr)
+    o.ingest_pt(np.broadcast_to(V.soccer_ball(s1, s2), (nr, ntm, npm1)))
+    o.load_diffusion([300.0, 600.0])
+    ff = FlowFiles(str(tmp_path / "flows.csv"), str(tmp_path), 0.0, ntm, npm1)
+    time = 0.0
+    used = set()
+    for row in num[0][1:]:
+        if ff.update(time):
+            vt, vp = ff.arrays()
+        used.add(ff.idx)
+        o.flow_reset()
+        o.flow_set(np.broadcast_to(vt, (nr, npm, nt)), np.broadcast_to(vp, (nr, npm, ntm)))
+        o.flow_add_dr([46.0] * nr, [-262.0] * nr, [-379.0] * nr)
+        o.set_lf_alpha()
+        assert o.flow_dtmax(1) == row[3], f"CFL limit at t={time}"
+        dt = row[2]
+        o.advection_step(0.5 * dt)
+        o.diffusion_step(dt, 0)
+        o.advection_step(0.5 * dt)
+        time = time + dt
+    assert used == {0, 1, 2}
+    assert_same(R.model.get_state("tp"), o.get_f(), "driver with flows from files")
 
 
 def test_fortran_driver_matches_python(gpu_available, tmp_path):
