@@ -271,7 +271,9 @@ def main():
     fused = groups["rkg2_fused"]["launches"] > 0
     dom = "rkg2_fused" if fused else "rkg2"
     ms_d, n_d, b_d = groups[dom]["ms"], groups[dom]["launches"], groups[dom]["alg_GB"] * 1e9
-    kname = f"k_rkg2_fused<{tuning['fused_stages']}>" if fused else "k_rkg2_stage("
+    fk = {1: "k_rkg2_fused", 2: "k_rkg2_march", 3: "k_rkg2_fused2"}[
+        tuning.get("fused_kernel", tune.DEFAULTS["fused_kernel"])]
+    kname = f"{fk}<{tuning['fused_stages']}>" if fused else "k_rkg2_stage("
     avg_ms = ms_d / max(n_d, 1)
     achieved = (b_d / 1e9) / (ms_d / 1e3) if ms_d > 0 else 0.0
     traffic = load_traffic(kname)

@@ -774,6 +774,311 @@ __global__ __launch_bounds__(1024) void k_rkg2_march(FusedArgs a) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Fused RKG2 stages, two phi columns per thread (k_rkg2_fused2<T>): the
+// schedule of k_rkg2_fused<T> (bands, lag-one march, 3-row LDS rings, late
+// north pole) with thread x owning columns 2x and 2x+1.  The block has half
+// as many waves, so the block-uniform bookkeeping (row conditions, ring
+// slots) is issued half as often per column, the east neighbour of the even
+// column and the west neighbour of the odd column come from registers, and
+// row loads, LDS stores and row stores move 16 bytes per lane.  Every point
+// is computed with the expressions of k_rkg2_stage: bit-identical results.
+// The block has a multiple of 64 threads (full wavefronts for the polar
+// sums) and LDS rows of 2 * blockDim doubles.
+// ---------------------------------------------------------------------------
+// Prefetch distance of the row loads in steps; each load gets its own slot
+// (2 * distance slots), so a slot is consumed before it is loaded again and
+// the loads stay in flight across the unrolled loop's back edge.
+#ifndef HFT_FUSED2_PD
+#define HFT_FUSED2_PD 1
+#endif
+constexpr int kPD = HFT_FUSED2_PD;
+constexpr int kNS = 2 * kPD;
+
+template <int T>
+struct Fused2State {
+  double w[T][4][2];
+  double p0[T][2];
+  double hu[4][2], hd[4][2];
+  double pf[kNS][4][2];
+};
+
+// Columns c0, c0+1 of a row (c0 even, c0 < nu <= pitch, pitch even): one
+// 16-byte access.  When nu is odd the last thread's second column is the
+// row's first padding column, which no kernel reads as data; it may receive
+// that thread's (unused) value.
+__device__ __forceinline__ void load_pair(const double *row, int c0, double &x, double &y) {
+  double2 v = *reinterpret_cast<const double2 *>(row + c0);
+  x = v.x;
+  y = v.y;
+}
+
+__device__ __forceinline__ void store_pair(double *row, int c0, int nu, double x, double y) {
+  if (c0 < nu) *reinterpret_cast<double2 *>(row + c0) = make_double2(x, y);
+}
+
+template <int T, int U>
+__device__ __forceinline__ void fused2_step(const FusedArgs &a, Fused2State<T> &S, double *lds,
+                                            const double *rowc, int i, int nsteps, int lo0,
+                                            int hi0, int j0, int j1, int nst, bool final_here,
+                                            int c0, int cl, int cw0, int ce0, int ce1,
+                                            bool col1, int nrow,
+                                            double D2, const double (&P1)[2],
+                                            const double (&P5)[2], const double *A,
+                                            const double *B, const double *U0, const double *DT,
+                                            double *OA, double *OB, const double (&u0_pole)[2],
+                                            const double (&dt_pole)[2], const double (*cfs)[5],
+                                            double *const *Wst) {
+  const Mesh &m = a.m;
+  const int nu = m.nu, ntm = m.ntm, pitch = m.pitch;
+  constexpr int s0 = U & 3, s1 = (U + 3) & 3, s2 = (U + 2) & 3;
+  constexpr int p0s = U % kNS, pls = (U + kPD) % kNS;  // slot read now, slot loaded now
+  int q = lo0 + i;
+  double a0[2], b1[2], u1[2], d1[2];
+#pragma unroll
+  for (int e = 0; e < 2; e++) {
+    a0[e] = S.pf[p0s][0][e];
+    b1[e] = S.pf[p0s][1][e];
+    u1[e] = S.pf[p0s][2][e];
+    d1[e] = S.pf[p0s][3][e];
+  }
+  {
+    // prefetch step i+kPD into its own slot.  The loads are
+    // unconditional (rows and columns clamped into the field): values of rows
+    // outside the band's range or of columns >= nu never reach a stored
+    // output, and straight-line loads let each wait cover only the load it
+    // needs instead of every load in flight.
+    int qn = min(q + kPD, hi0 - 1), rn = min(max(q + kPD - 1, 0), ntm - 1);
+    size_t oa = (size_t)qn * pitch + cl, o = (size_t)rn * pitch + cl;
+    load_pair(A + oa, 0, S.pf[pls][0][0], S.pf[pls][0][1]);
+    load_pair(B + o, 0, S.pf[pls][1][0], S.pf[pls][1][1]);
+    load_pair(U0 + o, 0, S.pf[pls][2][0], S.pf[pls][2][1]);
+    load_pair(DT + o, 0, S.pf[pls][3][0], S.pf[pls][3][1]);
+  }
+#pragma unroll
+  for (int e = 0; e < 2; e++) {
+    S.hu[s0][e] = u1[e];
+    S.hd[s0][e] = d1[e];
+    S.w[0][s0][e] = a0[e];
+  }
+  if (q < hi0) {
+    *reinterpret_cast<double2 *>(lds + (size_t)(q % 3) * nrow + c0) = make_double2(a0[0], a0[1]);
+    if (q == 0) {
+      S.p0[0][0] = a0[0];
+      S.p0[0][1] = a0[1];
+    }
+  }
+#pragma unroll
+  for (int t = 1; t <= T; t++) {
+    int R = q - t;
+    int rlo = max(0, j0 - T + t), rhi = min(ntm, j1 + T - t);
+    const double *lp = lds + (size_t)((t - 1) * 3) * nrow;  // level t-1 ring
+    const double *k = cfs[t - 1];
+    const int ha = (t == 1) ? s0 : (t == 2 ? s1 : (t == 3 ? s2 : ((U + 1) & 3)));
+    bool act = t <= nst && R >= rlo && R < rhi;
+    double v[2] = {0.0, 0.0};
+    if (act && R == 1 && rlo == 0) {
+      // late north-pole row 0 of stage t, from level t-1 rows 0 and 1 in LDS
+      const double *r0 = lp + 0 * nrow, *r1p = lp + 1 * nrow;
+      double fn = wave_pole_sum(nu, [&](int kk) { return (D2 * (r1p[kk] - r0[kk])) * m.dp[kk + 1]; });
+      double Lv = fn * m.bc_diff_n;
+      double bp0 = 0.0, bp1 = 0.0;
+      if (t == 1 && c0 < nu) load_pair(B, c0, bp0, bp1);
+      double vv[2];
+#pragma unroll
+      for (int e = 0; e < 2; e++) {
+        double xc = S.p0[t - 1][e];
+        double ukm2 = (t == 1) ? (e == 0 ? bp0 : bp1) : S.p0[t >= 2 ? t - 2 : 0][e];
+        double x = k[0] * xc;
+        x = x + k[1] * ukm2;
+        x = x + k[2] * u0_pole[e];
+        x = x + k[3] * Lv;
+        x = x + k[4] * dt_pole[e];
+        vv[e] = x;
+      }
+      if (t < T) {
+        S.p0[t < T ? t : 0][0] = vv[0];
+        S.p0[t < T ? t : 0][1] = vv[1];
+        *reinterpret_cast<double2 *>(lds + (size_t)(t * 3 + 0) * nrow + c0) = make_double2(vv[0], vv[1]);
+      }
+      if (0 >= j0 && 0 < j1) {
+        if (t == nst) store_pair(OA, c0, nu, vv[0], vv[1]);
+        if (!final_here && t == T - 1) store_pair(OB, c0, nu, vv[0], vv[1]);
+        if (Wst[t - 1]) store_pair(Wst[t - 1], c0, nu, vv[0], vv[1]);
+      }
+    }
+    if (act && R > 0) {
+      double Lv[2];
+      if (R == ntm - 1) {
+        const double *r0 = lp + ((ntm - 1) % 3) * nrow, *r1p = lp + ((ntm - 2) % 3) * nrow;
+        double fs = wave_pole_sum(nu, [&](int kk) { return (D2 * (r0[kk] - r1p[kk])) * m.dp[kk + 1]; });
+        Lv[0] = Lv[1] = -fs * m.bc_diff_s;
+      } else {
+        const double *rc = lp + (R % 3) * nrow;
+        const double *rr = rowc + 4 * (R - lo0);
+        double sti = rr[0];
+        double c2 = rr[1];
+        double c4 = rr[2];
+        double xc0 = S.w[t - 1][s1][0], xc1 = S.w[t - 1][s1][1];
+        double xw[2], xe[2];
+        xw[0] = rc[cw0];
+        xe[0] = col1 ? xc1 : rc[ce0];
+        xw[1] = xc0;
+        xe[1] = rc[ce1];
+#pragma unroll
+        for (int e = 0; e < 2; e++) {
+          double xc = S.w[t - 1][s1][e];
+          double xs = S.w[t - 1][s0][e];
+          double xn = (R == 1) ? S.p0[t - 1][e] : S.w[t - 1][s2][e];
+          double c1 = (P1[e] * sti) * sti;
+          double c5 = (P5[e] * sti) * sti;
+          double c3 = -(((c1 + c2) + c4) + c5);
+          double L = c1 * xw[e];
+          L = L + c2 * xn;
+          L = L + c3 * xc;
+          L = L + c4 * xs;
+          L = L + c5 * xe[e];
+          Lv[e] = L;
+        }
+      }
+#pragma unroll
+      for (int e = 0; e < 2; e++) {
+        double xc = S.w[t - 1][s1][e];
+        double ukm2 = (t == 1) ? b1[e] : S.w[t >= 2 ? t - 2 : 0][s2][e];
+        double x = k[0] * xc;
+        x = x + k[1] * ukm2;
+        x = x + k[2] * S.hu[ha][e];
+        x = x + k[3] * Lv[e];
+        x = x + k[4] * S.hd[ha][e];
+        v[e] = x;
+      }
+      if (R >= j0 && R < j1) {
+        size_t o = (size_t)R * pitch;
+        if (t == nst) store_pair(OA + o, c0, nu, v[0], v[1]);
+        if (!final_here && t == T - 1) store_pair(OB + o, c0, nu, v[0], v[1]);
+        if (Wst[t - 1]) store_pair(Wst[t - 1] + o, c0, nu, v[0], v[1]);
+      }
+    }
+    if (t < T) {
+      S.w[t < T ? t : 0][s0][0] = v[0];
+      S.w[t < T ? t : 0][s0][1] = v[1];
+      if (act && R > 0)
+        *reinterpret_cast<double2 *>(lds + (size_t)(t * 3 + (R % 3)) * nrow + c0) = make_double2(v[0], v[1]);
+    }
+  }
+  lds_barrier();
+}
+
+template <int T>
+__global__ __launch_bounds__(512) void k_rkg2_fused2(FusedArgs a) {
+  extern __shared__ double lds[];  // [T][3][2 * blockDim]
+  __shared__ double cfs[4][5];
+  __shared__ double rowc[4 * kFusedMaxRows];
+  const Mesh &m = a.m;
+  const int nu = m.nu, ntm = m.ntm;
+  const int nrow = 2 * (int)blockDim.x;
+  int L = xcd_remap(blockIdx.x, gridDim.x);
+  int z = L / a.nbands, band = L - z * a.nbands;
+  int r = a.order[z];
+  int c0 = 2 * (int)threadIdx.x;
+  int cl = min(c0, m.pitch - 2);  // column pair of the loads, inside the row
+  bool col1 = c0 + 1 < nu;
+  int e0 = c0 < nu ? c0 : 0, e1 = col1 ? c0 + 1 : 0;
+  int cw0 = wrapc(e0 - 1, nu), ce0 = wrapc(e0 + 1, nu), ce1 = wrapc(e1 + 1, nu);
+  int j0 = band * a.H, j1 = min(ntm, j0 + a.H);
+  int lo0 = max(0, j0 - T), hi0 = min(ntm, j1 + T);
+  int64_t sr = a.s[r];
+  int nst = (int)((sr - a.k0 + 1) < T ? (sr - a.k0 + 1) : T);
+  bool final_here = (a.k0 + nst - 1) == sr;
+  size_t base = (size_t)r * a.rstride;
+  const double *A = a.A + base, *B = a.B + base, *U0 = a.f + base, *DT = a.dty0 + base;
+  double *OA = a.outA + base, *OB = a.outB + base;
+  double *Wst[T];
+#pragma unroll
+  for (int t = 0; t < T; t++) Wst[t] = a.W[t] ? a.W[t] + base : nullptr;
+  double D = a.D[r];
+  double D2 = D + D;
+  double P1[2], P5[2];
+  P1[0] = (D * m.dph_i[e0 + 1]) * m.dp_i[e0 + 1];
+  P5[0] = (D * m.dph_i[e0 + 2]) * m.dp_i[e0 + 1];
+  P1[1] = (D * m.dph_i[e1 + 1]) * m.dp_i[e1 + 1];
+  P5[1] = (D * m.dph_i[e1 + 2]) * m.dp_i[e1 + 1];
+  if (threadIdx.x < T * 5) {
+    int t = threadIdx.x / 5, e = threadIdx.x - 5 * t;
+    double v = 0.0;
+    if (t < nst) {
+      const StageCoef &k = a.coef[(size_t)(a.k0 + t) * a.nr_table + r];
+      v = e == 0 ? k.a : (e == 1 ? k.b : (e == 2 ? k.c : (e == 3 ? k.d : k.g)));
+    }
+    cfs[t][e] = v;
+  }
+  for (int R = lo0 + (int)threadIdx.x; R < hi0; R += blockDim.x) {
+    double *rr = rowc + 4 * (R - lo0);
+    if (R > 0 && R < ntm - 1) {
+      double sti = m.st_i[R];
+      double c2 = D * m.dth_i[R];
+      c2 = c2 * m.dt_i[R];
+      c2 = c2 * sti;
+      c2 = c2 * m.sth[R];
+      double c4 = D * m.dth_i[R + 1];
+      c4 = c4 * m.dt_i[R];
+      c4 = c4 * sti;
+      c4 = c4 * m.sth[R + 1];
+      rr[0] = sti;
+      rr[1] = c2;
+      rr[2] = c4;
+    } else {
+      rr[0] = rr[1] = rr[2] = 0.0;
+    }
+    rr[3] = 0.0;
+  }
+  Fused2State<T> S;
+#pragma unroll
+  for (int t = 0; t < T; t++)
+#pragma unroll
+    for (int e = 0; e < 2; e++) {
+      S.p0[t][e] = 0.0;
+#pragma unroll
+      for (int d = 0; d < 4; d++) S.w[t][d][e] = 0.0;
+    }
+#pragma unroll
+  for (int d = 0; d < 4; d++)
+#pragma unroll
+    for (int e = 0; e < 2; e++) S.hu[d][e] = S.hd[d][e] = 0.0;
+  double u0_pole[2] = {0.0, 0.0}, dt_pole[2] = {0.0, 0.0};
+  if (lo0 == 0 && c0 < nu) {
+    load_pair(U0, c0, u0_pole[0], u0_pole[1]);
+    load_pair(DT, c0, dt_pole[0], dt_pole[1]);
+  }
+  int nsteps = (hi0 - lo0) + T;
+#pragma unroll
+  for (int i = 0; i < kPD; i++) {
+    int q = min(lo0 + i, hi0 - 1), rn = min(max(lo0 + i - 1, 0), ntm - 1);
+    size_t oa = (size_t)q * m.pitch + cl, o = (size_t)rn * m.pitch + cl;
+    load_pair(A + oa, 0, S.pf[i][0][0], S.pf[i][0][1]);
+    load_pair(B + o, 0, S.pf[i][1][0], S.pf[i][1][1]);
+    load_pair(U0 + o, 0, S.pf[i][2][0], S.pf[i][2][1]);
+    load_pair(DT + o, 0, S.pf[i][3][0], S.pf[i][3][1]);
+  }
+  lds_barrier();
+  for (int i = 0; i < nsteps; i += 4) {
+    fused2_step<T, 0>(a, S, lds, rowc, i, nsteps, lo0, hi0, j0, j1, nst, final_here, c0, cl, cw0,
+                      ce0, ce1, col1, nrow, D2, P1, P5, A, B, U0, DT, OA, OB, u0_pole, dt_pole, cfs, Wst);
+    if (i + 1 < nsteps)
+      fused2_step<T, 1>(a, S, lds, rowc, i + 1, nsteps, lo0, hi0, j0, j1, nst, final_here, c0,
+                        cl, cw0, ce0, ce1, col1, nrow, D2, P1, P5, A, B, U0, DT, OA, OB, u0_pole, dt_pole,
+                        cfs, Wst);
+    if (i + 2 < nsteps)
+      fused2_step<T, 2>(a, S, lds, rowc, i + 2, nsteps, lo0, hi0, j0, j1, nst, final_here, c0,
+                        cl, cw0, ce0, ce1, col1, nrow, D2, P1, P5, A, B, U0, DT, OA, OB, u0_pole, dt_pole,
+                        cfs, Wst);
+    if (i + 3 < nsteps)
+      fused2_step<T, 3>(a, S, lds, rowc, i + 3, nsteps, lo0, hi0, j0, j1, nst, final_here, c0,
+                        cl, cw0, ce0, ce1, col1, nrow, D2, P1, P5, A, B, U0, DT, OA, OB, u0_pole, dt_pole,
+                        cfs, Wst);
+  }
+}
+
 // Plain operator application y = L x (tests, PTL pass).
 struct LapArgs {
   Mesh m;

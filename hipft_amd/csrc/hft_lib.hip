@@ -108,7 +108,7 @@ struct hft_ctx {
   // tuning (hft_set_tuning): fused multi-stage RKG2 launches
   int fused = 1, fused_T = 4, band_rows = 32, fused_min = 8;
   int fused_target_blocks = 0;  // shrink bands while fewer blocks than this (0: off)
-  int fused_kernel = 1;         // 1: k_rkg2_fused, 2: k_rkg2_march (lean bookkeeping)
+  int fused_kernel = 3;  // 1: k_rkg2_fused, 2: k_rkg2_march, 3: k_rkg2_fused2 (2 columns/thread)
   int adv_band = 1, adv_band_rows = 16;  // banded WENO3 stage kernel
   // independent mode: per-realization CFL sub-steps of the advection step
   int adv_substeps = 1;
@@ -824,6 +824,16 @@ int launch_fused(hft_ctx *c, int64_t k, int active, const std::vector<int> &ord,
   }
   int blocks = active * fa.nbands;
   bool store = W[0] || W[1] || W[2] || W[3];
+  if (c->fused_kernel == 3) {
+    // two phi columns per thread: a multiple of 64 threads, LDS rows of 2 * threads
+    int threads = ((fa.ncol / 2 + 63) / 64) * 64;
+    size_t sh2 = (size_t)T * 3 * 2 * threads * sizeof(double);
+    switch (T) {
+      case 2: return launch_ex(c, kGroupFused, k_rkg2_fused2<2>, blocks, threads, sh2, fa, units);
+      case 3: return launch_ex(c, kGroupFused, k_rkg2_fused2<3>, blocks, threads, sh2, fa, units);
+      default: return launch_ex(c, kGroupFused, k_rkg2_fused2<4>, blocks, threads, sh2, fa, units);
+    }
+  }
   if (c->fused_kernel == 2 && !store) {
     switch (T) {
       case 2: return launch_ex(c, kGroupFused, k_rkg2_march<2>, blocks, fa.ncol, shmem, fa, units);
@@ -1028,6 +1038,9 @@ int hft_create(int ntm, int npm1, const double *t, const double *p, int nr, int 
     hipFuncSetAttribute((const void *)k_rkg2_march<2>, hipFuncAttributeMaxDynamicSharedMemorySize, 2 * row);
     hipFuncSetAttribute((const void *)k_rkg2_march<3>, hipFuncAttributeMaxDynamicSharedMemorySize, 3 * row);
     hipFuncSetAttribute((const void *)k_rkg2_march<4>, hipFuncAttributeMaxDynamicSharedMemorySize, 4 * row);
+    hipFuncSetAttribute((const void *)k_rkg2_fused2<2>, hipFuncAttributeMaxDynamicSharedMemorySize, 2 * row);
+    hipFuncSetAttribute((const void *)k_rkg2_fused2<3>, hipFuncAttributeMaxDynamicSharedMemorySize, 3 * row);
+    hipFuncSetAttribute((const void *)k_rkg2_fused2<4>, hipFuncAttributeMaxDynamicSharedMemorySize, 4 * row);
     set_adjoint_attributes();
   }
   *out = c;
@@ -1826,7 +1839,7 @@ int hft_set_tuning(hft_ctx *c, const char *key, int value) {
     if (value < 0) return fail(HFT_E_ARG, "fused_target_blocks must be >= 0");
     c->fused_target_blocks = value;
   } else if (!strcmp(key, "fused_kernel")) {
-    if (value < 1 || value > 2) return fail(HFT_E_ARG, "fused_kernel must be 1 or 2");
+    if (value < 1 || value > 3) return fail(HFT_E_ARG, "fused_kernel must be 1, 2 or 3");
     c->fused_kernel = value;
   } else if (!strcmp(key, "graphs")) {
     c->graphs_on = value != 0;

@@ -24,7 +24,7 @@ DIFF_KEYS = ("fused", "fused_stages", "band_rows", "fused_min_realizations",
 ADV_KEYS = ("advection_band", "advection_band_rows")
 KEYS = DIFF_KEYS + ADV_KEYS
 DEFAULTS = dict(fused=1, fused_stages=4, band_rows=32, fused_min_realizations=8,
-                fused_target_blocks=0, fused_kernel=1, advection_band=1, advection_band_rows=16)
+                fused_target_blocks=0, fused_kernel=3, advection_band=1, advection_band_rows=16)
 
 
 def adv_candidates():
@@ -34,8 +34,8 @@ def adv_candidates():
 
 def candidates(nr):
     out = [dict(fused=0, fused_stages=4, band_rows=32, fused_min_realizations=1,
-                fused_target_blocks=0, fused_kernel=1)]
-    for K in (1, 2):
+                fused_target_blocks=0, fused_kernel=3)]
+    for K in (1, 3):
         for T in (3, 4):
             for H in (32, 64, 128):
                 for m in sorted({1, 4, 8, 16, 32} & set(range(1, max(nr, 1) + 1))):

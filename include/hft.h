@@ -201,6 +201,9 @@ int hft_stream(hft_ctx *ctx, void **stream);
  *   "fused_target_blocks"    when a fused launch would have fewer blocks
  *                            (realizations x bands) than this, its bands are
  *                            made thinner (>= 4 rows) to fill the GPU (0: off)
+ *   "fused_kernel"           fused launch kernel: 1 = one phi column per
+ *                            thread, 2 = the same with lean bookkeeping,
+ *                            3 = two phi columns per thread (default)
  *   "advection_band"         1 = WENO3 stages by theta bands computing each
  *                            face once (default); 0 = one thread per point
  *   "advection_band_rows"    theta rows per block of the banded WENO3 (16)

@@ -430,7 +430,7 @@ def test_atten_map_vs_reference(gpu_available):
     assert err <= 1e-10, cols
 
 
-@pytest.mark.parametrize("kernel", [1, 2])
+@pytest.mark.parametrize("kernel", [1, 2, 3])
 @pytest.mark.parametrize("T,H,B", [(2, 4, 0), (3, 7, 0), (4, 32, 0), (4, 5, 0), (4, 32, 40),
                                    (3, 64, 1000)])
 def test_fused_rkg2_bitexact(gpu_available, T, H, B, kernel):
@@ -448,7 +448,7 @@ def test_fused_rkg2_bitexact(gpu_available, T, H, B, kernel):
     assert_same(g.get_state("tp"), o.get_f(), f"fused kernel {kernel} T={T} H={H} B={B}")
 
 
-@pytest.mark.parametrize("kernel", [1, 2])
+@pytest.mark.parametrize("kernel", [1, 2, 3])
 def test_fused_matches_single_stage_full_size(gpu_available, kernel):
     """512x1024 batch, independent stage counts: fused and one-stage-per-launch
     paths agree bit for bit."""
