@@ -202,6 +202,28 @@ __global__ __launch_bounds__(kBlock) void k_rkg2_stage(StageArgs a) {
   o[p] = v;
 }
 
+// Final stages of a super-step into f: realization real[z] is copied from
+// src[which[z]], z = 0..n-1, in 16-byte words (rstride is even).  One launch
+// replaces a device-to-device copy per realization.
+constexpr int kCopyMax = 256;
+struct CopyFinalsArgs {
+  double *dst;
+  const double *src[4];
+  size_t rstride;
+  int n, per;  // realizations, blocks per realization
+  uint16_t real[kCopyMax];
+  uint8_t which[kCopyMax];
+};
+
+__global__ __launch_bounds__(kBlock) void k_copy_finals(CopyFinalsArgs a) {
+  int z = blockIdx.x / a.per;
+  size_t i = (size_t)(blockIdx.x - z * a.per) * kBlock + threadIdx.x;
+  if (i >= a.rstride / 2) return;
+  size_t p = (size_t)a.real[z] * a.rstride + 2 * i;
+  *reinterpret_cast<double2 *>(a.dst + p) =
+      *reinterpret_cast<const double2 *>(a.src[a.which[z]] + p);
+}
+
 // ---------------------------------------------------------------------------
 // Fused RKG2 stages: T consecutive stages k0..k0+T-1 in one launch.
 //

@@ -906,6 +906,25 @@ int superstep_launches(hft_ctx *c, int n, const std::vector<int> &ord,
       k += 1;
     }
   }
+  if (nr <= kCopyMax) {
+    CopyFinalsArgs ca;
+    ca.dst = c->f;
+    ca.rstride = c->rstride;
+    ca.n = 0;
+    double *bufs[4] = {c->buf0, c->buf1, c->buf2, c->buf3};
+    for (int q = 0; q < 4; q++) ca.src[q] = bufs[q];
+    for (int r = 0; r < nr; r++) {
+      if (!final_loc[r]) continue;
+      int q = 0;
+      while (q < 4 && bufs[q] != final_loc[r]) q++;
+      if (q == 4) return fail(HFT_E_STATE, "final RKG2 stage in an unknown buffer");
+      ca.real[ca.n] = (uint16_t)r;
+      ca.which[ca.n] = (uint8_t)q;
+      ca.n++;
+    }
+    ca.per = (int)((c->rstride / 2 + kBlock - 1) / kBlock);
+    return launch(c, 0, k_copy_finals, ca.n * ca.per, ca, 0.0);
+  }
   for (int r = 0; r < nr; r++)
     if (final_loc[r])
       HIPCHK(hipMemcpyAsync(c->f + (size_t)r * c->rstride, final_loc[r] + (size_t)r * c->rstride,
