@@ -55,6 +55,11 @@ _SIGS = [
                                              ctypes.c_double, ctypes.c_double]),
     ("hft_set_flow_profiles", ctypes.c_int, [ctypes.c_void_p, _D, _D]),
     ("hft_set_flow_arrays", ctypes.c_int, [ctypes.c_void_p, _D, _D, ctypes.c_int]),
+    ("hft_set_source", ctypes.c_int, [ctypes.c_void_p, _D]),
+    ("hft_rfe_generate", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _D, ctypes.c_int, _D]),
+    ("hft_rfe_weights", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]),
+    ("hft_get_rfe", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _D]),
+    ("hft_source_step", ctypes.c_int, [ctypes.c_void_p, ctypes.c_double]),
     ("hft_set_flow_attenuation", ctypes.c_int, [ctypes.c_void_p, _D]),
     ("hft_update_flow", ctypes.c_int, [ctypes.c_void_p]),
     ("hft_flow_dtmax", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _D, _D]),
@@ -198,6 +203,39 @@ class FluxTransport:
         _check(lib().hft_get_state(self._h, _p(out), LAYOUT_TP if layout == "tp" else LAYOUT_PT),
                "hft_get_state")
         return out
+
+    # ---- sources ----------------------------------------------------------
+    def set_source(self, src_tp=None):
+        """Constant source map (G/hr), reference storage (npm, ntm) as a C
+        array (Fortran src(ntm,npm)), shared by all realizations; None
+        removes it."""
+        if src_tp is None:
+            _check(lib().hft_set_source(self._h, None), "hft_set_source")
+            return
+        a = np.ascontiguousarray(src_tp, dtype=np.float64)
+        if a.shape != (self.npm, self.ntm):
+            raise ValueError(f"set_source: shape {a.shape}, expected ({self.npm}, {self.ntm})")
+        _check(lib().hft_set_source(self._h, _p(a)), "hft_set_source")
+
+    def rfe_generate(self, slot, rnd, model, uf_per_hour):
+        """Random flux emergence frame into slot 0/1 from the random map
+        rnd [npm-2][ntm] (C order of rnd_nums(1:ntm, 2:npm-1))."""
+        a = np.ascontiguousarray(rnd, dtype=np.float64)
+        if a.shape != (self.npm - 2, self.ntm):
+            raise ValueError(f"rfe_generate: shape {a.shape}, expected ({self.npm - 2}, {self.ntm})")
+        _check(lib().hft_rfe_generate(self._h, int(slot), _p(a), int(model),
+                                      _p(_vec(uf_per_hour, self.nr))), "hft_rfe_generate")
+
+    def rfe_weights(self, first_slot, alpha):
+        _check(lib().hft_rfe_weights(self._h, int(first_slot), float(alpha)), "hft_rfe_weights")
+
+    def get_rfe(self, slot):
+        out = np.empty(self.tp_shape)
+        _check(lib().hft_get_rfe(self._h, int(slot), _p(out)), "hft_get_rfe")
+        return out
+
+    def source_step(self, dt):
+        _check(lib().hft_source_step(self._h, float(dt)), "hft_source_step")
 
     # ---- flows ------------------------------------------------------------
     def set_flow_analytic(self, p0=None, p2=None, p4=None, p1=None, p3=None, p5=None,

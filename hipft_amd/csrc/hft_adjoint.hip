@@ -477,6 +477,7 @@ int adjoint_check(hft_ctx *c, const double *lam_dev, int64_t ld, int subcycles, 
     return fail(HFT_E_STATE, "adjoint: 2-D flows (from files or attenuated) have no adjoint");
   if (c->flow_set && c->nu > 1024)
     return fail(HFT_E_ARG, "adjoint: more than 1024 unique phi columns");
+  if (sources_active(c)) return fail(HFT_E_STATE, "adjoint: source terms are not supported");
   if (c->flow_set) {
     std::vector<int> nsub;
     int nmax, rc = advection_substeps(c, strang ? half * dtl : dtl, nsub, nmax);

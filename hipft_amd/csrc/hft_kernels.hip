@@ -1795,6 +1795,100 @@ __global__ __launch_bounds__(kBlock) void k_apply_balanced(double *f, const doub
   }
 }
 
+// ---------------------------------------------------------------------------
+// Sources (update_source, src/hipft.f90:3875-3947; generate_rfe 8809-8921;
+// source_step 4697-4733).
+// ---------------------------------------------------------------------------
+// Random flux emergence frame: rfe = rnd * scale_r (model 2) or
+// ((rnd * scale_r) * da_ti(j)) * dp_i(k) (model 1), rnd one shared map.
+__global__ __launch_bounds__(kBlock) void k_rfe_frame(double *rfe, const double *rnd,
+                                                      const double *scale, const double *da_ti,
+                                                      const double *dp_i, int model, int ntm,
+                                                      int nu, int pitch, size_t rstride) {
+  int r = blockIdx.y;
+  size_t n = (size_t)ntm * pitch;
+  double sc = scale[r];
+  for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * kBlock) {
+    int c = (int)(i % pitch);
+    if (c >= nu) continue;
+    int j = (int)(i / pitch);
+    double v;
+    if (model == 1) {
+      v = rnd[i] * sc;
+      v = v * da_ti[j];
+      v = v * dp_i[c + 1];
+    } else {
+      v = rnd[i] * sc;
+    }
+    rfe[(size_t)r * rstride + i] = v;
+  }
+}
+
+// Pole rows of a frame := sum_{k=1}^{npm-2} rfe(pole,k) dp(k) / (2 pi), in
+// the reference's sequential k order (k = 1 is the seam column nu-1).
+// Block r, thread 0 sums; the block writes the row.
+__global__ __launch_bounds__(64) void k_rfe_poles(double *rfe, const double *dp, int ntm, int nu,
+                                                  int pitch, size_t rstride, double twopi_i) {
+  __shared__ double fpole[2];
+  double *base = rfe + (size_t)blockIdx.x * rstride;
+  if (threadIdx.x < 2) {
+    const double *row = base + (size_t)(threadIdx.x == 0 ? 0 : ntm - 1) * pitch;
+    double s = 0.0;
+    s = s + row[nu - 1] * dp[0];
+    for (int c = 0; c <= nu - 2; c++) s = s + row[c] * dp[c + 1];
+    fpole[threadIdx.x] = s * twopi_i;
+  }
+  __syncthreads();
+  for (int c = threadIdx.x; c < nu; c += blockDim.x) {
+    base[c] = fpole[0];
+    base[(size_t)(ntm - 1) * pitch + c] = fpole[1];
+  }
+}
+
+// map := balanced map (balance_flux, src/hipft.f90:8788-8801); ratio 0 leaves
+// the map unchanged (zero net flux).
+__global__ __launch_bounds__(kBlock) void k_balance_inplace(double *map, const double *ratio,
+                                                            int ntm, int nu, int pitch,
+                                                            size_t rstride) {
+  int r = blockIdx.y;
+  size_t n = (size_t)ntm * pitch;
+  double rt = ratio[r];
+  if (rt == 0.0) return;
+  for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * kBlock) {
+    int c = (int)(i % pitch);
+    if (c >= nu) continue;
+    size_t p = (size_t)r * rstride + i;
+    double br = map[p];
+    map[p] = br > 0 ? br / rt : br * rt;
+  }
+}
+
+// f += dt * source, source = ((0 + file) + (1 - alpha) rfe1) + alpha rfe2
+// with the absent terms left out (update_source's order).
+__global__ __launch_bounds__(kBlock) void k_source_step(double *f, const double *file,
+                                                        const double *r1, const double *r2,
+                                                        double alpha, double dt, int ntm, int nu,
+                                                        int pitch, size_t rstride) {
+  int r = blockIdx.y;
+  size_t n = (size_t)ntm * pitch;
+  const double one = 1.0;
+  for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * kBlock) {
+    int c = (int)(i % pitch);
+    if (c >= nu) continue;
+    size_t p = (size_t)r * rstride + i;
+    double s = 0.0;
+    if (file) s = s + file[i];
+    if (r1) {
+      s = s + (one - alpha) * r1[p];
+      s = s + alpha * r2[p];
+    }
+    f[p] = f[p] + dt * s;
+  }
+}
+
 // 2-D flows: vt (nt rows) and vp (ntm rows, faces q) per realization.
 struct AttenArgs {
   const double *f;

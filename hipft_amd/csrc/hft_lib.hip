@@ -50,6 +50,7 @@ const double twopi = 6.2831853071795864, twopi_i = 0.15915494309189535;
 const double d2r = 0.017453292519943295;
 const double km_s_to_rs_hr = 0.005172413793103448, m_s_to_rs_hr = 5.172413793103448e-06;
 const double safety = 0.95;
+const double rsun_cm2 = 4.84416e21, input_flux_fac = 1.0e21;
 const double kHuge = 1.7976931348623157e308, kTiny = 2.2250738585072014e-308;
 
 // x**n, integer n >= 0, by square-and-multiply (the expansion compilers emit
@@ -142,6 +143,13 @@ struct hft_ctx {
   double *d_vtf = nullptr, *d_vpf = nullptr, *d_mft = nullptr, *d_drp = nullptr,
          *d_omp = nullptr;
   bool ff_mf = false, ff_dr = false, ff_om = false;
+  // sources: a constant map (source from file) and two random flux
+  // emergence frames blended as (1 - alpha) rfe[first] + alpha rfe[1 - first]
+  double *d_src = nullptr, *d_rfe[2] = {nullptr, nullptr}, *d_rnd = nullptr;
+  double *d_rfe_dati = nullptr, *d_rfe_scale = nullptr;
+  bool src_on = false;
+  int rfe_first = -1;
+  double rfe_alpha = 0.0;
   // diffusion
   bool diff_set = false;
   std::vector<double> D, dt_euler_r;
@@ -1076,7 +1084,8 @@ int hft_destroy(hft_ctx *c) {
                   c->d_absmax, c->d_dtmin, c->d_target, c->d_part, c->d_hist, c->d_da_t,
                   c->d_sn_t, c->d_cs_t, c->d_tt, c->d_da_p, c->d_cs_p, c->d_sn_p, c->d_pcol,
                   c->d_map, c->d_apar, c->d_ipar, c->d_vtf, c->d_vpf, c->d_mft, c->d_drp,
-                  c->d_omp};
+                  c->d_omp, c->d_src, c->d_rfe[0], c->d_rfe[1], c->d_rnd, c->d_rfe_dati,
+                  c->d_rfe_scale};
   for (void *q : ptrs)
     if (q) hipFree(q);
   if (c->h_coef) hipHostFree(c->h_coef);
@@ -1627,19 +1636,24 @@ int hft_diffusion_stats(hft_ctx *c, int64_t *s, int64_t *cycles) {
   return 0;
 }
 
+static bool sources_active(const hft_ctx *c);
+
 int hft_flux_transport_step(hft_ctx *c, double dtl, int strang, int subcycles, hft_cycle_fn cb,
                             void *user) {
   if (!c) return fail(HFT_E_ARG, "null context");
   int rc;
-  bool adv = c->flow_set, dif = c->diff_set;
+  bool adv = c->flow_set, dif = c->diff_set, src = sources_active(c);
   if (strang) {
     double h = dtl * half;
+    if (src && (rc = hft_source_step(c, h))) return rc;
     if (adv && (rc = hft_advection_step(c, h))) return rc;
     if (dif && (rc = hft_diffusion_step(c, dtl, subcycles, cb, user))) return rc;
     if (adv && (rc = hft_advection_step(c, h))) return rc;
+    if (src && (rc = hft_source_step(c, h))) return rc;
   } else {
     if (adv && (rc = hft_advection_step(c, dtl))) return rc;
     if (dif && (rc = hft_diffusion_step(c, dtl, subcycles, cb, user))) return rc;
+    if (src && (rc = hft_source_step(c, dtl))) return rc;
   }
   return 0;
 }
@@ -1750,6 +1764,137 @@ int hft_assimilate(hft_ctx *c, const double *map, int per_real, int custom, cons
                            add_mode, balance);
   if (rc) return rc;
   HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+// ---- sources: load_source (src/hipft.f90:7050-7217), update_source
+// (3875-3947), generate_rfe (8809-8921), source_step (4697-4733) ----------
+int hft_set_source(hft_ctx *c, const double *src) {
+  if (!c) return fail(HFT_E_ARG, "null context");
+  if (!src) {
+    c->src_on = false;
+    return 0;
+  }
+  int ntm = c->ntm, pitch = c->pitch, rc;
+  if (!c->d_src && (rc = dalloc(&c->d_src, (size_t)ntm * pitch))) return rc;
+  std::vector<double> h((size_t)ntm * pitch, 0.0);
+  for (int k = 2; k <= c->npm - 1; k++)
+    for (int j = 0; j < ntm; j++) h[(size_t)j * pitch + (k - 2)] = src[j + (size_t)ntm * (k - 1)];
+  HIPCHK(hipMemcpy(c->d_src, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+  c->src_on = true;
+  return 0;
+}
+
+// Multiplicative flux balance of every realization of a device field in
+// place (balance_flux, src/hipft.f90:8752-8801).
+static int balance_field(hft_ctx *c, double *fld) {
+  int nr = c->nr;
+  AnaArgs fa{};
+  fa.f = fld; fa.ntm = c->ntm; fa.nu = c->nu; fa.pitch = c->pitch; fa.nr = nr;
+  fa.rstride = c->rstride; fa.da_t = c->d_da_t; fa.da_p = c->d_da_p; fa.pcol = c->d_pcol;
+  fa.rows_per_block = c->ana_rows; fa.part = c->d_part; fa.nblk = c->ana_nblk;
+  hipLaunchKernelGGL(k_flux_part, dim3(c->ana_nblk, nr), dim3(kBlock), 0, c->stream, fa);
+  HIPCHK(hipGetLastError());
+  std::vector<double> part((size_t)nr * c->ana_nblk * 2);
+  HIPCHK(hipMemcpyAsync(part.data(), c->d_part, part.size() * sizeof(double),
+                        hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  std::vector<double> ratio(nr);
+  for (int r = 0; r < nr; r++) {
+    double fp = 0, fm = 0;
+    for (int b = 0; b < c->ana_nblk; b++) {
+      fp += part[((size_t)r * c->ana_nblk + b) * 2];
+      fm += part[((size_t)r * c->ana_nblk + b) * 2 + 1];
+    }
+    ratio[r] = (fm + fp == zero) ? 0.0 : std::sqrt(std::fabs(fp / fm));
+  }
+  HIPCHK(hipMemcpy(c->d_apar, ratio.data(), nr * sizeof(double), hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_balance_inplace, dim3(256, nr), dim3(kBlock), 0, c->stream, fld,
+                     (const double *)c->d_apar, c->ntm, c->nu, c->pitch, c->rstride);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int hft_rfe_generate(hft_ctx *c, int slot, const double *rnd, int model,
+                     const double *uf_per_hour) {
+  if (!c || !rnd || !uf_per_hour) return fail(HFT_E_ARG, "hft_rfe_generate: null argument");
+  if (slot != 0 && slot != 1) return fail(HFT_E_ARG, "hft_rfe_generate: slot must be 0 or 1");
+  if (model != 1 && model != 2) return fail(HFT_E_ARG, "hft_rfe_generate: model must be 1 or 2");
+  int nr = c->nr, ntm = c->ntm, npm = c->npm, nu = c->nu, pitch = c->pitch, rc;
+  if (!c->d_rfe[slot] && (rc = dalloc(&c->d_rfe[slot], c->nfield))) return rc;
+  if (!c->d_rnd) {
+    if ((rc = dalloc(&c->d_rnd, (size_t)ntm * pitch)) || (rc = dalloc(&c->d_rfe_dati, ntm)) ||
+        (rc = dalloc(&c->d_rfe_scale, nr)))
+      return rc;
+  }
+  std::vector<double> h((size_t)ntm * pitch, 0.0);
+  for (int q = 0; q < nu; q++)
+    for (int j = 0; j < ntm; j++) h[(size_t)j * pitch + q] = rnd[(size_t)q * ntm + j];
+  // per-realization scale (7173-7194) and the model-1 cell factor (8860-8874)
+  std::vector<double> scale(nr), dati(ntm, 0.0);
+  for (int r = 0; r < nr; r++) {
+    double uf = uf_per_hour[r] * input_flux_fac;
+    scale[r] = model == 1 ? uf / (rsun_cm2 * (double)(ntm * (npm - 2)))
+                          : uf / (rsun_cm2 * four * pi);
+  }
+  for (int jr = 1; jr <= ntm; jr++) {
+    if (jr == 1) {
+      double tav = half * (c->t[1] + c->t[2]);
+      dati[0] = one / (quarter * std::sin(tav) * c->dth[2]);
+    } else if (jr == ntm) {
+      double tav = half * (c->t[ntm] + c->t[ntm - 1]);
+      dati[ntm - 1] = one / (quarter * std::sin(tav) * c->dth[ntm]);
+    } else {
+      dati[jr - 1] = c->st_i[jr] * c->dt_i[jr];
+    }
+  }
+  HIPCHK(hipMemcpy(c->d_rnd, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_rfe_scale, scale.data(), nr * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_rfe_dati, dati.data(), ntm * sizeof(double), hipMemcpyHostToDevice));
+  double *fr = c->d_rfe[slot];
+  hipLaunchKernelGGL(k_rfe_frame, dim3(256, nr), dim3(kBlock), 0, c->stream, fr,
+                     (const double *)c->d_rnd, (const double *)c->d_rfe_scale,
+                     (const double *)c->d_rfe_dati, c->m.dp_i, model, ntm, nu, pitch, c->rstride);
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(k_rfe_poles, dim3(nr), dim3(64), 0, c->stream, fr, c->m.dp, ntm, nu, pitch,
+                     c->rstride, twopi_i);
+  HIPCHK(hipGetLastError());
+  if ((rc = balance_field(c, fr))) return rc;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int hft_rfe_weights(hft_ctx *c, int first_slot, double alpha) {
+  if (!c) return fail(HFT_E_ARG, "null context");
+  if (first_slot < -1 || first_slot > 1) return fail(HFT_E_ARG, "hft_rfe_weights: slot");
+  if (first_slot >= 0 && (!c->d_rfe[0] || !c->d_rfe[1]))
+    return fail(HFT_E_STATE, "hft_rfe_weights: both frames must be generated first");
+  c->rfe_first = first_slot;
+  c->rfe_alpha = alpha;
+  return 0;
+}
+
+int hft_get_rfe(hft_ctx *c, int slot, double *out_tp) {
+  if (!c || !out_tp) return fail(HFT_E_ARG, "null argument");
+  if ((slot != 0 && slot != 1) || !c->d_rfe[slot])
+    return fail(HFT_E_STATE, "hft_get_rfe: frame %d not generated", slot);
+  std::vector<double> dev;
+  int rc = get_field(c, c->d_rfe[slot], dev);
+  if (rc) return rc;
+  dev_to_tp(c, dev, out_tp);
+  return 0;
+}
+
+static bool sources_active(const hft_ctx *c) { return c->src_on || c->rfe_first >= 0; }
+
+int hft_source_step(hft_ctx *c, double dtl) {
+  if (!c) return fail(HFT_E_ARG, "null context");
+  const double *r1 = c->rfe_first >= 0 ? c->d_rfe[c->rfe_first] : nullptr;
+  const double *r2 = c->rfe_first >= 0 ? c->d_rfe[1 - c->rfe_first] : nullptr;
+  hipLaunchKernelGGL(k_source_step, dim3(256, c->nr), dim3(kBlock), 0, c->stream, c->f,
+                     (const double *)(c->src_on ? c->d_src : nullptr), r1, r2, c->rfe_alpha, dtl,
+                     c->ntm, c->nu, c->pitch, c->rstride);
+  HIPCHK(hipGetLastError());
   return 0;
 }
 

@@ -4,10 +4,10 @@
 ``hipft.in`` names, see hipft_amd/namelist.py) and advances the batch on the
 GPU, producing the reference's history rows:
 
-* update_step (3338): update_flow (3371-3523) and update_timestep
-  (4308-4461: end-of-run, flow CFL, dt_max, output-time and assimilation-time
-  cuts),
-* flux_transport_step (4463-4507),
+* update_step (3338): update_flow (3371-3523, flows from files 3525-3673),
+  update_source (3875-3947) and update_timestep (4308-4461: end-of-run, flow
+  CFL, dt_max, output-time, assimilation-time, flow-file and RFE cuts),
+* flux_transport_step (4463-4507) with the source steps when sources are on,
 * apply_data_assimilation (4148-4306),
 * analysis_step (3098) and output_histories (2607-2693), including the extra
   rows written inside diffusion sub-cycles when verbose > 1 (4659-4669).
@@ -50,6 +50,9 @@ DEFAULTS = dict(
     assimilate_data_lat_limits=None, assimilate_data_mult_fac=1.0, assimilate_data_add=False,
     assimilate_data_balance_flux=False, assimilate_data_map_list_filename=None,
     assimilate_data_map_root_dir=".", base_dir=".",
+    advance_source=False, source_rfe_model=0, source_rfe_total_unsigned_flux_per_hour=0.0,
+    source_rfe_total_unsigned_flux_per_hours=None, source_rfe_seed=-9999,
+    source_rfe_lifetime=0.3, source_from_file=False, source_filename=None, source_map=None,
 )
 
 
@@ -146,6 +149,24 @@ def run(cfg, max_steps=None, mode="coupled"):
         c["assimilation_maps"] = al.map
     assim_idx = 0
     assimilate = bool(c["assimilate_data"]) and len(assim_times) > 0
+    rfe = None
+    if c["advance_source"]:
+        # load_source (7050-7217)
+        from . import sources
+        src = c["source_map"]
+        if src is None and c["source_from_file"]:
+            from . import h5io
+            sf, sc = h5io.read_map(resolve(c["source_filename"]))
+            # interp2d of the transposed file map (theta, phi) onto t x p (7100-7104)
+            p_full = np.concatenate([R.p, [R.p[1] + 2.0 * math.pi]])
+            src = sources.interp2d(sc[1], sc[0], sf, R.t, p_full).T
+        if src is not None:
+            m.set_source(src)
+        if c["source_rfe_model"] > 0:
+            rfe = sources.RandomFluxEmergence(
+                m, c["source_rfe_model"],
+                per_realization(c, "source_rfe_total_unsigned_flux_per_hour", nr),
+                c["source_rfe_lifetime"], c["source_rfe_seed"], c["time_start"])
 
     def want(time):
         cad = c["output_history_time_cadence"]
@@ -216,6 +237,8 @@ def run(cfg, max_steps=None, mode="coupled"):
             if c["flow_attenuate"]:
                 m.set_flow_attenuation(coef["flow_attenuate_value"])
                 st["flow_needs"] = True
+        if rfe is not None:
+            rfe.update(time)  # update_source (3875-3947)
         if st["ts_needs"]:
             st["dtime"] = c["time_end"] - time
             if c["advance_flow"]:
@@ -239,6 +262,16 @@ def run(cfg, max_steps=None, mode="coupled"):
             st["dtime"] = assim_times[assim_idx] - time
             st["ts_needs"] = True
             st["tsflow_needs"] = True
+        if flow_files is not None and time + st["dtime"] >= flow_files.time_next:
+            st["dtime"] = flow_files.time_next - time
+            st["ts_needs"] = True
+            st["tsflow_needs"] = True
+            st["flow_needs"] = True
+        if rfe is not None:
+            st["dtime"], cut = rfe.cut(time, st["dtime"])
+            if cut:
+                st["ts_needs"] = True
+                st["tsflow_needs"] = True
         if time + st["dtime"] >= c["time_end"]:
             st["dtime"] = c["time_end"] - time
             st["ts_needs"] = False

@@ -85,6 +85,7 @@ LIST_KEYS = {
     "flow_dr_coef_p4_values", "flow_mf_coef_p1_values", "flow_mf_coef_p3_values",
     "flow_mf_coef_p5_values", "flow_attenuate_values", "assimilate_data_mu_powers",
     "assimilate_data_mu_limits", "assimilate_data_lat_limits",
+    "source_rfe_total_unsigned_flux_per_hours",
 }
 
 

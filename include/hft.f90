@@ -71,6 +71,34 @@ module hft
       type(c_ptr), value :: ctx, vt, vp
       integer(c_int), value :: per_realization
     end function
+    integer(c_int) function hft_set_source(ctx, src) bind(c, name="hft_set_source")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, src
+    end function
+    integer(c_int) function hft_rfe_generate(ctx, slot, rnd, model, uf_per_hour) &
+        bind(c, name="hft_rfe_generate")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: slot, model
+      real(c_double), intent(in) :: rnd(*), uf_per_hour(*)
+    end function
+    integer(c_int) function hft_rfe_weights(ctx, first_slot, alpha) bind(c, name="hft_rfe_weights")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: first_slot
+      real(c_double), value :: alpha
+    end function
+    integer(c_int) function hft_get_rfe(ctx, slot, out) bind(c, name="hft_get_rfe")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: slot
+      real(c_double), intent(out) :: out(*)
+    end function
+    integer(c_int) function hft_source_step(ctx, dt) bind(c, name="hft_source_step")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: dt
+    end function
     integer(c_int) function hft_set_flow_attenuation(ctx, value) bind(c, name="hft_set_flow_attenuation")
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx, value

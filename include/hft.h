@@ -141,8 +141,10 @@ int hft_dt_euler(hft_ctx *ctx, double *dt, double *dt_r);
 
 /* ---- time step ------------------------------------------------------------
  * flux_transport_step, src/hipft.f90:4463-4507: Strang A(dt/2) D(dt) A(dt/2)
- * when strang != 0, else A(dt) D(dt).  subcycles = diffusion_subcycles
- * (0 = automatic PTL sub-cycling).  cb/user may be NULL. */
+ * when strang != 0, else A(dt) D(dt); with a source term set (below),
+ * S(dt/2) A(dt/2) D(dt) A(dt/2) S(dt/2), else A(dt) D(dt) S(dt).
+ * subcycles = diffusion_subcycles (0 = automatic PTL sub-cycling).  cb/user
+ * may be NULL. */
 int hft_flux_transport_step(hft_ctx *ctx, double dt, int strang, int subcycles,
                             hft_cycle_fn cb, void *user);
 /* advection_step (SSPRK(4,3) + WENO3, src/hipft.f90:4509-4549, 7515-7582).
@@ -155,6 +157,31 @@ int hft_diffusion_step(hft_ctx *ctx, double dt, int subcycles, hft_cycle_fn cb, 
 /* Last diffusion step summary: per realization stage count of the last
  * cycle (s[nr]) and number of cycles (cycles[nr]); either may be NULL. */
 int hft_diffusion_stats(hft_ctx *ctx, int64_t *s, int64_t *cycles);
+
+/* ---- sources ---------------------------------------------------------------
+ * The source term of update_source (src/hipft.f90:3875-3947):
+ *   source = ((0 + map) + (1 - alpha) rfe[first]) + alpha rfe[1 - first]
+ * with absent terms left out; source_step (4697-4733) is f += dt * source.
+ * hft_set_source: a constant map (source_from_file after its interpolation
+ * onto the mesh, 7074-7124), reference storage (ntm, npm), theta fastest,
+ * G/hr; shared by all realizations; NULL removes it. */
+int hft_set_source(hft_ctx *ctx, const double *src);
+/* Random flux emergence frame into slot 0 or 1 (generate_rfe 8809-8921 and
+ * the balance_flux of load_source / update_source): rnd is the reference's
+ * random map rnd_nums(1:ntm, 2:npm-1) (already times pi/sqrt(2 pi)),
+ * theta fastest; model 1 scales by the unsigned flux per cell, model 2 per
+ * area, uf_per_hour[nr] = source_rfe_total_unsigned_flux_per_hour(s) in
+ * 1e21 Mx/hr (7167-7194); pole rows are phi averages; each realization's
+ * frame is flux balanced. */
+int hft_rfe_generate(hft_ctx *ctx, int slot, const double *rnd, int model,
+                     const double *uf_per_hour);
+/* Blend of the two frames in the source term: (1 - alpha) rfe[first_slot] +
+ * alpha rfe[1 - first_slot]; first_slot = -1 leaves the frames out. */
+int hft_rfe_weights(hft_ctx *ctx, int first_slot, double alpha);
+/* Frame of a slot, TP layout (ntm, npm, nr). */
+int hft_get_rfe(hft_ctx *ctx, int slot, double *out_tp);
+/* source_step: f += dt * source. */
+int hft_source_step(hft_ctx *ctx, double dt);
 
 /* ---- operators (for tests and adjoint checks) -----------------------------
  * y = A x for the whole batch, TP layout, x and y host arrays. */

@@ -1000,6 +1000,68 @@ void orc_balance_flux(const orc *o, double *map) {
   }
 }
 
+/* ---- sources: generate_rfe (src/hipft.f90:8809-8921) with the balance_flux
+ * of load_source / update_source (7203-7208, 3932-3935), and source_step
+ * (4697-4733).  rnd is rnd_nums(1:ntm, 2:npm-1), theta fastest (the random
+ * numbers already times pi/sqrt(2 pi), 8842-8846); uf_per_hour[nr] in
+ * 1e21 Mx/hr (7167-7194); out (ntm,npm,nr). */
+void orc_rfe_frame(const orc *o, const double *rnd, int model, const double *uf_per_hour,
+                   double *out) {
+  const double rsun_cm2 = 4.84416e21, input_flux_fac = 1.0e21;
+  int ntm = o->ntm, npm = o->npm, nr = o->nr;
+  int num_cells = ntm * (npm - 2);
+  for (int i = 1; i <= nr; i++) {
+    double sc;
+    if (model == 1)
+      sc = uf_per_hour[i - 1] * input_flux_fac / (rsun_cm2 * (double)num_cells);
+    else
+      sc = uf_per_hour[i - 1] * input_flux_fac / (rsun_cm2 * four * pi);
+    for (int k = 2; k <= npm - 1; k++)
+      for (int j = 1; j <= ntm; j++) {
+        double rn = rnd[(j - 1) + (size_t)ntm * (k - 2)];
+        double v;
+        if (model == 1) {
+          double da_ti;
+          if (j == 1) {
+            double tav = half * (o->t[1] + o->t[2]);
+            da_ti = one / (quarter * sin(tav) * o->dth[2]);
+          } else if (j == ntm) {
+            double tav = half * (o->t[ntm] + o->t[ntm - 1]);
+            da_ti = one / (quarter * sin(tav) * o->dth[ntm]);
+          } else {
+            da_ti = o->st_i[j] * o->dt_i[j];
+          }
+          v = rn * sc * da_ti * o->dp_i[k];
+        } else {
+          v = rn * sc;
+        }
+        FF(o, out, j, k, i) = v;
+      }
+    for (int j = 1; j <= ntm; j++) {
+      FF(o, out, j, 1, i) = FF(o, out, j, npm - 1, i);
+      FF(o, out, j, npm, i) = FF(o, out, j, 2, i);
+    }
+    double fn1 = zero, fs1 = zero;
+    for (int k = 1; k <= npm - 2; k++) {
+      fn1 = fn1 + FF(o, out, 1, k, i) * o->dp[k];
+      fs1 = fs1 + FF(o, out, ntm, k, i) * o->dp[k];
+    }
+    fn1 = fn1 * twopi_i;
+    fs1 = fs1 * twopi_i;
+    for (int k = 1; k <= npm; k++) {
+      FF(o, out, 1, k, i) = fn1;
+      FF(o, out, ntm, k, i) = fs1;
+    }
+    orc_balance_flux(o, &FF(o, out, 1, 1, i));
+  }
+}
+
+/* f += dt * src, src (ntm,npm,nr). */
+void orc_source_step(orc *o, double dtl, const double *src) {
+  size_t n = (size_t)o->ntm * o->npm * o->nr;
+  for (size_t q = 0; q < n; q++) o->f[q] = o->f[q] + dtl * src[q];
+}
+
 /* ---- apply_data_assimilation + assimilate_new_data -----------------------
  * src/hipft.f90:4066-4079, 4083-4146, 4242-4275 */
 void orc_assimilate(orc *o, const double *data_in, int custom_from_mu, const double *mu_power,

@@ -84,6 +84,12 @@ void orc_assimilate(orc *o, const double *data, int custom_from_mu, const double
 void orc_get_flux(const orc *o, const double *map /*(ntm,npm)*/, double *fp, double *fm);
 void orc_balance_flux(const orc *o, double *map);
 
+/* Sources: random flux emergence frame (rnd theta fastest over k = 2..npm-1,
+ * out (ntm,npm,nr)) and source_step f += dt * src (ntm,npm,nr). */
+void orc_rfe_frame(const orc *o, const double *rnd, int model, const double *uf_per_hour,
+                   double *out);
+void orc_source_step(orc *o, double dtl, const double *src);
+
 /* Spherical-harmonic validation field Y(np,nt) (Holmes et al. recursion). */
 void orc_tesseral(int m, int l, const double *pvec, const double *tvec, int np, int nt,
                   double *Y);

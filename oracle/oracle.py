@@ -87,6 +87,8 @@ def lib():
                                      ctypes.c_double, ctypes.c_int, ctypes.c_int]
         L.orc_get_flux.argtypes = [ctypes.c_void_p, _D, _D, _D]
         L.orc_balance_flux.argtypes = [ctypes.c_void_p, _D]
+        L.orc_rfe_frame.argtypes = [ctypes.c_void_p, _D, ctypes.c_int, _D, _D]
+        L.orc_source_step.argtypes = [ctypes.c_void_p, ctypes.c_double, _D]
         L.orc_tesseral.argtypes = [ctypes.c_int, ctypes.c_int, _D, _D, ctypes.c_int,
                                    ctypes.c_int, _D]
         _LIB = L
@@ -278,6 +280,27 @@ class Oracle:
         lib().orc_assimilate(self._h, _p(d), int(custom_from_mu), _p(_vec(mu_power, n)),
                              _p(_vec(mu_limit, n)), _p(_vec(lat_limit, n)), float(mult_fac),
                              int(add_mode), int(balance))
+
+    def rfe_frame(self, rnd, model, uf_per_hour):
+        """Random flux emergence frame (nr, npm, ntm) from rnd [npm-2][ntm]."""
+        r = np.ascontiguousarray(rnd, dtype=np.float64)
+        assert r.shape == (self.npm - 2, self.ntm), r.shape
+        out = np.zeros((self.nr, self.npm, self.ntm))
+        lib().orc_rfe_frame(self._h, _p(r), int(model), _p(_vec(uf_per_hour, self.nr)), _p(out))
+        return out
+
+    def source_step(self, dt, src):
+        """f += dt * src, src (nr, npm, ntm)."""
+        a = np.ascontiguousarray(np.broadcast_to(src, (self.nr, self.npm, self.ntm)),
+                                 dtype=np.float64)
+        lib().orc_source_step(self._h, float(dt), _p(a))
+
+    def get_flux(self, map_tp):
+        """(positive, negative) flux of one map (npm, ntm), get_flux order."""
+        a = np.ascontiguousarray(map_tp, dtype=np.float64)
+        fp, fm = np.zeros(1), np.zeros(1)
+        lib().orc_get_flux(self._h, _p(a), _p(fp), _p(fm))
+        return float(fp[0]), float(fm[0])
 
 
 def set_threads(n):

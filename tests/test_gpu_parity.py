@@ -622,6 +622,99 @@ def test_driver_flows_from_files(gpu_available, tmp_path):
     assert_same(R.model.get_state("tp"), o.get_f(), "driver with flows from files")
 
 
+@pytest.mark.parametrize("model", [1, 2])
+def test_rfe_frames_match_oracle(gpu_available, model):
+    """Random flux emergence frames from the same random map: scaling, pole
+    averages (sequential sum as generate_rfe) and flux balance.  The balance
+    ratio comes from block partial sums on the GPU and the sequential get_flux
+    order in the oracle, so frames agree to 1e-14."""
+    nr = 3
+    o, g = pair(45, 91, nr=nr)
+    rng = np.random.default_rng(21)
+    uf = [150.0, 60.0, 300.0]
+    for slot in (0, 1):
+        rnd = (np.pi / np.sqrt(2 * np.pi)) * rng.standard_normal((o.npm - 2, o.ntm))
+        g.rfe_generate(slot, rnd, model, uf)
+        assert_same(g.get_rfe(slot), o.rfe_frame(rnd, model, uf), f"RFE frame model {model}",
+                    1e-14)
+
+
+def _source_setup(o, g, nr, seed):
+    rng = np.random.default_rng(seed)
+    src = 1e-3 * rng.standard_normal((o.npm, o.ntm))
+    src[0], src[-1] = src[-2], src[1]
+    g.set_source(src)
+    for slot in (0, 1):
+        rnd = (np.pi / np.sqrt(2 * np.pi)) * rng.standard_normal((o.npm - 2, o.ntm))
+        g.rfe_generate(slot, rnd, 1, [100.0 + 50.0 * r for r in range(nr)])
+    fr = [g.get_rfe(0), g.get_rfe(1)]
+    return src, fr
+
+
+def _blend(src, fr, first, alpha):
+    """update_source's source term: ((0 + map) + (1 - alpha) rfe1) + alpha rfe2."""
+    s = 0.0 + src[None]
+    s = s + (1.0 - alpha) * fr[first]
+    return s + alpha * fr[1 - first]
+
+
+def test_source_step_bitexact(gpu_available):
+    nr = 2
+    o, g = pair(45, 91, nr=nr)
+    src, fr = _source_setup(o, g, nr, 5)
+    for first, alpha, dt in ((0, 0.25, 0.7), (1, 0.8, 1.3)):
+        g.rfe_weights(first, alpha)
+        g.source_step(dt)
+        o.source_step(dt, _blend(src, fr, first, alpha))
+        assert_same(g.get_state("tp"), o.get_f(), f"source step first={first}")
+    g.rfe_weights(-1, 0.0)
+    g.source_step(0.5)
+    o.source_step(0.5, np.broadcast_to(0.0 + src[None], (nr,) + src.shape))
+    assert_same(g.get_state("tp"), o.get_f(), "source step, map only")
+
+
+def test_flux_transport_step_with_sources(gpu_available):
+    """Strang splitting with sources: S(dt/2) A(dt/2) D(dt) A(dt/2) S(dt/2)
+    (flux_transport_step, src/hipft.f90:4490-4499), bit for bit."""
+    nr = 2
+    o, g = pair(61, 121, nr=nr)
+    _flows(o, g, nr)
+    eta = [300.0, 700.0]
+    o.load_diffusion(eta)
+    g.set_diffusion(eta)
+    src, fr = _source_setup(o, g, nr, 9)
+    g.rfe_weights(1, 0.4)
+    s = _blend(src, fr, 1, 0.4)
+    dt = 1.0
+    g.flux_transport_step(dt, strang=True, subcycles=0)
+    o.source_step(0.5 * dt, s)
+    o.advection_step(0.5 * dt)
+    o.diffusion_step(dt, 0)
+    o.advection_step(0.5 * dt)
+    o.source_step(0.5 * dt, s)
+    assert_same(g.get_state("tp"), o.get_f(), "Strang step with sources")
+
+
+def test_driver_random_flux_emergence(gpu_available):
+    """A run with random flux emergence (model 1): frames renewed every
+    lifetime with time steps cut to at most half a lifetime (update_timestep
+    4431-4442); the balanced sources keep the signed flux at round-off while
+    the unsigned flux grows; the same seed gives the same run."""
+    cfg = dict(res_nt=31, res_np=61, n_realizations=2, advance_source=True, source_rfe_model=1,
+               source_rfe_total_unsigned_flux_per_hours=[200.0, 50.0], source_rfe_lifetime=0.3,
+               source_rfe_seed=12345, advance_diffusion=True,
+               diffusion_coef_constants=[300.0, 300.0], strang_splitting=True, time_end=1.5,
+               dt_max=1.0)
+    sol, num, R = driver.run(cfg)
+    dts = num[0][1:, 2]
+    assert dts.max() <= 0.15 + 1e-12 and abs(num[0][-1, 1] - 1.5) < 1e-12
+    fp, fm = sol[:, -1, 5], sol[:, -1, 6]
+    assert np.all(np.abs(fp + fm) <= 1e-9 * fp)
+    assert fp[0] > 2.0 * fp[1] > 0.0  # four times the budget, more unsigned flux
+    sol2, _, _ = driver.run(cfg)
+    assert np.array_equal(sol, sol2)
+
+
 def test_fortran_driver_matches_python(gpu_available, tmp_path):
     """The Fortran time loop (hipft_amd/fortran/hft_run.f90, through the
     iso_c_binding module include/hft.f90) gives the same history rows and
