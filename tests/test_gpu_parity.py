@@ -565,10 +565,11 @@ def test_driver_writes_map_files(gpu_available, tmp_path):
 
 def test_driver_flows_from_files(gpu_available, tmp_path):
     """The driver with use_flow_from_files: a flow list of three phi-dependent
-    flow pairs (HDF5, m/s) switched at 3 h and 6 h and wrapped at 9 h, plus
-    analytic differential rotation, Strang splitting and diffusion.  Replayed
-    on the oracle step by step (same dt sequence, same active flow): the CFL
-    limit of every step and the final state agree bit for bit."""
+    flow pairs (HDF5, m/s) switched at 2.4 h and 6 h and wrapped at 9.6 h
+    (time steps cut to those times, update_timestep 4410-4420), plus analytic
+    differential rotation, Strang splitting and diffusion.  Replayed on the
+    oracle step by step (same dt sequence, same active flow): the CFL limit of
+    every step and the final state agree bit for bit."""
     from hipft_amd import h5io, validation as V
     from hipft_amd.inputs import FlowFiles
     s1, s2 = V.uniform_scales(31, 61)
@@ -589,7 +590,7 @@ def test_driver_flows_from_files(gpu_available, tmp_path):
         names.append((f"vt{i}.h5", f"vp{i}.h5"))
     with open(tmp_path / "flows.csv", "w") as fh:
         fh.write("DAY        VT          VP\n")
-        for d, (a, b) in zip([0.0, 0.125, 0.25], names):
+        for d, (a, b) in zip([0.0, 0.1, 0.25], names):
             fh.write(f"{d:11.5f} {a:11s} {b:11s}\n")
     nr = 2
     cfg = dict(res_nt=31, res_np=61, n_realizations=nr, validation_run=1, advance_flow=True,
@@ -619,6 +620,9 @@ def test_driver_flows_from_files(gpu_available, tmp_path):
         o.advection_step(0.5 * dt)
         time = time + dt
     assert used == {0, 1, 2}
+    times = list(num[0][:, 1])
+    for tc in (2.4, 6.0, 9.6):
+        assert any(abs(x - tc) < 1e-12 for x in times), (tc, times)
     assert_same(R.model.get_state("tp"), o.get_f(), "driver with flows from files")
 
 
