@@ -1,0 +1,215 @@
+"""GPU parity at the shapes of the five benchmark configurations.
+
+1. 181x360 uniform, one realization, 6 bipoles, flows, eta = 500, Strang,
+   dt = 1 h: two days against the oracle, bit for bit.
+2. 512x1024 clustered, one realization, 200 bipoles: one day, then a
+   mu^4-weighted assimilation of the state plus 5 G noise within 60 degrees
+   of disk centre; bit for bit before the assimilation, 1e-13 after (pow).
+3. The 64-realization benchmark batch (independent stage counts): one step,
+   three realizations (smallest, median, largest eta) against single-
+   realization oracle runs, bit for bit.
+4. The adjoint configuration (256x512, 16 realizations): gradient of a final
+   misfit over 20 steps against central differences of the forward HIP run
+   (h = 1e-6 relative on map directions, 1e-5 on eta), agreement 1e-6.
+5. The gradient-assimilation shape (512x1024, 128 realizations, 20 bipoles
+   a map): a two-step window, the eta gradient of two realizations against
+   central differences, with the device memory the batch and its adjoint
+   hold.
+"""
+import math
+import os
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+from hipft_amd import workloads as W
+from hipft_amd.core import FluxTransport
+from oracle.oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def assert_same(a, b, what, rtol=0.0):
+    if rtol == 0.0:
+        bad = np.argwhere(a != b)
+        assert bad.size == 0, f"{what}: {len(bad)} entries differ, max |diff| " \
+                              f"{np.abs(a - b).max():.3e}"
+    else:
+        err = np.abs(a - b).max() / np.abs(b).max()
+        assert err <= rtol, f"{what}: max rel error {err:.3e} > {rtol:.1e}"
+
+
+def oracle_with(t, p, maps, vt, dr, eta):
+    """Single- or multi-realization oracle with 1-D meridional profiles vt
+    [nr][nt], DR coefficients dr [nr][3] and diffusivities eta [nr]."""
+    nr = len(eta)
+    o = Oracle(t, p, nr)
+    o.ingest_pt(maps)
+    o.flow_reset()
+    o.flow_set(np.ascontiguousarray(np.broadcast_to(vt[:, None, :], (nr, o.npm, o.nt))),
+               np.zeros((nr, o.np_, o.ntm)))
+    o.flow_add_dr(dr[:, 0], dr[:, 1], dr[:, 2])
+    o.set_lf_alpha()
+    o.load_diffusion(list(eta))
+    return o
+
+
+def gpu_with(t, p, maps, vt, dr, eta, mode):
+    g = FluxTransport(t, p, len(eta), mode=mode)
+    g.set_state(maps, layout="pt")
+    g.set_flow_profiles(vt, None)
+    g.set_flow_analytic(p0=dr[:, 0], p2=dr[:, 1], p4=dr[:, 2])
+    g.set_diffusion(eta)
+    return g
+
+
+def oracle_step(o, dt=1.0):
+    o.advection_step(0.5 * dt)
+    o.diffusion_step(dt, 0)
+    o.advection_step(0.5 * dt)
+
+
+def test_config1_plain_two_days(gpu_available):
+    t, p = W.uniform_theta(181), W.uniform_phi(360)
+    maps = W.bipole_map(t, p, 6, 1, fixed_sigma=3.0, fixed_b=100.0)[None]
+    vt = W.meridional_profile(t, 15.0)[None]
+    dr = np.array([W.dr_coefficients()])
+    eta = np.array([500.0])
+    o = oracle_with(t, p, maps, vt, dr, eta)
+    g = gpu_with(t, p, maps, vt, dr, eta, "coupled")
+    for _ in range(48):
+        oracle_step(o)
+        g.flux_transport_step(1.0, strang=True, subcycles=0)
+    assert_same(g.get_state("tp"), o.get_f(), "181x360, 48 steps")
+    h_g, h_o = g.analysis()[0], o.analysis()[0]
+    assert np.allclose(h_g, h_o, rtol=1e-12, atol=1e-12 * abs(h_o[3]))
+    g.close()
+
+
+def test_config2_day_and_assimilation(gpu_available):
+    t, p = W.clustered_theta(512, 2.0), W.uniform_phi(1024)
+    maps = W.bipole_map(t, p, 200, 2)[None]
+    vt = W.meridional_profile(t, 15.0)[None]
+    dr = np.array([W.dr_coefficients()])
+    eta = np.array([500.0])
+    o = oracle_with(t, p, maps, vt, dr, eta)
+    g = gpu_with(t, p, maps, vt, dr, eta, "coupled")
+    for _ in range(24):
+        oracle_step(o)
+        g.flux_transport_step(1.0, strang=True, subcycles=0)
+    assert_same(g.get_state("tp"), o.get_f(), "512x1024, 24 steps")
+    rng = np.random.default_rng(3)
+    obs = g.get_state("pt")[0] + rng.normal(0.0, 5.0, (len(t), len(p)))
+    d3 = np.stack([obs, np.zeros_like(obs), W.mu_map(t, p, 40.0)])
+    kw = dict(custom_from_mu=True, mu_power=4.0, mu_limit=0.5)
+    o.assimilate(d3[None], **kw)
+    g.assimilate(d3, **kw)
+    assert_same(g.get_state("tp"), o.get_f(), "assimilation at 512x1024", 1e-13)
+    g.close()
+
+
+def test_config3_batch_step_vs_single_runs(gpu_available):
+    sys.path.insert(0, ROOT)
+    import bench
+    wl = bench.make_workload(64)
+    g = bench.setup_model(wl)
+    g.flux_transport_step(1.0, strang=True, subcycles=0)
+    f = g.get_state("tp")
+    s, _ = g.diffusion_stats()
+    order = np.argsort(wl["eta"])
+    for r in (order[0], order[32], order[-1]):
+        o = oracle_with(wl["t"], wl["p"], wl["maps"][r:r + 1], wl["vt"][r:r + 1],
+                        wl["dr"][r:r + 1], wl["eta"][r:r + 1])
+        oracle_step(o)
+        assert_same(f[r:r + 1], o.get_f(), f"benchmark realization {r} (eta {wl['eta'][r]:.0f})")
+    assert s.max() > 2 * s.min()  # the batch's stage counts differ
+    g.close()
+
+
+def _window_model(ntm, npm1, nr, seed, clustered=False, nbip=200):
+    from hipft_amd.adjoint import FlowModel
+    t = W.clustered_theta(ntm, 2.0) if clustered else W.uniform_theta(ntm)
+    p = W.uniform_phi(npm1)
+    vt_unit = np.stack([W.meridional_profile(t, 1.0) for _ in range(nr)])
+    dr_unit = np.array([W.dr_coefficients(1.0) for _ in range(nr)])
+    # default launch shapes: tuning a new grid and batch here would take minutes
+    model = FlowModel(t, p, nr, vt_unit=vt_unit, dr_unit=dr_unit, autotune=False)
+    rng = np.random.default_rng(seed)
+    maps = np.stack([W.bipole_map(t, p, nbip, seed + 1 + r) for r in range(nr)])
+    model.ft.set_state(maps, layout="pt")
+    model.bind_stream()
+    eta = torch.tensor(np.exp(rng.uniform(math.log(100.0), math.log(1000.0), nr)),
+                       dtype=torch.float64)
+    v0 = torch.tensor(rng.uniform(10.0, 20.0, nr), dtype=torch.float64)
+    om = torch.tensor(rng.uniform(0.9, 1.1, nr), dtype=torch.float64)
+    return model, eta, v0, om, rng
+
+
+def test_config4_adjoint_central_differences(gpu_available):
+    from hipft_amd.adjoint import run
+    nsteps = 20
+    model, eta, v0, om, rng = _window_model(256, 512, 16, 77)
+    f0 = model.get()
+    with torch.no_grad():
+        tgt = run(f0, eta * 1.2, v0, om, model, nsteps, checkpoint_every=nsteps)
+        tgt = tgt + torch.as_tensor(rng.normal(0.0, 2.0, tuple(tgt.shape)), device="cuda")
+    x = f0.clone().requires_grad_(True)
+    e = eta.clone().requires_grad_(True)
+    y = run(x, e, v0, om, model, nsteps, checkpoint_every=5)
+    loss = 0.5 * ((y - tgt) ** 2).sum()
+    gx, ge = torch.autograd.grad(loss, (x, e))
+
+    def L(xx, ee):
+        with torch.no_grad():
+            return 0.5 * float(((run(xx, ee, v0, om, model, nsteps, checkpoint_every=nsteps)
+                                 - tgt) ** 2).sum())
+
+    scale = float(f0.abs().max())
+    h = 1e-6
+    for _ in range(4):
+        d = torch.as_tensor(rng.standard_normal(tuple(f0.shape)), device="cuda") * scale
+        fd = (L(f0 + h * d, eta) - L(f0 - h * d, eta)) / (2 * h)
+        ad = float((gx * d).sum())
+        assert abs(fd - ad) <= 1e-6 * abs(ad), (fd, ad)
+    h = 1e-5
+    for r in (0, 9):
+        dv = torch.zeros(16, dtype=torch.float64)
+        dv[r] = h * float(eta[r])
+        fd = (L(f0, eta + dv) - L(f0, eta - dv)) / (2 * float(dv[r]))
+        assert abs(fd - float(ge[r])) <= 1e-6 * abs(float(ge[r])), (r, fd, float(ge[r]))
+    model.ft.close()
+
+
+def test_config5_shape_window_gradient(gpu_available):
+    from hipft_amd.adjoint import run
+    free0, total = torch.cuda.mem_get_info()
+    # 20 bipoles a map: the gradient check does not need 200
+    model, eta, v0, om, rng = _window_model(512, 1024, 128, 350, clustered=True, nbip=20)
+    f0 = model.get()
+    with torch.no_grad():
+        tgt = run(f0, torch.full_like(eta, 350.0), v0, om, model, 2, checkpoint_every=2)
+    e = eta.clone().requires_grad_(True)
+    y = run(f0, e, v0, om, model, 2, checkpoint_every=1)
+    loss = 0.5 * ((y - tgt) ** 2).sum()
+    (ge,) = torch.autograd.grad(loss, (e,))
+    torch.cuda.synchronize()
+    used_gb = (free0 - torch.cuda.mem_get_info()[0]) / 2 ** 30
+    print(f"\n512x1024 x 128 realizations, two-step window: device memory held {used_gb:.1f} GiB")
+    assert used_gb < 0.5 * total / 2 ** 30
+
+    def L(ee):
+        with torch.no_grad():
+            return 0.5 * float(((run(f0, ee, v0, om, model, 2, checkpoint_every=2)
+                                 - tgt) ** 2).sum())
+
+    h = 1e-5
+    for r in (0, 127):
+        dv = torch.zeros(128, dtype=torch.float64)
+        dv[r] = h * float(eta[r])
+        fd = (L(eta + dv) - L(eta - dv)) / (2 * float(dv[r]))
+        assert abs(fd - float(ge[r])) <= 1e-6 * abs(float(ge[r])), (r, fd, float(ge[r]))
+    model.ft.close()
