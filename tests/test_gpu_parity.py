@@ -719,6 +719,24 @@ def test_driver_random_flux_emergence(gpu_available):
     assert np.array_equal(sol, sol2)
 
 
+def test_wide_grid_fallback_kernels(gpu_available):
+    """More than 1024 unique phi columns: the band and fused kernels (one
+    thread or thread pair per column) give way to the pointwise stage
+    kernels; a Strang step still matches the oracle bit for bit."""
+    o, g = pair(33, 1300, nr=2)
+    assert g.npm - 2 > 1024
+    _flows(o, g, 2)
+    eta = [400.0, 900.0]
+    o.load_diffusion(eta)
+    g.set_diffusion(eta)
+    g.set_tuning(fused=1, advection_band=1)
+    g.flux_transport_step(1.0, strang=True, subcycles=0)
+    o.advection_step(0.5)
+    o.diffusion_step(1.0, 0)
+    o.advection_step(0.5)
+    assert_same(g.get_state("tp"), o.get_f(), "Strang step at 33x1300")
+
+
 def test_fortran_driver_matches_python(gpu_available, tmp_path):
     """The Fortran time loop (hipft_amd/fortran/hft_run.f90, through the
     iso_c_binding module include/hft.f90) gives the same history rows and
