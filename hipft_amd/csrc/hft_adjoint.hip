@@ -171,11 +171,7 @@ int superstep_store(hft_ctx *c, const std::vector<int> &reals, const std::vector
       k += 1;
     }
   }
-  for (int r = 0; r < nr; r++)
-    if (final_loc[r])
-      HIPCHK(hipMemcpyAsync(c->f + (size_t)r * c->rstride, final_loc[r] + (size_t)r * c->rstride,
-                            c->rstride * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-  return 0;
+  return copy_finals(c, final_loc);
 }
 
 // Adjoint of one round of RKG2 cycles from its input U0 (realizations of

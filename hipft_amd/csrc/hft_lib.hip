@@ -856,6 +856,8 @@ int launch_fused(hft_ctx *c, int64_t k, int active, const std::vector<int> &ord,
   }
 }
 
+int copy_finals(hft_ctx *c, const std::vector<double *> &final_loc);
+
 // The launches of one RKG2 super-step (tables already uploaded).
 int superstep_launches(hft_ctx *c, int n, const std::vector<int> &ord,
                        const std::vector<int64_t> &s, bool use_af) {
@@ -914,6 +916,13 @@ int superstep_launches(hft_ctx *c, int n, const std::vector<int> &ord,
       k += 1;
     }
   }
+  return copy_finals(c, final_loc);
+}
+
+// Final stages of realizations that finished in fused launches into f
+// (final_loc[r] = the buffer holding realization r's last stage, or null).
+int copy_finals(hft_ctx *c, const std::vector<double *> &final_loc) {
+  int nr = c->nr;
   if (nr <= kCopyMax) {
     CopyFinalsArgs ca;
     ca.dst = c->f;
