@@ -423,8 +423,11 @@ class FluxTransport:
         _check(lib().hft_get_tuning(self._h, key.encode(), ctypes.byref(v)), "hft_get_tuning")
         return v.value
 
-    def set_timing(self, enable):
-        _check(lib().hft_set_timing(self._h, int(bool(enable))), "hft_set_timing")
+    def set_timing(self, enable, phases=False):
+        """Reset the timers; enable: events around every launch, phases:
+        events at the boundaries of the step's phases (the phase_* groups)."""
+        _check(lib().hft_set_timing(self._h, int(bool(enable)) | (2 if phases else 0)),
+               "hft_set_timing")
 
     # ---- device-resident state and the adjoint -----------------------------
     def set_state_device(self, ptr, ld):

@@ -84,11 +84,19 @@ struct TimedPair {
 };
 
 // Kernel timing groups; launches timed under "rkg2_fused" also count in "rkg2".
+// The phase_* groups time whole phases of the step (events at the phase
+// boundaries on the context's stream), the reference's wtime_flux_transport*
+// and wtime_source buckets.
 const char *kGroups[] = {"rkg2", "weno", "ptl", "analysis", "assim", "flow", "rkg2_fused",
-                         "adjoint"};
-constexpr int kNGroups = 8;
+                         "adjoint", "phase_advection", "phase_diffusion", "phase_source",
+                         "phase_flux_transport"};
+constexpr int kNGroups = 12;
 constexpr int kGroupFused = 6;
 constexpr int kGroupAdjoint = 7;
+constexpr int kPhaseAdvection = 8;
+constexpr int kPhaseDiffusion = 9;
+constexpr int kPhaseSource = 10;
+constexpr int kPhaseStep = 11;
 
 }  // namespace
 
@@ -187,7 +195,8 @@ struct hft_ctx {
   int64_t adj_mem_mb = 32768;
   hipStream_t own_stream = nullptr;  // the context's stream when a caller's is in use
   // timing
-  bool timing = false;
+  bool timing = false;        // events around every launch
+  bool phase_timing = false;  // events at the boundaries of the step's phases
   std::vector<TimedPair> pending;
   std::vector<hipEvent_t> free_ev;
   double t_ms[kNGroups] = {0};
@@ -252,6 +261,36 @@ int launch_ex(hft_ctx *c, int group, K kern, int blocks, int threads, size_t shm
     HIPCHK(hipEventRecord(b, c->stream));
     c->pending.push_back({a, b, group, units});
   }
+  return 0;
+}
+
+// Phase timing: phase_begin records the opening event (none when phase
+// timing is off), phase_end the closing one and queues the pair; a failed
+// phase returns its events to the pool.
+int phase_begin(hft_ctx *c, hipEvent_t *a) {
+  *a = nullptr;
+  if (!c->phase_timing) return 0;
+  if (c->pending.size() > 4096) {
+    int rc = drain_timing(c);
+    if (rc) return rc;
+  }
+  int rc = ev_get(c, a);
+  if (rc) return rc;
+  HIPCHK(hipEventRecord(*a, c->stream));
+  return 0;
+}
+
+int phase_end(hft_ctx *c, hipEvent_t a, int group, int body_rc) {
+  if (!a) return body_rc;
+  if (body_rc) {
+    c->free_ev.push_back(a);
+    return body_rc;
+  }
+  hipEvent_t b = nullptr;
+  int rc = ev_get(c, &b);
+  if (rc) return rc;
+  HIPCHK(hipEventRecord(b, c->stream));
+  c->pending.push_back({a, b, group, 0.0});
   return 0;
 }
 
@@ -1480,9 +1519,18 @@ static int advection_substeps(hft_ctx *c, double dtl, std::vector<int> &nsub, in
   return 0;
 }
 
+static int advection_step_impl(hft_ctx *c, double dtl);
+
 int hft_advection_step(hft_ctx *c, double dtl) {
   if (!c) return fail(HFT_E_ARG, "null context");
   if (!c->flow_set) return fail(HFT_E_STATE, "hft_advection_step: no flow set");
+  hipEvent_t ev;
+  int rc = phase_begin(c, &ev);
+  if (rc) return rc;
+  return phase_end(c, ev, kPhaseAdvection, advection_step_impl(c, dtl));
+}
+
+static int advection_step_impl(hft_ctx *c, double dtl) {
   int rc, nmax;
   std::vector<int> nsub;
   if ((rc = advection_substeps(c, dtl, nsub, nmax))) return rc;
@@ -1539,7 +1587,12 @@ static int diffusion_impl(hft_ctx *c, double dtl, int subcycles, hft_cycle_fn cb
                           const std::function<int(int)> *pre, const SuperstepFn *superstep);
 
 int hft_diffusion_step(hft_ctx *c, double dtl, int subcycles, hft_cycle_fn cb, void *user) {
-  return diffusion_impl(c, dtl, subcycles, cb, user, nullptr, nullptr);
+  if (!c) return fail(HFT_E_ARG, "null context");
+  hipEvent_t ev;
+  int rc = phase_begin(c, &ev);
+  if (rc) return rc;
+  return phase_end(c, ev, kPhaseDiffusion,
+                   diffusion_impl(c, dtl, subcycles, cb, user, nullptr, nullptr));
 }
 
 static int diffusion_impl(hft_ctx *c, double dtl, int subcycles, hft_cycle_fn cb, void *user,
@@ -1647,9 +1700,20 @@ int hft_diffusion_stats(hft_ctx *c, int64_t *s, int64_t *cycles) {
 
 static bool sources_active(const hft_ctx *c);
 
+static int flux_transport_impl(hft_ctx *c, double dtl, int strang, int subcycles,
+                               hft_cycle_fn cb, void *user);
+
 int hft_flux_transport_step(hft_ctx *c, double dtl, int strang, int subcycles, hft_cycle_fn cb,
                             void *user) {
   if (!c) return fail(HFT_E_ARG, "null context");
+  hipEvent_t ev;
+  int rc = phase_begin(c, &ev);
+  if (rc) return rc;
+  return phase_end(c, ev, kPhaseStep, flux_transport_impl(c, dtl, strang, subcycles, cb, user));
+}
+
+static int flux_transport_impl(hft_ctx *c, double dtl, int strang, int subcycles,
+                               hft_cycle_fn cb, void *user) {
   int rc;
   bool adv = c->flow_set, dif = c->diff_set, src = sources_active(c);
   if (strang) {
@@ -1896,8 +1960,7 @@ int hft_get_rfe(hft_ctx *c, int slot, double *out_tp) {
 
 static bool sources_active(const hft_ctx *c) { return c->src_on || c->rfe_first >= 0; }
 
-int hft_source_step(hft_ctx *c, double dtl) {
-  if (!c) return fail(HFT_E_ARG, "null context");
+static int source_step_impl(hft_ctx *c, double dtl) {
   const double *r1 = c->rfe_first >= 0 ? c->d_rfe[c->rfe_first] : nullptr;
   const double *r2 = c->rfe_first >= 0 ? c->d_rfe[1 - c->rfe_first] : nullptr;
   hipLaunchKernelGGL(k_source_step, dim3(256, c->nr), dim3(kBlock), 0, c->stream, c->f,
@@ -1905,6 +1968,14 @@ int hft_source_step(hft_ctx *c, double dtl) {
                      c->ntm, c->nu, c->pitch, c->rstride);
   HIPCHK(hipGetLastError());
   return 0;
+}
+
+int hft_source_step(hft_ctx *c, double dtl) {
+  if (!c) return fail(HFT_E_ARG, "null context");
+  hipEvent_t ev;
+  int rc = phase_begin(c, &ev);
+  if (rc) return rc;
+  return phase_end(c, ev, kPhaseSource, source_step_impl(c, dtl));
 }
 
 int hft_assimilate_device(hft_ctx *c, const double *map_dev, int per_real, int custom,
@@ -2068,7 +2139,8 @@ int hft_set_timing(hft_ctx *c, int enable) {
     c->t_n[g] = 0;
     c->t_units[g] = 0;
   }
-  c->timing = enable != 0;
+  c->timing = (enable & 1) != 0;
+  c->phase_timing = (enable & 2) != 0;
   return 0;
 }
 

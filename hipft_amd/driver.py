@@ -10,7 +10,16 @@ GPU, producing the reference's history rows:
 * flux_transport_step (4463-4507) with the source steps when sources are on,
 * apply_data_assimilation (4148-4306),
 * analysis_step (3098) and output_histories (2607-2693), including the extra
-  rows written inside diffusion sub-cycles when verbose > 1 (4659-4669).
+  rows written inside diffusion sub-cycles when verbose > 1 (4659-4669),
+* the run's timers (the reference's `timing` module, src/hipft.f90:475-489):
+  host wall-clock for setup, update, data assimilation, flux transport,
+  analysis and I/O; the advection, diffusion and source parts of the flux
+  transport from HIP events at the phase boundaries on the library's stream.
+
+``main(path)`` is the reference's executable: it reads ``hipft.in``, runs the
+case and writes the history files, hipft_realization_parameters.out
+(set_realization_parameters, 2189-2385) and hipft_timing.out (write_timing,
+1414-1552).
 
 History rows are [STEP, TIME, 14 diagnostics] (sol) and [STEP, TIME, DTIME,
 DTIME_ADV_STB, DTIME_ADV_USED, DTIME_DIFF_STB, N_DIFF_PER_STEP,
@@ -18,6 +27,8 @@ N_DIFF_CYCLES] (num), per realization.
 """
 import math
 import os
+import sys
+import time as _clock
 
 import numpy as np
 
@@ -114,10 +125,32 @@ class Run:
         return self.model.analysis(fval_pt=fv)
 
 
+TIMERS = ("total", "setup", "update", "data_assimilation", "flux_transport", "advection",
+          "diffusion", "source", "analysis", "io", "mpi_overhead")
+
+
 def run(cfg, max_steps=None, mode="coupled"):
-    """Run a case; returns (sol[nr][rows][16], num[nr][rows][8], Run)."""
+    """Run a case; returns (sol[nr][rows][16], num[nr][rows][8], Run).
+    Run.timing holds the run's timers in seconds (TIMERS)."""
+    wall0 = _clock.perf_counter()
+    tm = dict.fromkeys(TIMERS, 0.0)
+
+    class _span:  # adds the wall-clock time of a block to a timer
+        def __init__(self, key, sync=False):
+            self.key, self.sync = key, sync
+
+        def __enter__(self):
+            self.t = _clock.perf_counter()
+
+        def __exit__(self, *exc):
+            if self.sync and exc[0] is None:
+                m.synchronize()
+            tm[self.key] += _clock.perf_counter() - self.t
+
     R = Run(cfg, mode)
+    R.timing = tm
     c, m, nr = R.c, R.model, R.nr
+    m.set_timing(False, phases=True)
     st = dict(time=c["time_start"], ntime=0, dtime=0.0, dtmax_flow=0.0, dt_adv_used=0.0,
               dt_euler=0.0, sts_s=np.zeros(nr, dtype=np.int64),
               subcycles=np.full(nr, c["diffusion_subcycles"], dtype=np.int64), idx_hist=0,
@@ -176,7 +209,8 @@ def run(cfg, max_steps=None, mode="coupled"):
         if not want(time):
             return
         st["idx_hist"] += 1
-        h = R.analysis(time)
+        with _span("analysis"):
+            h = R.analysis(time)
         for r in range(nr):
             sol[r].append([st["ntime"], time] + list(h[r]))
             num[r].append([st["ntime"], time, st["dtime"], st["dtmax_flow"], st["dt_adv_used"],
@@ -190,6 +224,10 @@ def run(cfg, max_steps=None, mode="coupled"):
             st["idx_out"] - (c["output_map_idx_start"] - 1)) * cad
 
     def write_map(name):
+        with _span("io"):
+            _write_map(name)
+
+    def _write_map(name):
         # write_map (src/hipft.f90:2801-2917): PT maps, one-point phi overlap
         d = c["output_map_directory"]
         if not d:
@@ -210,9 +248,12 @@ def run(cfg, max_steps=None, mode="coupled"):
         st["idx_out"] += 1
         write_map(f"_idx{st['idx_out']:06d}")
     steps = 0
+    tm["setup"] = _clock.perf_counter() - wall0
     while True:
         st["ntime"] += 1
         time = st["time"]
+        upd = _span("update", sync=True)
+        upd.__enter__()
         if c["advance_flow"] and st["flow_needs"]:
             st["flow_needs"] = False
             st["tsflow_needs"] = True
@@ -289,20 +330,25 @@ def run(cfg, max_steps=None, mode="coupled"):
 
         if c["advance_flow"]:
             st["dt_adv_used"] = dt * 0.5 if c["strang_splitting"] else dt
-        m.flux_transport_step(dt, strang=c["strang_splitting"], subcycles=c["diffusion_subcycles"],
-                              callback=on_cycle if c["advance_diffusion"] else None)
+        upd.__exit__(None, None, None)
+        with _span("flux_transport", sync=True):
+            m.flux_transport_step(dt, strang=c["strang_splitting"],
+                                  subcycles=c["diffusion_subcycles"],
+                                  callback=on_cycle if c["advance_diffusion"] else None)
         if c["advance_diffusion"] and c["diffusion_subcycles"] == 0:
             st["subcycles"] = m.diffusion_stats()[1]
         st["time"] = time + dt
         if assimilate and st["time"] >= assim_times[assim_idx]:
-            mp = c["assimilation_maps"](assim_idx) if callable(c["assimilation_maps"]) \
-                else c["assimilation_maps"][assim_idx]
-            m.assimilate(mp, custom_from_mu=c["assimilate_data_custom_from_mu"],
-                         mu_power=per_realization(c, "assimilate_data_mu_power", nr),
-                         mu_limit=per_realization(c, "assimilate_data_mu_limit", nr),
-                         lat_limit=per_realization(c, "assimilate_data_lat_limit", nr),
-                         mult_fac=c["assimilate_data_mult_fac"], add_mode=c["assimilate_data_add"],
-                         balance=c["assimilate_data_balance_flux"])
+            with _span("data_assimilation", sync=True):
+                mp = c["assimilation_maps"](assim_idx) if callable(c["assimilation_maps"]) \
+                    else c["assimilation_maps"][assim_idx]
+                m.assimilate(mp, custom_from_mu=c["assimilate_data_custom_from_mu"],
+                             mu_power=per_realization(c, "assimilate_data_mu_power", nr),
+                             mu_limit=per_realization(c, "assimilate_data_mu_limit", nr),
+                             lat_limit=per_realization(c, "assimilate_data_lat_limit", nr),
+                             mult_fac=c["assimilate_data_mult_fac"],
+                             add_mode=c["assimilate_data_add"],
+                             balance=c["assimilate_data_balance_flux"])
             assim_idx += 1
             if assim_idx >= len(assim_times):
                 assimilate = False
@@ -314,7 +360,84 @@ def run(cfg, max_steps=None, mode="coupled"):
         if st["time"] >= c["time_end"] or (max_steps is not None and steps >= max_steps):
             break
     write_map("_final")  # 1390
+    for key in ("advection", "diffusion", "source"):
+        tm[key] = m.kernel_time("phase_" + key)[0] * 1e-3
+    m.set_timing(False)
+    tm["total"] = _clock.perf_counter() - wall0
     return np.array(sol), np.array(num), R
+
+
+def _fortran_e(x, width, digits):
+    """Fortran 1pe<width>.<digits>e3."""
+    mant, ex = f"{x:.{digits}E}".split("E")
+    return f"{mant}E{int(ex):+04d}".rjust(width)
+
+
+REALIZATION_PARAMETERS = (
+    "assimilate_data_lat_limit", "assimilate_data_mu_power", "assimilate_data_mu_limit",
+    "flow_dr_coef_p0_value", "flow_dr_coef_p2_value", "flow_dr_coef_p4_value",
+    "flow_mf_coef_p1_value", "flow_mf_coef_p3_value", "flow_mf_coef_p5_value",
+    "flow_attenuate_value", "diffusion_coef_constant", "source_rfe_total_unsigned_flux_per_hour")
+
+
+def write_realization_parameters(cfg, directory="."):
+    """hipft_realization_parameters.out (set_realization_parameters,
+    src/hipft.f90:2346-2383): one row of per-realization parameters for each
+    realization, written only for more than one realization.  Returns the
+    path, or None."""
+    c = dict(DEFAULTS)
+    c.update(cfg)
+    nr = int(c["n_realizations"])
+    if nr <= 1:
+        return None
+    cols = {k: per_realization(c, k, nr) for k in REALIZATION_PARAMETERS}
+    path = os.path.join(directory, "hipft_realization_parameters.out")
+    with open(path, "w") as fh:
+        # FMTI = (100(A41)), FMTR = (I41,a,100(1pe40.16e3,a))
+        fh.write("realization_number".rjust(41) +
+                 "".join((k + "s").rjust(41) for k in REALIZATION_PARAMETERS) + "\n")
+        for i in range(nr):
+            fh.write(f"{i + 1:41d} " +
+                     "".join(_fortran_e(cols[k][i], 40, 16) + " "
+                             for k in REALIZATION_PARAMETERS) + "\n")
+    return path
+
+
+def write_timing(tm, directory="."):
+    """hipft_timing.out (write_timing, src/hipft.f90:1414-1552) for one
+    process: its timers, then the run-time summary (average, minimum,
+    maximum and standard deviation over the one process)."""
+    dash = "-" * 40
+    per = [("Wall clock time:   ", "total"), None, ("--> Setup:         ", "setup"),
+           ("--> Update:        ", "update"), ("--> Data Assim:    ", "data_assimilation"),
+           ("--> Flux transport:", "flux_transport"), ("    --> Advecton:  ", "advection"),
+           ("    --> Diffusion  ", "diffusion"), ("    --> Source:    ", "source"),
+           ("--> Analysis:      ", "analysis"), ("--> I/O:           ", "io"),
+           ("--> MPI overhead:  ", "mpi_overhead")]
+    summ = [("Wall clock time:        ", "total"), ("--> Setup:              ", "setup"),
+            ("--> Update:             ", "update"),
+            ("--> Data Assimilation:  ", "data_assimilation"),
+            ("--> Flux transport:     ", "flux_transport"),
+            ("    --> Advecton:       ", "advection"), ("    --> Diffusion       ", "diffusion"),
+            ("    --> Source:         ", "source"), ("--> Analysis:           ", "analysis"),
+            ("--> I/O:                ", "io"), ("--> MPI overhead:       ", "mpi_overhead")]
+    lines = [dash, " ", f" Processor id = {0:13d}", " ", dash]
+    for item in per:
+        if item is None:
+            lines.append(dash)
+        else:
+            lines.append(item[0].rjust(20) + f"{float(tm[item[1]]):20.2f}")  # (a20,f20.2)
+    lines += [dash, " ", " Run time summary:", " -----------------", " ",
+              " " * 34 + "Avg         Min         Max      S. Dev",
+              " " * 34 + "---         ---         ---      ------"]
+    for label, key in summ:  # (1x,a,4f12.3)
+        v = float(tm[key])
+        lines.append(" " + label + f"{v:12.3f}{v:12.3f}{v:12.3f}{0.0:12.3f}")
+    lines += [" ", dash, " "]
+    path = os.path.join(directory, "hipft_timing.out")
+    with open(path, "w") as fh:
+        fh.write("\n".join(lines) + "\n")
+    return path
 
 
 def write_histories(sol, num, directory, first_index=1):
@@ -328,8 +451,7 @@ def write_histories(sol, num, directory, first_index=1):
                "N_DIFF_PER_STEP", "N_DIFF_CYCLES"]
 
     def e(x):  # Fortran 1pe23.15e3
-        mant, ex = f"{x:.15E}".split("E")
-        return f"{mant}E{int(ex):+04d}".rjust(23)
+        return _fortran_e(x, 23, 15)
 
     os.makedirs(directory, exist_ok=True)
     for r in range(sol.shape[0]):
@@ -343,3 +465,28 @@ def write_histories(sol, num, directory, first_index=1):
             for row in num[r]:
                 fh.write(f"{int(row[0]):10d} " + "".join(e(x) + " " for x in row[1:6]) +
                          f"{int(row[6]):15d} {int(row[7]):15d}\n")
+
+
+def main(path="hipft.in", directory=None, mode="coupled"):
+    """The reference's executable: run the case that the namelist file at
+    path describes (input files relative to the file's folder) and write its
+    output files into directory (default: the current one)."""
+    from . import namelist
+    t0 = _clock.perf_counter()
+    base = os.path.dirname(os.path.abspath(path))
+    directory = directory or "."
+    cfg = namelist.to_run_config(namelist.read(path), base)
+    write_realization_parameters(cfg, directory)
+    sol, num, R = run(cfg, mode=mode)
+    t1 = _clock.perf_counter()
+    write_histories(sol, num, directory)
+    tm = R.timing
+    tm["io"] += _clock.perf_counter() - t1
+    tm["total"] = _clock.perf_counter() - t0
+    write_timing(tm, directory)
+    R.model.close()
+    return sol, num, tm
+
+
+if __name__ == "__main__":
+    main(*sys.argv[1:2])

@@ -252,12 +252,20 @@ int hft_set_tuning(hft_ctx *ctx, const char *key, int value);
 int hft_get_tuning(hft_ctx *ctx, const char *key, int *value);
 
 /* Kernel timing with HIP events on the context's stream.  hft_set_timing
- * resets the counters and turns per-launch events on or off.
+ * resets the counters; enable is a bit set: 1 = events around every launch
+ * (this also turns off hipGraph replay of RKG2 super-steps), 2 = events at the
+ * boundaries of the step's phases only, 0 = off.
  * hft_kernel_time returns, for a kernel group ("rkg2", "weno", "ptl",
  * "analysis", "assim", "flow", "adjoint", or "rkg2_fused" = the multi-stage
  * RKG2 launches, which "rkg2" includes) since the reset: total device time (ms), number of
  * launches, and the algorithmic bytes those launches move (compulsory
- * reads + writes of FP64 fields over the ntm x (npm-2) unique points). */
+ * reads + writes of FP64 fields over the ntm x (npm-2) unique points).
+ * With phase events on, the groups "phase_advection", "phase_diffusion",
+ * "phase_source" and "phase_flux_transport" hold the stream time of every
+ * hft_advection_step, hft_diffusion_step, hft_source_step and
+ * hft_flux_transport_step call (the reference's wtime_flux_transport_advection,
+ * wtime_flux_transport_diffusion, wtime_source and wtime_flux_transport,
+ * src/hipft.f90:4505, 4546, 4692, 4730); their alg_bytes are 0. */
 int hft_set_timing(hft_ctx *ctx, int enable);
 int hft_kernel_time(hft_ctx *ctx, const char *group, double *total_ms, int64_t *launches,
                     double *alg_bytes);

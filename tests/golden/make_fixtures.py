@@ -2,9 +2,10 @@
 """Write the reference-testsuite golden fixtures used by the parity tests.
 
 This script reads DATA files that the reference's own regression suite holds
-(``testsuite/<case>/reference/hipft_history_{sol,num}_r*.out`` and the HDF5
-input maps ``testsuite/<case>/input/*.h5``) and stores them as compact numpy
-archives under ``tests/golden/``.  Nothing from the reference is imported or
+(``testsuite/<case>/reference/hipft_history_{sol,num}_r*.out``, the text
+outputs ``hipft_realization_parameters.out`` and ``hipft_timing.out``, and the
+HDF5 input maps ``testsuite/<case>/input/*.h5``) and stores them as compact
+numpy archives under ``tests/golden/``.  Nothing from the reference is imported or
 executed: the history files are parsed as text and the maps are read through
 the system HDF5 C library (``libhdf5.so``) with ``ctypes``.
 
@@ -114,6 +115,12 @@ def main(ref_root):
             arrays["scale_t"] = read_h5_float32(path, "dim2")
         with open(os.path.join(indir, "hipft.in")) as fh:
             arrays["hipft_in"] = np.array(fh.read())
+        for name, key in (("hipft_realization_parameters.out", "realization_parameters"),
+                          ("hipft_timing.out", "timing")):
+            path = os.path.join(refdir, name)
+            if os.path.exists(path):
+                with open(path) as fh:
+                    arrays[key] = np.array(fh.read())
         out = os.path.join(HERE, f"{case}.npz")
         np.savez_compressed(out, **arrays)
         print(case, {k: v.shape for k, v in arrays.items()}, os.path.getsize(out))

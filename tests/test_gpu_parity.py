@@ -563,6 +563,41 @@ def test_driver_writes_map_files(gpu_available, tmp_path):
     assert not np.array_equal(d0, d)
 
 
+def test_driver_main_writes_reference_outputs(gpu_available, tmp_path):
+    """driver.main, the counterpart of the reference executable, on the
+    diffuse_soccer namelist cut to 48 h: the history files hold the
+    reference's first rows, hipft_realization_parameters.out the four
+    diffusivities, and hipft_timing.out timers that add up (the diffusion
+    phase inside the flux transport, no advection or source time)."""
+    ref = golden("diffuse_soccer")
+    text = str(ref["hipft_in"]).replace("time_end = 672.0", "time_end = 48.0")
+    assert "time_end = 48.0" in text
+    (tmp_path / "hipft.in").write_text(text)
+    out = tmp_path / "out"
+    out.mkdir()
+    sol, num, tm = driver.main(str(tmp_path / "hipft.in"), str(out))
+    n = sol.shape[1]
+    assert num[0, -1, 1] == 48.0
+    assert np.array_equal(num[..., 0], ref["num"][:, :n, 0])
+    err, verr, _ = history_error(sol, ref["sol"][:, :n])
+    assert err <= 5e-12 and verr <= 1e-6
+    for r in range(4):
+        for kind, arr in (("sol", sol), ("num", num)):
+            rows = np.loadtxt(out / f"hipft_history_{kind}_r{r + 1:06d}.out", skiprows=1)
+            np.testing.assert_allclose(rows, arr[r], rtol=1e-15, atol=0)
+    params = np.loadtxt(out / "hipft_realization_parameters.out", skiprows=1)
+    assert np.array_equal(params[:, 11], [100.0, 200.0, 400.0, 800.0])
+    assert 0.0 < tm["diffusion"] <= tm["flux_transport"] + 1e-3
+    assert tm["advection"] == 0.0 and tm["source"] == 0.0
+    assert tm["total"] >= tm["setup"] + tm["flux_transport"]
+    summary = {}
+    for line in (out / "hipft_timing.out").read_text().split("\n"):
+        if line.startswith(" --> Flux transport:    ") or line.startswith(" Wall clock time:    "):
+            summary[line[:25].strip()] = [float(x) for x in line[25:].split()]
+    assert summary["Wall clock time:"][0] == round(tm["total"], 3)
+    assert summary["--> Flux transport:"][:3] == [round(tm["flux_transport"], 3)] * 3
+
+
 def test_driver_flows_from_files(gpu_available, tmp_path):
     """The driver with use_flow_from_files: a flow list of three phi-dependent
     flow pairs (HDF5, m/s) switched at 2.4 h and 6 h and wrapped at 9.6 h
