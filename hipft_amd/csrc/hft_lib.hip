@@ -117,6 +117,10 @@ struct hft_ctx {
   // tuning (hft_set_tuning): fused multi-stage RKG2 launches
   int fused = 1, fused_T = 4, band_rows = 32, fused_min = 8;
   int fused_target_blocks = 0;  // shrink bands while fewer blocks than this (0: off)
+  // 1: each fused launch picks its band count against whole rounds of
+  // co-resident blocks (fused_band_count); 0: bands of band_rows rows
+  int fused_bands = 0;
+  int fused_cap[5] = {0, 0, 0, 0, 0};  // co-resident k_rkg2_fused2<T> blocks on the device
   int fused_kernel = 3;  // 1: k_rkg2_fused, 2: k_rkg2_march, 3: k_rkg2_fused2 (2 columns/thread)
   int adv_band = 1, adv_band_rows = 16;  // banded WENO3 stage kernel
   // independent mode: per-realization CFL sub-steps of the advection step
@@ -839,6 +843,51 @@ int rkg2_superstep(hft_ctx *c, const std::vector<int> &reals, const std::vector<
   return 0;
 }
 
+// Blocks of k_rkg2_fused2<T> that the device holds at once (blocks per CU
+// from the occupancy calculator x CUs), cached per T.
+template <int T>
+int fused2_capacity(hft_ctx *c, int threads, size_t shmem, int *cap) {
+  if (c->fused_cap[T] == 0) {
+    int per = 0, cus = 0;
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_rkg2_fused2<T>, threads, shmem));
+    HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
+    c->fused_cap[T] = std::max(1, per) * std::max(1, cus);
+  }
+  *cap = c->fused_cap[T];
+  return 0;
+}
+
+// Band count of a fused launch over `active` realizations when the device
+// holds `cap` blocks at once.  A block marches its band's rows plus about 3T
+// rows of halo and pipeline fill, one block per slot, so a launch takes
+// rounds of cap blocks; a round's time grows with the rows per band and,
+// by about a third from an idle to a full device, with the blocks sharing
+// HBM (measured on 512 x 1024 bands of 32-128 rows).  Of the band counts
+// whose bands have 16..(kFusedMaxRows - 2T) rows, the one with the least
+// modelled time; a launch whose last round is nearly empty (active * nbands
+// just above a multiple of cap) moves to fewer, taller bands or to more,
+// shorter ones that fill the rounds.
+int fused_band_count(int ntm, int T, int active, int cap) {
+  const double kShare = 0.32;
+  int hmax = kFusedMaxRows - 2 * T;
+  int best = (ntm + hmax - 1) / hmax;
+  double best_t = 1e300;
+  for (int nb = std::max(1, (ntm + hmax - 1) / hmax); nb <= std::max(1, ntm / 16); nb++) {
+    int H = (ntm + nb - 1) / nb;
+    if (H > hmax) continue;
+    int64_t blocks = (int64_t)active * nb;
+    int64_t rounds = (blocks + cap - 1) / cap;
+    int64_t last = blocks - (rounds - 1) * cap;
+    double per = (double)(H + 3 * T);
+    double t = per * ((double)(rounds - 1) * (1.0 + kShare) + 1.0 + kShare * last / cap);
+    if (t < best_t) {
+      best_t = t;
+      best = nb;
+    }
+  }
+  return best;
+}
+
 // One fused launch: stages k..k+T-1 of the first `active` realizations of
 // ord, reading u_{k-1} (pA) and u_{k-2} (pB); the last stage goes to oA (or
 // each realization's final stage, if earlier), the one before to oB, and
@@ -856,9 +905,21 @@ int launch_fused(hft_ctx *c, int64_t k, int active, const std::vector<int> &ord,
     int64_t h = (int64_t)c->ntm * active / c->fused_target_blocks;
     fa.H = (int)std::max<int64_t>(4, std::min<int64_t>(fa.H, h));
   }
+  fa.ncol = ((c->nu + 63) / 64) * 64;
+  if (c->fused_bands && c->fused_kernel == 3) {
+    int threads = ((fa.ncol / 2 + 63) / 64) * 64, cap = 0, rc;
+    size_t sh2 = (size_t)T * 3 * 2 * threads * sizeof(double);
+    switch (T) {
+      case 2: rc = fused2_capacity<2>(c, threads, sh2, &cap); break;
+      case 3: rc = fused2_capacity<3>(c, threads, sh2, &cap); break;
+      default: rc = fused2_capacity<4>(c, threads, sh2, &cap); break;
+    }
+    if (rc) return rc;
+    int nb = fused_band_count(c->ntm, T, active, cap);
+    fa.H = (c->ntm + nb - 1) / nb;
+  }
   fa.nbands = (c->ntm + fa.H - 1) / fa.H;
   fa.k0 = (int)k;
-  fa.ncol = ((c->nu + 63) / 64) * 64;
   fa.A = pA; fa.B = pB; fa.f = c->f; fa.dty0 = dty0; fa.outA = oA; fa.outB = oB;
   fa.D = c->d_D; fa.coef = c->d_coef; fa.nr_table = nr; fa.s = c->d_s; fa.order = c->d_order;
   fa.rstride = c->rstride;
@@ -2085,6 +2146,9 @@ int hft_set_tuning(hft_ctx *c, const char *key, int value) {
   } else if (!strcmp(key, "fused_kernel")) {
     if (value < 1 || value > 3) return fail(HFT_E_ARG, "fused_kernel must be 1, 2 or 3");
     c->fused_kernel = value;
+  } else if (!strcmp(key, "fused_bands")) {
+    if (value < 0 || value > 1) return fail(HFT_E_ARG, "fused_bands must be 0 or 1");
+    c->fused_bands = value;
   } else if (!strcmp(key, "graphs")) {
     c->graphs_on = value != 0;
   } else if (!strcmp(key, "adjoint_segment")) {
@@ -2116,6 +2180,7 @@ int hft_get_tuning(hft_ctx *c, const char *key, int *value) {
   else if (!strcmp(key, "band_rows")) *value = c->band_rows;
   else if (!strcmp(key, "fused_min_realizations")) *value = c->fused_min;
   else if (!strcmp(key, "fused_target_blocks")) *value = c->fused_target_blocks;
+  else if (!strcmp(key, "fused_bands")) *value = c->fused_bands;
   else if (!strcmp(key, "fused_kernel")) *value = c->fused_kernel;
   else if (!strcmp(key, "graphs")) *value = c->graphs_on;
   else if (!strcmp(key, "graph_hits")) *value = (int)c->graph_hits;

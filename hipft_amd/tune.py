@@ -2,9 +2,10 @@
 
 The knobs (``hft_set_tuning``) are the stages per fused launch (temporal
 depth), the theta rows per block (tile height), the number of still-active
-realizations below which stages go one per launch, and the block count below
+realizations below which stages go one per launch, the block count below
 which a fused launch thins its bands (the tail of a batch whose
-realizations have different stage counts).  The WENO3 stages have their own
+realizations have different stage counts), and whether each launch instead
+picks its band count against whole rounds of co-resident blocks.  The WENO3 stages have their own
 knobs (banded or one thread per point, rows per band), timed on advection
 steps.  Results are identical
 for every setting, so the tuner only measures: it times diffusion super-steps
@@ -20,11 +21,12 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CACHE = os.path.join(HERE, "tuning_cache.json")
 
 DIFF_KEYS = ("fused", "fused_stages", "band_rows", "fused_min_realizations",
-             "fused_target_blocks", "fused_kernel")
+             "fused_target_blocks", "fused_kernel", "fused_bands")
 ADV_KEYS = ("advection_band", "advection_band_rows")
 KEYS = DIFF_KEYS + ADV_KEYS
 DEFAULTS = dict(fused=1, fused_stages=4, band_rows=32, fused_min_realizations=8,
-                fused_target_blocks=0, fused_kernel=3, advection_band=1, advection_band_rows=16)
+                fused_target_blocks=0, fused_kernel=3, fused_bands=0, advection_band=1,
+                advection_band_rows=16)
 
 
 def adv_candidates():
@@ -34,7 +36,7 @@ def adv_candidates():
 
 def candidates(nr):
     out = [dict(fused=0, fused_stages=4, band_rows=32, fused_min_realizations=1,
-                fused_target_blocks=0, fused_kernel=3)]
+                fused_target_blocks=0, fused_kernel=3, fused_bands=0)]
     for K in (1, 3):
         for T in (3, 4):
             for H in (32, 64, 128):
@@ -42,7 +44,12 @@ def candidates(nr):
                     for B in (0, 256):
                         out.append(dict(fused=1, fused_stages=T, band_rows=H,
                                         fused_min_realizations=m, fused_target_blocks=B,
-                                        fused_kernel=K))
+                                        fused_kernel=K, fused_bands=0))
+    # band count chosen per launch (two-column kernel only; band_rows unused)
+    for T in (3, 4):
+        for m in sorted({1, 4, 8, 16, 32} & set(range(1, max(nr, 1) + 1))):
+            out.append(dict(fused=1, fused_stages=T, band_rows=32, fused_min_realizations=m,
+                            fused_target_blocks=0, fused_kernel=3, fused_bands=1))
     return out
 
 

@@ -231,6 +231,11 @@ int hft_stream(hft_ctx *ctx, void **stream);
  *   "fused_kernel"           fused launch kernel: 1 = one phi column per
  *                            thread, 2 = the same with lean bookkeeping,
  *                            3 = two phi columns per thread (default)
+ *   "fused_bands"            1 = each fused launch of kernel 3 picks its band
+ *                            count (bands of 16..(160 - 2 x stages) rows) so
+ *                            that its blocks fill whole rounds of the blocks
+ *                            the device holds at once; band_rows and
+ *                            fused_target_blocks are then unused (0)
  *   "advection_band"         1 = WENO3 stages by theta bands computing each
  *                            face once (default); 0 = one thread per point
  *   "advection_band_rows"    theta rows per block of the banded WENO3 (16)

@@ -448,6 +448,43 @@ def test_fused_rkg2_bitexact(gpu_available, T, H, B, kernel):
     assert_same(g.get_state("tp"), o.get_f(), f"fused kernel {kernel} T={T} H={H} B={B}")
 
 
+@pytest.mark.parametrize("T", [3, 4])
+def test_fused_round_aware_bands_bitexact(gpu_available, T):
+    """Band counts chosen per launch against whole rounds of co-resident
+    blocks (fused_bands=1): a 181-row grid (ragged last band, independent
+    stage counts) against the oracle, and the 512x1024 batch against the
+    one-stage-per-launch path, bit for bit."""
+    o, g = pair(181, 240, nr=5, mode="independent")
+    maps0 = g.get_state("pt")
+    eta = [120.0, 300.0, 520.0, 760.0, 990.0]
+    g.set_diffusion(eta)
+    g.set_tuning(fused=1, fused_stages=T, fused_min_realizations=1, fused_kernel=3,
+                 fused_bands=1)
+    g.diffusion_step(6.0, subcycles=0)
+    f = g.get_state("tp")
+    for r in range(5):
+        o1 = Oracle(o.mesh["t"], o.mesh["p"][: o.npm1], 1)
+        o1.ingest_pt(maps0[r:r + 1])
+        o1.load_diffusion([eta[r]])
+        o1.diffusion_step(6.0, subcycles=0)
+        assert_same(f[r:r + 1], o1.get_f(), f"T={T} realization {r}")
+    t, p = grid(512, 1024)
+    nr = 6
+    maps = np.stack([W.bipole_map(t, p, 40, 200 + r) for r in range(nr)])
+    out = []
+    for fused in (0, 1):
+        m = FluxTransport(t, p, nr, mode="independent")
+        m.set_state(maps, layout="pt")
+        m.set_diffusion(np.array([100.0, 180.0, 320.0, 560.0, 800.0, 1000.0]))
+        m.set_tuning(fused=fused, fused_stages=T, fused_min_realizations=1, fused_kernel=3,
+                     fused_bands=fused)
+        m.diffusion_step(1.0, subcycles=0)
+        out.append(m.get_state("tp"))
+        m.close()
+    assert_same(out[1], out[0], f"512x1024 round-aware bands T={T}")
+    g.close()
+
+
 @pytest.mark.parametrize("kernel", [1, 2, 3])
 def test_fused_matches_single_stage_full_size(gpu_available, kernel):
     """512x1024 batch, independent stage counts: fused and one-stage-per-launch

@@ -52,6 +52,8 @@ def test_candidates_cover_knobs():
     assert any(c["fused"] == 0 for c in c1)
     assert all(c["fused_min_realizations"] <= 1 for c in c1 if c["fused"])
     assert {c["fused_kernel"] for c in c64 if c["fused"]} == {1, 3}
+    assert {c["fused_bands"] for c in c64 if c["fused"]} == {0, 1}
+    assert all(c["fused_kernel"] == 3 for c in c64 if c["fused_bands"])
     assert all(set(tune.DIFF_KEYS) <= set(c) for c in c64)
 
 
