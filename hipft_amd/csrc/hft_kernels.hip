@@ -1437,8 +1437,9 @@ __global__ __launch_bounds__(kBlock) void k_advection_stage(AdvArgs a) {
 // registers and computes the face below the current row once, carrying the
 // face above from the previous row.  Phi: each row's split fluxes go to LDS;
 // a thread forms its cell's west face, publishes it in LDS and takes the east
-// face from its neighbour.  Pole rows sum the upwind pole-face fluxes over the
-// full row (fixed wave order).  Expressions are those of advection_op.
+// face from its neighbour after the next row's barrier (one barrier per row).
+// Pole rows sum the upwind pole-face fluxes over the full row (fixed wave
+// order).  Expressions are those of advection_op.
 // ---------------------------------------------------------------------------
 struct AdvBandArgs {
   Mesh m;
@@ -1455,18 +1456,37 @@ struct AdvBandArgs {
   size_t rstride;
 };
 
+// Row-uniform values per row of the band table, and the most rows a band
+// may have (the host caps the band height at kAdvMaxBandRows).
+constexpr int kAdvRowVals = 17;
+constexpr int kAdvMaxBandRows = 128;
+
 template <bool F2D, bool HAS_Y>
 __global__ __launch_bounds__(1024) void k_advection_band(AdvBandArgs a) {
-  extern __shared__ double sh[];  // phi split fluxes LN, LP and faces: 3 x ncol
+  // phi split fluxes LN, LP and west faces, two buffers each by row parity,
+  // the pole-face terms (7 x ncol doubles), the own column's phi WENO
+  // weights (5 x ncol pairs, each thread reads only its own), then the row
+  // table (kAdvRowVals x (H + 6))
+  extern __shared__ double sh[];
   const Mesh &m = a.m;
   const int ncol = a.ncol, nu = m.nu, ntm = m.ntm, nt = m.nt, pitch = m.pitch;
-  double *sLN = sh, *sLP = sh + ncol, *sFP = sh + 2 * ncol;
+  double2 *sW = reinterpret_cast<double2 *>(sh + 7 * ncol);
   int L = xcd_remap(blockIdx.x, gridDim.x);
   int z = L / a.nbands, band = L - z * a.nbands;
   int r = a.order[z];
   int c = threadIdx.x;
   bool col = c < nu;
   int cc = col ? c : 0;
+  {
+    // west face of column cc: weights at reference faces cc (m1) and cc+1 (0)
+    WenoW w = weno_w(m.pCP, m.pMC, m.pMT, m.pCPT, m.pPT, m.pMCT, m.peps, cc, cc + 1);
+    sW[0 * ncol + c] = make_double2(w.mcm, w.cpm);
+    sW[1 * ncol + c] = make_double2(w.mc0, w.cp0);
+    sW[2 * ncol + c] = make_double2(w.ptm, w.mctm);
+    sW[3 * ncol + c] = make_double2(w.mt0, w.cpt0);
+    sW[4 * ncol + c] = make_double2(w.em, w.e0);
+  }
+  const double dpi = m.dp_i[cc + 1];
   FlowAcc<F2D> fa;
   fa.f1 = a.f1;
   fa.f2 = a.f2;
@@ -1481,21 +1501,61 @@ __global__ __launch_bounds__(1024) void k_advection_band(AdvBandArgs a) {
   const double half = 0.5;
   const double cdt = a.dtr ? a.cmul * a.dtr[r] : a.cdt;
   int j0 = band * a.H, j1 = min(ntm, j0 + a.H);
+  // The band's row-uniform values (rows and faces j0-2 .. j1+3) in LDS, read
+  // as broadcasts: waits on them do not wait on the row loads in flight.
+  // k = 0 vt(face), 1 at, 2 vp, 3 ap (1-D flows), 4 st_i, 5 sth, 6 dt_i,
+  // 7..16 the theta WENO weights of the face below row j.
+  const int R0 = j0 - 2, NR = j1 + 4 - R0;
+  double *tb = sh + 17 * ncol;
+  for (int idx = threadIdx.x; idx < kAdvRowVals * NR; idx += blockDim.x) {
+    int k = idx / NR, jj = R0 + (idx - k * NR);
+    double v = 0.0;
+    bool row = jj >= 0 && jj < ntm, face = jj >= 0 && jj < nt;
+    bool wf = jj >= 0 && jj + 1 < nt;
+    switch (k) {
+      case 0: if (!F2D && face) v = a.f1.vt[(size_t)r * nt + jj]; break;
+      case 1: if (!F2D && row) v = a.f1.at[(size_t)r * ntm + jj]; break;
+      case 2: if (!F2D && row) v = a.f1.vp[(size_t)r * ntm + jj]; break;
+      case 3: if (!F2D && row) v = a.f1.ap[(size_t)r * ntm + jj]; break;
+      case 4: if (row) v = m.st_i[jj]; break;
+      case 5: if (face) v = m.sth[jj]; break;
+      case 6: if (row) v = m.dt_i[jj]; break;
+      case 7: if (wf) v = m.tMC[jj]; break;
+      case 8: if (wf) v = m.tCP[jj]; break;
+      case 9: if (wf) v = m.tMC[jj + 1]; break;
+      case 10: if (wf) v = m.tCP[jj + 1]; break;
+      case 11: if (wf) v = m.tPT[jj]; break;
+      case 12: if (wf) v = m.tMCT[jj]; break;
+      case 13: if (wf) v = m.tMT[jj + 1]; break;
+      case 14: if (wf) v = m.tCPT[jj + 1]; break;
+      case 15: if (wf) v = m.teps[jj]; break;
+      default: if (wf) v = m.teps[jj + 1]; break;
+    }
+    tb[idx] = v;
+  }
+  lds_barrier();
+  auto T_ = [&](int k, int jj) { return tb[k * NR + (jj - R0)]; };
+  auto VT = [&](int jf0) { return F2D ? fa.vt(jf0, cc) : T_(0, jf0); };
+  auto AT = [&](int jj) { return F2D ? fa.at(jj, cc) : T_(1, jj); };
   // theta split fluxes of the own column: LN/LP of reference cell jr
-  auto ln_t = [&](int j0r, double xv) { return half * xv * (fa.vt(j0r, cc) + fa.at(j0r, cc)); };
-  auto lp_t = [&](int j0r, double xv) { return half * xv * (fa.vt(j0r + 1, cc) - fa.at(j0r, cc)); };
+  auto ln_t = [&](int j0r, double xv) { return half * xv * (VT(j0r) + AT(j0r)); };
+  auto lp_t = [&](int j0r, double xv) { return half * xv * (VT(j0r + 1) - AT(j0r)); };
   auto xat = [&](int j) { return (j >= 0 && j < ntm) ? x[(size_t)j * pitch + cc] : 0.0; };
   // theta face flux below row j (between rows j and j+1, reference face j+2)
   auto tface = [&](int j, const double *LN4, const double *LP4, double xj, double xj1) -> double {
     int jf = j + 2;  // reference face
     if (jf == 2 || jf == nt - 1) {
       const double one = 1.0;
-      double v = fa.vt(jf - 1, cc);
+      double v = VT(jf - 1);
       double cct = fsign1(v);
       // face 2: (one-cct)*x(row1) + (one+cct)*x(row0); face nt-1: rows ntm-1, ntm-2
       return v * half * ((one - cct) * xj1 + (one + cct) * xj);
     }
-    WenoW w = weno_w(m.tCP, m.tMC, m.tMT, m.tCPT, m.tPT, m.tMCT, m.teps, jf - 2, jf - 1);
+    // weights of weno_w(theta arrays, jf-2, jf-1), from the row table
+    WenoW w;
+    w.mcm = T_(7, j); w.cpm = T_(8, j); w.mc0 = T_(9, j); w.cp0 = T_(10, j);
+    w.ptm = T_(11, j); w.mctm = T_(12, j); w.mt0 = T_(13, j); w.cpt0 = T_(14, j);
+    w.em = T_(15, j); w.e0 = T_(16, j);
     // LN at rows j-1, j, j+1; LP at rows j, j+1, j+2 (slots 0..3 = rows j-1..j+2)
     return weno_face(LN4[0], LN4[1], LN4[2], LP4[1], LP4[2], LP4[3], w);
   };
@@ -1519,62 +1579,103 @@ __global__ __launch_bounds__(1024) void k_advection_band(AdvBandArgs a) {
     }
     ft_up = tface(j0 - 1, LNm, LPm, Xm[1], Xm[2]);
   }
-  for (int j = j0; j < j1; j++) {
+  // Row j is finished one barrier later, when its east phi face (the west
+  // face of column c+1) is in LDS: pj, pX, pfu, pfd, pfw hold its value, theta
+  // faces and west face meanwhile, and its store waits for the end of the
+  // next row.  LDS buffers alternate by row parity, so one barrier per row
+  // separates every write from the reads of it and from the next write to
+  // the same buffer.  Row j+4 of x is loaded during row j (clamped into the
+  // field, straight-line) into one of two registers used in turn, so a wait
+  // for it never waits for the load issued after it.
+  const double dpc = m.dp[cc + 1];
+  auto store = [&](int jr, double v) {
+    if (col) a.out[base + (size_t)jr * pitch + c] = v;
+  };
+  auto value = [&](double xv, double yv, double A) {
+    double v;
+    if (a.write_aop) {
+      v = A;
+    } else if (HAS_Y) {
+      v = a.ca * yv + a.cb * xv;
+      v = v - cdt * A;
+    } else {
+      v = xv - cdt * A;
+    }
+    return v;
+  };
+  auto xload = [&](int jj) { return x[(size_t)min(max(jj, 0), ntm - 1) * pitch + cc]; };
+  bool pend = false;
+  int pj = 0;
+  double pX = 0.0, pfu = 0.0, pfd = 0.0, pfw = 0.0;
+  bool due = false;
+  int dj = 0;
+  double dX = 0.0, dY = 0.0, dA = 0.0;
+  auto row = [&](int j, double xcur, double &xnext) {
+    // xcur: row j+3 of x, loaded during row j-1 (any value when outside)
     // X[1] = row j, X[0] = j-1, X[2] = j+1, X[3] = j+2
     bool pole = (j == 0 || j == ntm - 1);
+    const int b = j & 1;
+    double *sLNb = sh + b * ncol, *sLPb = sh + (2 + b) * ncol, *sFPb = sh + (4 + b) * ncol;
+    double *sPL = sh + 6 * ncol;
     double ft_dn = 0.0;
     if (j < ntm - 1) ft_dn = tface(j, LN4, LP4, X[1], X[2]);
-    double A;
     if (pole) {
-      // upwind pole-face flux of the own column, summed over the row
+      // upwind pole-face flux of the own column, summed over the row below
       double fface = (j == 0) ? ft_dn : ft_up;
-      if (col) sFP[c] = fface * m.dp[c + 1];
-      lds_barrier();
-      double s = wave_pole_sum(nu, [&](int k) { return sFP[k]; });
-      A = (j == 0) ? s * m.bc_flow_n : -s * m.bc_flow_s;
-      lds_barrier();
+      if (col) sPL[c] = fface * dpc;
     } else {
       // phi split fluxes of row j (own cell), shared through LDS
       double xv = X[1];
-      double av = fa.ap(j, cc);
-      sLN[c] = half * xv * (fa.vp(j, cc) + av);
-      sLP[c] = half * xv * (fa.vp(j, wrapc(cc + 1, nu)) - av);
-      lds_barrier();
+      double av = F2D ? fa.ap(j, cc) : T_(3, j);
+      double vw = F2D ? fa.vp(j, cc) : T_(2, j);
+      double ve = F2D ? fa.vp(j, wrapc(cc + 1, nu)) : T_(2, j);
+      sLNb[c] = half * xv * (vw + av);
+      sLPb[c] = half * xv * (ve - av);
+    }
+    lds_barrier();
+    if (pend) {
+      double fpe = sh[(4 + (pj & 1)) * ncol + wrapc(cc + 1, nu)];
+      double sti = T_(4, pj);
+      double at_ = (T_(5, pj + 1) * pfd - T_(5, pj) * pfu) * sti;
+      at_ = at_ * T_(6, pj);
+      double ap_ = (fpe - pfw) * sti;
+      ap_ = ap_ * dpi;
+      due = true;
+      dj = pj;
+      dX = pX;
+      dA = at_ + ap_;
+      if (HAS_Y) dY = a.y[base + (size_t)pj * pitch + cc];
+      pend = false;
+    }
+    xnext = xload(j + 4);  // after the y load, so a wait for y does not wait for it
+    if (pole) {
+      double s = wave_pole_sum(nu, [&](int k) { return sPL[k]; });
+      double yv = HAS_Y ? a.y[base + (size_t)j * pitch + cc] : 0.0;
+      store(j, value(X[1], yv, (j == 0) ? s * m.bc_flow_n : -s * m.bc_flow_s));
+    } else {
       // west face of the own cell: face q = c
       double fpw = 0.0;
       if (col) {
         int c2 = wrapc(cc - 2, nu), c1 = wrapc(cc - 1, nu), cp = wrapc(cc + 1, nu);
-        WenoW w = weno_w(m.pCP, m.pMC, m.pMT, m.pCPT, m.pPT, m.pMCT, m.peps, cc, cc + 1);
-        fpw = weno_face(sLN[c2], sLN[c1], sLN[cc], sLP[c1], sLP[cc], sLP[cp], w);
+        WenoW w;
+        double2 q0 = sW[0 * ncol + c], q1 = sW[1 * ncol + c], q2 = sW[2 * ncol + c];
+        double2 q3 = sW[3 * ncol + c], q4 = sW[4 * ncol + c];
+        w.mcm = q0.x; w.cpm = q0.y; w.mc0 = q1.x; w.cp0 = q1.y; w.ptm = q2.x;
+        w.mctm = q2.y; w.mt0 = q3.x; w.cpt0 = q3.y; w.em = q4.x; w.e0 = q4.y;
+        fpw = weno_face(sLNb[c2], sLNb[c1], sLNb[cc], sLPb[c1], sLPb[cc], sLPb[cp], w);
       }
-      sFP[c] = fpw;
-      lds_barrier();
-      double fpe = sFP[wrapc(cc + 1, nu)];
-      double sti = m.st_i[j];
-      double at_ = (m.sth[j + 1] * ft_dn - m.sth[j] * ft_up) * sti;
-      at_ = at_ * m.dt_i[j];
-      double ap_ = (fpe - fpw) * sti;
-      ap_ = ap_ * m.dp_i[cc + 1];
-      A = at_ + ap_;
-      lds_barrier();
-    }
-    if (col) {
-      size_t off = (size_t)j * pitch + c;
-      double v;
-      if (a.write_aop) {
-        v = A;
-      } else if (HAS_Y) {
-        v = a.ca * a.y[base + off] + a.cb * X[1];
-        v = v - cdt * A;
-      } else {
-        v = X[1] - cdt * A;
-      }
-      a.out[base + off] = v;
+      sFPb[c] = fpw;
+      pend = true;
+      pj = j;
+      pX = X[1];
+      pfu = ft_up;
+      pfd = ft_dn;
+      pfw = fpw;
     }
     // advance the window by one row
     ft_up = ft_dn;
     int jn = j + 3;
-    double xn = xat(jn);
+    double xn = (jn < ntm) ? xcur : 0.0;
 #pragma unroll
     for (int d = 0; d < 3; d++) {
       X[d] = X[d + 1];
@@ -1584,6 +1685,26 @@ __global__ __launch_bounds__(1024) void k_advection_band(AdvBandArgs a) {
     X[3] = xn;
     LN4[3] = (jn < ntm) ? ln_t(jn, xn) : 0.0;
     LP4[3] = (jn < ntm) ? lp_t(jn, xn) : 0.0;
+    if (due) {
+      store(dj, value(dX, dY, dA));
+      due = false;
+    }
+  };
+  double xa = xload(j0 + 3), xb = 0.0;
+  for (int j = j0; j < j1; j += 2) {
+    row(j, xa, xb);
+    if (j + 1 < j1) row(j + 1, xb, xa);
+  }
+  if (pend) {
+    lds_barrier();
+    double fpe = sh[(4 + (pj & 1)) * ncol + wrapc(cc + 1, nu)];
+    double sti = T_(4, pj);
+    double at_ = (T_(5, pj + 1) * pfd - T_(5, pj) * pfu) * sti;
+    at_ = at_ * T_(6, pj);
+    double ap_ = (fpe - pfw) * sti;
+    ap_ = ap_ * dpi;
+    double yv = HAS_Y ? a.y[base + (size_t)pj * pitch + cc] : 0.0;
+    store(pj, value(pX, yv, at_ + ap_));
   }
 }
 

@@ -1525,12 +1525,12 @@ static int advection_stage(hft_ctx *c, const double *x, const double *y, double 
     a.x = x; a.y = y; a.out = out; a.ca = ca; a.cb = cb; a.cdt = cdt;
     a.dtr = dtr; a.cmul = cmul;
     a.write_aop = aop_only ? 1 : 0;
-    a.H = c->adv_band_rows;
+    a.H = std::min(c->adv_band_rows, kAdvMaxBandRows);
     a.nbands = (c->ntm + a.H - 1) / a.H;
     a.ncol = ((c->nu + 63) / 64) * 64;
     a.order = order; a.rstride = c->rstride;
     int blocks = n * a.nbands;
-    size_t shmem = (size_t)3 * a.ncol * sizeof(double);
+    size_t shmem = ((size_t)17 * a.ncol + (size_t)kAdvRowVals * (a.H + 6)) * sizeof(double);
     if (c->flow2d) {
       if (y) return launch_ex(c, 1, k_advection_band<true, true>, blocks, a.ncol, shmem, a, u);
       return launch_ex(c, 1, k_advection_band<true, false>, blocks, a.ncol, shmem, a, u);

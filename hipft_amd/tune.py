@@ -31,7 +31,7 @@ DEFAULTS = dict(fused=1, fused_stages=4, band_rows=32, fused_min_realizations=8,
 
 def adv_candidates():
     return [dict(advection_band=0, advection_band_rows=16)] + \
-        [dict(advection_band=1, advection_band_rows=h) for h in (4, 8, 16, 32)]
+        [dict(advection_band=1, advection_band_rows=h) for h in (8, 16, 32, 64, 128)]
 
 
 def candidates(nr):
