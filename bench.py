@@ -257,9 +257,13 @@ def main():
         return
     for _ in range(args.warmup):
         step(m)
-    # timed region: throughput with per-kernel HIP events on the library's stream
-    m.set_timing(True)
+    # timed region: throughput of the plain step loop
     el = time_steps(m, args.steps)
+    # second pass over as many steps with HIP events around every launch on
+    # the library's stream (per-kernel times for the roofline; the events
+    # themselves cost time, so this pass does not give the throughput)
+    m.set_timing(True)
+    el_ev = time_steps(m, args.steps)
     groups = {}
     for g in ("rkg2", "rkg2_fused", "weno", "ptl", "analysis"):
         ms, n, b = m.kernel_time(g)
@@ -291,7 +295,9 @@ def main():
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "realization-steps/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": round(1e3 * el / args.steps, 3), "higher_is_better": True,
+        "ms_per_step": round(1e3 * el / args.steps, 3),
+        "ms_per_step_with_kernel_events": round(1e3 * el_ev / args.steps, 3),
+        "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"batched: 512x1024 theta-clustered x2, {nr} realizations, FP64, "
                                "dt=1h Strang, WENO3/SSPRK(4,3) + RKG2/PTL, independent "

@@ -38,7 +38,9 @@ enum {
   AB_L0 = 16,
   AB_R = 17,     // 3: rolling stage states of the checkpoint pass
   AB_U1 = 20,
-  AB_COUNT = 21
+  AB_Q3 = 21,    // fourth rotating buffer of the fused sweep
+  AB_L1 = 22,    // lam_1 (fused sweep)
+  AB_COUNT = 23
 };
 
 int fill_dev(hft_ctx *c, double *p, size_t n, double v) {
@@ -81,14 +83,16 @@ int vjp_stage(hft_ctx *c, const double *x, const double *lam, const double *P, d
   a.rstride = c->rstride;
   a.part = c->d_apA;
   int blocks = c->nr * a.nbands;
-  size_t shmem = ((size_t)9 * a.ncol + 16) * sizeof(double);
+  size_t shmem = ((size_t)19 * a.ncol + 16) * sizeof(double);
   double u = 4.0 * c->nr;
-  if (frozen) {
-    if (P) return launch_ex(c, kGroupAdjoint, k_advection_vjp_band<true, true>, blocks, a.ncol, shmem, a, u);
-    return launch_ex(c, kGroupAdjoint, k_advection_vjp_band<true, false>, blocks, a.ncol, shmem, a, u);
-  }
-  if (P) return launch_ex(c, kGroupAdjoint, k_advection_vjp_band<false, true>, blocks, a.ncol, shmem, a, u);
-  return launch_ex(c, kGroupAdjoint, k_advection_vjp_band<false, false>, blocks, a.ncol, shmem, a, u);
+  // blocks of up to 512 threads get the register budget of two waves per SIMD
+  bool b512 = a.ncol <= 512;
+#define HFT_VJP(FZ, HP)                                                                         \
+  (b512 ? launch_ex(c, kGroupAdjoint, k_advection_vjp_band<FZ, HP, 512>, blocks, a.ncol, shmem, a, u) \
+        : launch_ex(c, kGroupAdjoint, k_advection_vjp_band<FZ, HP, 1024>, blocks, a.ncol, shmem, a, u))
+  if (frozen) return P ? HFT_VJP(true, true) : HFT_VJP(true, false);
+  return P ? HFT_VJP(false, true) : HFT_VJP(false, false);
+#undef HFT_VJP
 }
 
 // Adjoint of one SSPRK(4,3) step (forward: X1 = X0 - h W X0, X2 = X1 - h W X1,
@@ -174,11 +178,80 @@ int superstep_store(hft_ctx *c, const std::vector<int> &reals, const std::vector
   return copy_finals(c, final_loc);
 }
 
+// Adjoint of one round over its stored stages with k_rkg2_adj_fused: T
+// transposed stages per launch, realizations in relative steps (all start
+// at their own s_r together; the s-sorted prefix with stages left is
+// active).  lam is the incoming adjoint (level 0 of the first launch) and
+// receives the adjoint of the round's input.
+int cycle_adjoint_fused(hft_ctx *c, const CycleRec &rec, const double *U0, double *lam) {
+  constexpr int T = 4;
+  int nr = c->nr, rc;
+  std::vector<int> ord;
+  if ((rc = upload_rkg2(c, rec.reals, rec.dt, rec.s, ord, &rec.W))) return rc;
+  int n = (int)ord.size();
+  const std::vector<int64_t> &s = rec.s;
+  double *Q[4];
+  for (int i = 0; i < 3; i++)
+    if (!(Q[i] = adj_buf(c, c->adj_fix, AB_LB + i, &rc))) return rc;
+  if (!(Q[3] = adj_buf(c, c->adj_fix, AB_Q3, &rc))) return rc;
+  double *L1 = adj_buf(c, c->adj_fix, AB_L1, &rc);
+  if (!L1) return rc;
+  double *L0 = adj_buf(c, c->adj_fix, AB_L0, &rc);
+  if (!L0) return rc;
+  AdjFusedArgs fa;
+  fa.m = c->m;
+  fa.H = std::min(c->adj_fused_rows, c->ntm);
+  fa.nbands = (c->ntm + fa.H - 1) / fa.H;
+  fa.ncol = ((c->nu + 63) / 64) * 64;
+  fa.ctiles = (c->nu + kBlock - 1) / kBlock;
+  fa.L1 = L1; fa.L0 = L0; fa.u0 = U0; fa.D = c->d_D; fa.coef = c->d_coef; fa.nr_table = nr;
+  fa.s = c->d_s; fa.dt = c->d_cdt; fa.order = c->d_order; fa.rstride = c->rstride;
+  fa.part = c->d_apD;
+  size_t shmem = (size_t)T * 3 * fa.ncol * sizeof(double);
+  const double *A = lam, *B = lam;  // B is not read at m0 = 0
+  for (int64_t m0 = 0;; m0 += T) {
+    int active = 0;  // realizations with stages left: s_r - 1 - m0 >= 1
+    while (active < n && s[ord[active]] - 1 - m0 >= 1) active++;
+    if (active == 0) break;
+    double *oA = nullptr, *oB = nullptr;
+    for (double *x : Q) {
+      if (x == A || x == B) continue;
+      if (!oA) oA = x;
+      else if (!oB) oB = x;
+    }
+    fa.m0 = m0;
+    fa.A = A; fa.B = B; fa.outA = oA; fa.outB = oB;
+    double units = 0;
+    for (int z = 0; z < active; z++)
+      units += 7.0 * (double)std::min<int64_t>(T, s[ord[z]] - 1 - m0);  // as stage by stage
+    rc = fa.ncol <= 512 ? launch_ex(c, kGroupAdjoint, k_rkg2_adj_fused<T, 512>, active * fa.nbands,
+                                    fa.ncol, shmem, fa, units)
+                        : launch_ex(c, kGroupAdjoint, k_rkg2_adj_fused<T, 1024>, active * fa.nbands,
+                                    fa.ncol, shmem, fa, units);
+    if (rc) return rc;
+    A = oA;
+    B = oB;
+  }
+  AdjCloseArgs ca;
+  ca.m = c->m; ca.g = tiles_for(c, n); ca.lam1 = L1; ca.L0 = L0; ca.u0 = U0; ca.D = c->d_D;
+  ca.dt = c->d_cdt; ca.ub1 = c->d_ub1; ca.order = c->d_order; ca.rstride = c->rstride;
+  ca.part = c->d_apD;
+  if ((rc = launch(c, kGroupAdjoint, k_rkg2_adj_close, nblocks(ca.g), ca, 3.0 * n))) return rc;
+  dim3 gi((unsigned)((c->rstride + kBlock - 1) / kBlock), (unsigned)n);
+  hipLaunchKernelGGL(k_copy_reals, gi, dim3(kBlock), 0, c->stream, lam, (const double *)L0,
+                     (const int *)c->d_order, c->rstride);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
 // Adjoint of one round of RKG2 cycles from its input U0 (realizations of
 // the round only; the others pass through unchanged).  Stage states come
 // from rec.W when the forward recomputation kept them, else from segment
 // recomputation.
 int cycle_adjoint(hft_ctx *c, const CycleRec &rec, const double *U0, double *lam) {
+  if (!rec.W.empty() && c->adj_fused && c->nu <= 1024 &&
+      c->adj_fused_rows + 8 <= kFusedMaxRows)
+    return cycle_adjoint_fused(c, rec, U0, lam);
   int nr = c->nr, rc;
   std::vector<int> ord;
   if ((rc = upload_rkg2(c, rec.reals, rec.dt, rec.s, ord))) return rc;
@@ -489,14 +562,16 @@ int adjoint_check(hft_ctx *c, const double *lam_dev, int64_t ld, int subcycles, 
 }  // namespace
 
 void set_adjoint_attributes() {
-  int lds = (9 * 1024 + 16) * (int)sizeof(double);
-  hipFuncSetAttribute((const void *)k_advection_vjp_band<true, true>,
+  hipFuncSetAttribute((const void *)k_rkg2_adj_fused<4, 1024>,
+                      hipFuncAttributeMaxDynamicSharedMemorySize, 4 * 3 * 1024 * (int)sizeof(double));
+  int lds = (19 * 1024 + 16) * (int)sizeof(double);
+  hipFuncSetAttribute((const void *)k_advection_vjp_band<true, true, 1024>,
                       hipFuncAttributeMaxDynamicSharedMemorySize, lds);
-  hipFuncSetAttribute((const void *)k_advection_vjp_band<true, false>,
+  hipFuncSetAttribute((const void *)k_advection_vjp_band<true, false, 1024>,
                       hipFuncAttributeMaxDynamicSharedMemorySize, lds);
-  hipFuncSetAttribute((const void *)k_advection_vjp_band<false, true>,
+  hipFuncSetAttribute((const void *)k_advection_vjp_band<false, true, 1024>,
                       hipFuncAttributeMaxDynamicSharedMemorySize, lds);
-  hipFuncSetAttribute((const void *)k_advection_vjp_band<false, false>,
+  hipFuncSetAttribute((const void *)k_advection_vjp_band<false, false, 1024>,
                       hipFuncAttributeMaxDynamicSharedMemorySize, lds);
 }
 

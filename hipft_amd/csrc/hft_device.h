@@ -33,9 +33,12 @@ struct Mesh {
 // u_k = a*u_{k-1} + b*u_{k-2} + c*u0 + d*L(u_{k-1}) + g*dty0
 // (diffusion_step_sts_cd stage k, src/hipft.f90:5226-5231, with
 //  c = one-uj-vj and d = ubj*dt evaluated on the host in reference order).
+// w: the stored stage u_{k-1} of the forward recomputation (field base),
+// set only for adjoint sweeps over stored stages.
 struct StageCoef {
   double a, b, c, d, g;
-  double pad[3];
+  const double *w;
+  double pad[2];
 };
 
 // Per-realization 1-D flow profiles for phi-independent flows.

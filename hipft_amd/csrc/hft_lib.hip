@@ -196,6 +196,7 @@ struct hft_ctx {
   double *d_apD = nullptr, *d_apA = nullptr, *d_gout = nullptr;
   size_t apA_cap = 0;
   int adj_band_rows = 16, adj_seg = 0;
+  int adj_fused = 1, adj_fused_rows = 16;  // multi-stage RKG2 adjoint over stored stages
   int64_t adj_mem_mb = 32768;
   hipStream_t own_stream = nullptr;  // the context's stream when a caller's is in use
   // timing
@@ -725,7 +726,8 @@ struct CoefCache {
 // count (descending) into ord / d_order, coefficients of stage k of
 // realization r at d_coef[k*nr + r], s, cycle dt and ubj(1) per realization.
 int upload_rkg2(hft_ctx *c, const std::vector<int> &reals, const std::vector<double> &dt2,
-                const std::vector<int64_t> &s, std::vector<int> &ord) {
+                const std::vector<int64_t> &s, std::vector<int> &ord,
+                const std::vector<double *> *W = nullptr) {
   int n = (int)reals.size();
   int nr = c->nr;
   ord = reals;
@@ -762,6 +764,8 @@ int upload_rkg2(hft_ctx *c, const std::vector<int> &reals, const std::vector<dou
       cf.c = one - cache.u[k] - cache.v[k];
       cf.d = cache.ub[k] * dtl;
       cf.g = cache.g[k];
+      // stored stage u_{k-1} (adjoint sweeps over a kept forward super-step)
+      cf.w = (W && (size_t)(k - 1) < W->size()) ? (*W)[k - 1] : nullptr;
     }
     c->h_smalld[r] = dtl;           // cycle dt
     c->h_smalld[nr + r] = cache.ub[1];
@@ -2160,6 +2164,12 @@ int hft_set_tuning(hft_ctx *c, const char *key, int value) {
   } else if (!strcmp(key, "adjoint_band_rows")) {
     if (value < 1 || value > 4096) return fail(HFT_E_ARG, "adjoint_band_rows must be 1..4096");
     c->adj_band_rows = value;
+  } else if (!strcmp(key, "adjoint_fused")) {
+    c->adj_fused = value != 0;
+  } else if (!strcmp(key, "adjoint_fused_rows")) {
+    if (value < 4 || value > kFusedMaxRows - 8)
+      return fail(HFT_E_ARG, "adjoint_fused_rows must be 4..%d", kFusedMaxRows - 8);
+    c->adj_fused_rows = value;
   } else if (!strcmp(key, "advection_substeps")) {
     c->adv_substeps = value != 0;
   } else if (!strcmp(key, "advection_band")) {
@@ -2188,6 +2198,8 @@ int hft_get_tuning(hft_ctx *c, const char *key, int *value) {
   else if (!strcmp(key, "adjoint_segment")) *value = c->adj_seg;
   else if (!strcmp(key, "adjoint_mem_mb")) *value = (int)c->adj_mem_mb;
   else if (!strcmp(key, "adjoint_band_rows")) *value = c->adj_band_rows;
+  else if (!strcmp(key, "adjoint_fused")) *value = c->adj_fused;
+  else if (!strcmp(key, "adjoint_fused_rows")) *value = c->adj_fused_rows;
   else if (!strcmp(key, "advection_substeps")) *value = c->adv_substeps;
   else if (!strcmp(key, "advection_band")) *value = c->adv_band;
   else if (!strcmp(key, "advection_band_rows")) *value = c->adv_band_rows;

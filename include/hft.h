@@ -252,7 +252,13 @@ int hft_stream(hft_ctx *ctx, void **stream);
  *                            in "adjoint_mem_mb", else sqrt(s))
  *   "adjoint_mem_mb"         device memory for stored RKG2 stages (32768)
  *   "adjoint_band_rows"      theta rows per block of the WENO3 adjoint (16)
- * Results do not depend on these settings (bit-identical). */
+ *   "adjoint_fused"          1 = when a cycle's RKG2 stages are stored, its
+ *                            adjoint takes 4 transposed stages per launch
+ *                            (default); 0 = one launch per stage
+ *   "adjoint_fused_rows"     theta rows per block of those launches (16)
+ * Forward results do not depend on these settings (bit-identical); adjoint
+ * results agree to rounding (the adjoint kernels contract multiply-adds and
+ * sum in launch-dependent order). */
 int hft_set_tuning(hft_ctx *ctx, const char *key, int value);
 int hft_get_tuning(hft_ctx *ctx, const char *key, int *value);
 

@@ -131,6 +131,33 @@ def test_segmented_rkg2_adjoint_matches_stored(gpu_available, fused_min):
         assert np.allclose(grads, out[0][1], rtol=1e-12, atol=0)
 
 
+@pytest.mark.parametrize("shape", [(65, 129, 16, 12.0), (67, 700, 8, 1.0), (130, 257, 32, 4.0),
+                                   (97, 193, 4, 4.0), (41, 81, 64, 12.0)])
+def test_fused_rkg2_adjoint_matches_one_stage_launches(gpu_available, shape):
+    """The multi-stage RKG2 adjoint launch over stored stages (T stages per
+    launch, realizations with different stage counts entering together and
+    finishing inside launches) gives the adjoint of the one-stage launches:
+    bands of 4..32 rows, one band over the whole grid, a grid wider than 512
+    phi columns (1024-thread blocks).  Stage counts (oracle): 7/13/19,
+    11/21/31, 15/31/43, 9/17/25, 3/5/7."""
+    ntm, npm1, rows, dt = shape
+    print(f"fused RKG2 adjoint: {ntm}x{npm1}, {rows}-row bands, dt {dt}", flush=True)
+    o, g, ts = build(ntm, npm1, 3, [120.0, 500.0, 1000.0], flow=False)
+    x0 = dev_state(g)
+    rng = np.random.default_rng(3)
+    tgt = torch.as_tensor(rng.standard_normal(tuple(x0.shape)), device="cuda")
+    out = []
+    for fused in (0, 1):
+        g.set_tuning(adjoint_fused=fused, adjoint_fused_rows=rows)
+        _, lam, grads = gpu_step_and_adjoint(g, x0, tgt, dt, False, 1)
+        out.append((lam.cpu(), grads))
+    s_r = g.diffusion_stats()[0]
+    assert len(set(int(s) for s in s_r)) == 3, f"three different stage counts {s_r}"
+    assert rel(out[1][0], out[0][0]) < 1e-13
+    assert np.allclose(out[1][1][:, 0], out[0][1][:, 0], rtol=1e-12, atol=0)
+    g.close()
+
+
 @pytest.mark.parametrize("mem_mb", [32768, 1])
 def test_multi_step_adjoint_matches_step_by_step(gpu_available, mem_mb):
     """hft_steps_adjoint over 3 steps (one storing forward pass, or step
