@@ -263,6 +263,7 @@ struct FusedArgs {
   const int *order;
   size_t rstride;
   double *W[4];        // optional: stage k0+t-1 rows into W[t-1] (null: not kept)
+  const double *cdt;   // [nr] cycle time step (k_rkg2_fused2 forms dty0 = dt L(u0) itself)
 };
 
 constexpr int kFusedMaxRows = 160;  // H + 2T rows of per-row coefficients
@@ -810,9 +811,11 @@ __global__ __launch_bounds__(1024) void k_rkg2_march(FusedArgs a) {
 // ---------------------------------------------------------------------------
 // Prefetch distance of the row loads in steps; each load gets its own slot
 // (2 * distance slots), so a slot is consumed before it is loaded again and
-// the loads stay in flight across the unrolled loop's back edge.
+// the loads stay in flight across the unrolled loop's back edge.  Two steps:
+// with dty0 formed in the kernel (three row loads per step instead of four)
+// the slots fit the register budget of two wavefronts per SIMD.
 #ifndef HFT_FUSED2_PD
-#define HFT_FUSED2_PD 1
+#define HFT_FUSED2_PD 2
 #endif
 constexpr int kPD = HFT_FUSED2_PD;
 constexpr int kNS = 2 * kPD;
@@ -821,8 +824,9 @@ template <int T>
 struct Fused2State {
   double w[T][4][2];
   double p0[T][2];
-  double hu[4][2], hd[4][2];
-  double pf[kNS][4][2];
+  double hu[4][2], hd[4][2];  // u0 and dty0 of rows q-1-d
+  double u0n[2];              // u0 of row q-1 (loaded one step ahead of hu)
+  double pf[kNS][3][2];       // prefetched u_{k0-1} (row q), u_{k0-2} (row q-1), u0 (row q)
 };
 
 // Columns c0, c0+1 of a row (c0 even, c0 < nu <= pitch, pitch even): one
@@ -847,7 +851,7 @@ __device__ __forceinline__ void fused2_step(const FusedArgs &a, Fused2State<T> &
                                             bool col1, int nrow,
                                             double D2, const double (&P1)[2],
                                             const double (&P5)[2], const double *A,
-                                            const double *B, const double *U0, const double *DT,
+                                            const double *B, const double *U0, double dtr,
                                             double *OA, double *OB, const double (&u0_pole)[2],
                                             const double (&dt_pole)[2], const double (*cfs)[5],
                                             double *const *Wst) {
@@ -856,13 +860,12 @@ __device__ __forceinline__ void fused2_step(const FusedArgs &a, Fused2State<T> &
   constexpr int s0 = U & 3, s1 = (U + 3) & 3, s2 = (U + 2) & 3;
   constexpr int p0s = U % kNS, pls = (U + kPD) % kNS;  // slot read now, slot loaded now
   int q = lo0 + i;
-  double a0[2], b1[2], u1[2], d1[2];
+  double a0[2], b1[2], u0q[2];
 #pragma unroll
   for (int e = 0; e < 2; e++) {
     a0[e] = S.pf[p0s][0][e];
     b1[e] = S.pf[p0s][1][e];
-    u1[e] = S.pf[p0s][2][e];
-    d1[e] = S.pf[p0s][3][e];
+    u0q[e] = S.pf[p0s][2][e];
   }
   {
     // prefetch step i+kPD into its own slot.  The loads are
@@ -874,16 +877,19 @@ __device__ __forceinline__ void fused2_step(const FusedArgs &a, Fused2State<T> &
     size_t oa = (size_t)qn * pitch + cl, o = (size_t)rn * pitch + cl;
     load_pair(A + oa, 0, S.pf[pls][0][0], S.pf[pls][0][1]);
     load_pair(B + o, 0, S.pf[pls][1][0], S.pf[pls][1][1]);
-    load_pair(U0 + o, 0, S.pf[pls][2][0], S.pf[pls][2][1]);
-    load_pair(DT + o, 0, S.pf[pls][3][0], S.pf[pls][3][1]);
+    load_pair(U0 + oa, 0, S.pf[pls][2][0], S.pf[pls][2][1]);
   }
 #pragma unroll
   for (int e = 0; e < 2; e++) {
-    S.hu[s0][e] = u1[e];
-    S.hd[s0][e] = d1[e];
+    S.hu[s0][e] = S.u0n[e];  // u0 of row q-1
+    S.u0n[e] = u0q[e];
     S.w[0][s0][e] = a0[e];
   }
+  // u0 ring (after the T stage levels' rings): rows of u0 for the phi
+  // neighbours of dty0 = dt L(u0), formed by stage 1 one row behind
+  double *lu0 = lds + (size_t)(T * 3) * nrow;
   if (q < hi0) {
+    *reinterpret_cast<double2 *>(lu0 + (size_t)(q % 3) * nrow + c0) = make_double2(u0q[0], u0q[1]);
     *reinterpret_cast<double2 *>(lds + (size_t)(q % 3) * nrow + c0) = make_double2(a0[0], a0[1]);
     if (q == 0) {
       S.p0[0][0] = a0[0];
@@ -928,6 +934,41 @@ __device__ __forceinline__ void fused2_step(const FusedArgs &a, Fused2State<T> &
         if (!final_here && t == T - 1) store_pair(OB, c0, nu, vv[0], vv[1]);
         if (Wst[t - 1]) store_pair(Wst[t - 1], c0, nu, vv[0], vv[1]);
       }
+    }
+    if (act && R > 0 && t == 1) {
+      // dty0 = dt L(u0) of row R (as k_rkg2_stage1 / the PTL pass form it:
+      // same coefficient and summation order), kept for the later stages
+      double Lu[2];
+      if (R == ntm - 1) {
+        const double *r0 = lu0 + ((ntm - 1) % 3) * nrow, *r1p = lu0 + ((ntm - 2) % 3) * nrow;
+        double fs = wave_pole_sum(nu, [&](int kk) { return (D2 * (r0[kk] - r1p[kk])) * m.dp[kk + 1]; });
+        Lu[0] = Lu[1] = -fs * m.bc_diff_s;
+      } else {
+        const double *rc = lu0 + (R % 3) * nrow;
+        const double *rr = rowc + 4 * (R - lo0);
+        double sti = rr[0];
+        double c2 = rr[1];
+        double c4 = rr[2];
+        double xw[2], xe[2];
+        xw[0] = rc[cw0];
+        xe[0] = col1 ? S.hu[s0][1] : rc[ce0];
+        xw[1] = S.hu[s0][0];
+        xe[1] = rc[ce1];
+#pragma unroll
+        for (int e = 0; e < 2; e++) {
+          double c1 = (P1[e] * sti) * sti;
+          double c5 = (P5[e] * sti) * sti;
+          double c3 = -(((c1 + c2) + c4) + c5);
+          double L = c1 * xw[e];
+          L = L + c2 * S.hu[s1][e];
+          L = L + c3 * S.hu[s0][e];
+          L = L + c4 * S.u0n[e];
+          L = L + c5 * xe[e];
+          Lu[e] = L;
+        }
+      }
+      S.hd[s0][0] = dtr * Lu[0];
+      S.hd[s0][1] = dtr * Lu[1];
     }
     if (act && R > 0) {
       double Lv[2];
@@ -993,7 +1034,7 @@ __device__ __forceinline__ void fused2_step(const FusedArgs &a, Fused2State<T> &
 
 template <int T>
 __global__ __launch_bounds__(512) void k_rkg2_fused2(FusedArgs a) {
-  extern __shared__ double lds[];  // [T][3][2 * blockDim]
+  extern __shared__ double lds[];  // [T + 1][3][2 * blockDim]: stage levels 0..T-1, then u0
   __shared__ double cfs[4][5];
   __shared__ double rowc[4 * kFusedMaxRows];
   const Mesh &m = a.m;
@@ -1013,13 +1054,14 @@ __global__ __launch_bounds__(512) void k_rkg2_fused2(FusedArgs a) {
   int nst = (int)((sr - a.k0 + 1) < T ? (sr - a.k0 + 1) : T);
   bool final_here = (a.k0 + nst - 1) == sr;
   size_t base = (size_t)r * a.rstride;
-  const double *A = a.A + base, *B = a.B + base, *U0 = a.f + base, *DT = a.dty0 + base;
+  const double *A = a.A + base, *B = a.B + base, *U0 = a.f + base;
   double *OA = a.outA + base, *OB = a.outB + base;
   double *Wst[T];
 #pragma unroll
   for (int t = 0; t < T; t++) Wst[t] = a.W[t] ? a.W[t] + base : nullptr;
   double D = a.D[r];
   double D2 = D + D;
+  double dtr = a.cdt[r];
   double P1[2], P5[2];
   P1[0] = (D * m.dph_i[e0 + 1]) * m.dp_i[e0 + 1];
   P5[0] = (D * m.dph_i[e0 + 2]) * m.dp_i[e0 + 1];
@@ -1067,10 +1109,15 @@ __global__ __launch_bounds__(512) void k_rkg2_fused2(FusedArgs a) {
   for (int d = 0; d < 4; d++)
 #pragma unroll
     for (int e = 0; e < 2; e++) S.hu[d][e] = S.hd[d][e] = 0.0;
+  S.u0n[0] = S.u0n[1] = 0.0;
   double u0_pole[2] = {0.0, 0.0}, dt_pole[2] = {0.0, 0.0};
-  if (lo0 == 0 && c0 < nu) {
-    load_pair(U0, c0, u0_pole[0], u0_pole[1]);
-    load_pair(DT, c0, dt_pole[0], dt_pole[1]);
+  if (lo0 == 0) {
+    // dty0 of the north-pole row, dt L(u0) with the polar closure's sum
+    // (every lane takes part in the wave sum)
+    double fn = wave_pole_sum(nu, [&](int kk) { return (D2 * (U0[m.pitch + kk] - U0[kk])) * m.dp[kk + 1]; });
+    double Lp = fn * m.bc_diff_n;
+    dt_pole[0] = dt_pole[1] = dtr * Lp;
+    if (c0 < nu) load_pair(U0, c0, u0_pole[0], u0_pole[1]);
   }
   int nsteps = (hi0 - lo0) + T;
 #pragma unroll
@@ -1079,24 +1126,23 @@ __global__ __launch_bounds__(512) void k_rkg2_fused2(FusedArgs a) {
     size_t oa = (size_t)q * m.pitch + cl, o = (size_t)rn * m.pitch + cl;
     load_pair(A + oa, 0, S.pf[i][0][0], S.pf[i][0][1]);
     load_pair(B + o, 0, S.pf[i][1][0], S.pf[i][1][1]);
-    load_pair(U0 + o, 0, S.pf[i][2][0], S.pf[i][2][1]);
-    load_pair(DT + o, 0, S.pf[i][3][0], S.pf[i][3][1]);
+    load_pair(U0 + oa, 0, S.pf[i][2][0], S.pf[i][2][1]);
   }
   lds_barrier();
   for (int i = 0; i < nsteps; i += 4) {
     fused2_step<T, 0>(a, S, lds, rowc, i, nsteps, lo0, hi0, j0, j1, nst, final_here, c0, cl, cw0,
-                      ce0, ce1, col1, nrow, D2, P1, P5, A, B, U0, DT, OA, OB, u0_pole, dt_pole, cfs, Wst);
+                      ce0, ce1, col1, nrow, D2, P1, P5, A, B, U0, dtr, OA, OB, u0_pole, dt_pole, cfs, Wst);
     if (i + 1 < nsteps)
       fused2_step<T, 1>(a, S, lds, rowc, i + 1, nsteps, lo0, hi0, j0, j1, nst, final_here, c0,
-                        cl, cw0, ce0, ce1, col1, nrow, D2, P1, P5, A, B, U0, DT, OA, OB, u0_pole, dt_pole,
+                        cl, cw0, ce0, ce1, col1, nrow, D2, P1, P5, A, B, U0, dtr, OA, OB, u0_pole, dt_pole,
                         cfs, Wst);
     if (i + 2 < nsteps)
       fused2_step<T, 2>(a, S, lds, rowc, i + 2, nsteps, lo0, hi0, j0, j1, nst, final_here, c0,
-                        cl, cw0, ce0, ce1, col1, nrow, D2, P1, P5, A, B, U0, DT, OA, OB, u0_pole, dt_pole,
+                        cl, cw0, ce0, ce1, col1, nrow, D2, P1, P5, A, B, U0, dtr, OA, OB, u0_pole, dt_pole,
                         cfs, Wst);
     if (i + 3 < nsteps)
       fused2_step<T, 3>(a, S, lds, rowc, i + 3, nsteps, lo0, hi0, j0, j1, nst, final_here, c0,
-                        cl, cw0, ce0, ce1, col1, nrow, D2, P1, P5, A, B, U0, DT, OA, OB, u0_pole, dt_pole,
+                        cl, cw0, ce0, ce1, col1, nrow, D2, P1, P5, A, B, U0, dtr, OA, OB, u0_pole, dt_pole,
                         cfs, Wst);
   }
 }

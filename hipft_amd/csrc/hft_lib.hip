@@ -912,7 +912,7 @@ int launch_fused(hft_ctx *c, int64_t k, int active, const std::vector<int> &ord,
   fa.ncol = ((c->nu + 63) / 64) * 64;
   if (c->fused_bands && c->fused_kernel == 3) {
     int threads = ((fa.ncol / 2 + 63) / 64) * 64, cap = 0, rc;
-    size_t sh2 = (size_t)T * 3 * 2 * threads * sizeof(double);
+    size_t sh2 = (size_t)(T + 1) * 3 * 2 * threads * sizeof(double);
     switch (T) {
       case 2: rc = fused2_capacity<2>(c, threads, sh2, &cap); break;
       case 3: rc = fused2_capacity<3>(c, threads, sh2, &cap); break;
@@ -927,6 +927,7 @@ int launch_fused(hft_ctx *c, int64_t k, int active, const std::vector<int> &ord,
   fa.A = pA; fa.B = pB; fa.f = c->f; fa.dty0 = dty0; fa.outA = oA; fa.outB = oB;
   fa.D = c->d_D; fa.coef = c->d_coef; fa.nr_table = nr; fa.s = c->d_s; fa.order = c->d_order;
   fa.rstride = c->rstride;
+  fa.cdt = c->d_cdt;
   for (int t = 0; t < 4; t++) fa.W[t] = W[t];
   size_t shmem = (size_t)T * 3 * fa.ncol * sizeof(double);
   double units = 0;
@@ -937,9 +938,10 @@ int launch_fused(hft_ctx *c, int64_t k, int active, const std::vector<int> &ord,
   int blocks = active * fa.nbands;
   bool store = W[0] || W[1] || W[2] || W[3];
   if (c->fused_kernel == 3) {
-    // two phi columns per thread: a multiple of 64 threads, LDS rows of 2 * threads
+    // two phi columns per thread: a multiple of 64 threads, LDS rows of 2 *
+    // threads (3-row rings of the T stage levels and of u0)
     int threads = ((fa.ncol / 2 + 63) / 64) * 64;
-    size_t sh2 = (size_t)T * 3 * 2 * threads * sizeof(double);
+    size_t sh2 = (size_t)(T + 1) * 3 * 2 * threads * sizeof(double);
     switch (T) {
       case 2: return launch_ex(c, kGroupFused, k_rkg2_fused2<2>, blocks, threads, sh2, fa, units);
       case 3: return launch_ex(c, kGroupFused, k_rkg2_fused2<3>, blocks, threads, sh2, fa, units);
@@ -1178,9 +1180,9 @@ int hft_create(int ntm, int npm1, const double *t, const double *p, int nr, int 
     hipFuncSetAttribute((const void *)k_rkg2_march<2>, hipFuncAttributeMaxDynamicSharedMemorySize, 2 * row);
     hipFuncSetAttribute((const void *)k_rkg2_march<3>, hipFuncAttributeMaxDynamicSharedMemorySize, 3 * row);
     hipFuncSetAttribute((const void *)k_rkg2_march<4>, hipFuncAttributeMaxDynamicSharedMemorySize, 4 * row);
-    hipFuncSetAttribute((const void *)k_rkg2_fused2<2>, hipFuncAttributeMaxDynamicSharedMemorySize, 2 * row);
-    hipFuncSetAttribute((const void *)k_rkg2_fused2<3>, hipFuncAttributeMaxDynamicSharedMemorySize, 3 * row);
-    hipFuncSetAttribute((const void *)k_rkg2_fused2<4>, hipFuncAttributeMaxDynamicSharedMemorySize, 4 * row);
+    hipFuncSetAttribute((const void *)k_rkg2_fused2<2>, hipFuncAttributeMaxDynamicSharedMemorySize, 3 * row);
+    hipFuncSetAttribute((const void *)k_rkg2_fused2<3>, hipFuncAttributeMaxDynamicSharedMemorySize, 4 * row);
+    hipFuncSetAttribute((const void *)k_rkg2_fused2<4>, hipFuncAttributeMaxDynamicSharedMemorySize, 5 * row);
     set_adjoint_attributes();
   }
   *out = c;
