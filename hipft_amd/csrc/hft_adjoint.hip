@@ -157,7 +157,7 @@ int superstep_store(hft_ctx *c, const std::vector<int> &reals, const std::vector
     int T = c->fused_T;
     if (fused_ok && active >= c->fused_min) {
       // several stages per launch, each stage's rows also kept in rec.W
-      double *Wt[4] = {nullptr, nullptr, nullptr, nullptr};
+      double *Wt[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
       for (int t = 0; t < T; t++)
         if (k + t < smax) Wt[t] = rec.W[k + t];
       if ((rc = launch_fused(c, k, active, ord, s, pA, pB, DTY, c->buf0, c->buf2, Wt))) return rc;

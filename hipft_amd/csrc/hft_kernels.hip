@@ -262,7 +262,7 @@ struct FusedArgs {
   const int64_t *s;
   const int *order;
   size_t rstride;
-  double *W[4];        // optional: stage k0+t-1 rows into W[t-1] (null: not kept)
+  double *W[5];        // optional: stage k0+t-1 rows into W[t-1] (null: not kept)
   const double *cdt;   // [nr] cycle time step (k_rkg2_fused2 forms dty0 = dt L(u0) itself)
 };
 
@@ -811,22 +811,23 @@ __global__ __launch_bounds__(1024) void k_rkg2_march(FusedArgs a) {
 // ---------------------------------------------------------------------------
 // Prefetch distance of the row loads in steps; each load gets its own slot
 // (2 * distance slots), so a slot is consumed before it is loaded again and
-// the loads stay in flight across the unrolled loop's back edge.  Two steps:
-// with dty0 formed in the kernel (three row loads per step instead of four)
-// the slots fit the register budget of two wavefronts per SIMD.
-#ifndef HFT_FUSED2_PD
-#define HFT_FUSED2_PD 2
-#endif
-constexpr int kPD = HFT_FUSED2_PD;
-constexpr int kNS = 2 * kPD;
+// the loads stay in flight across the unrolled loop's back edge.  Two steps
+// for up to four stages: with dty0 formed in the kernel (three row loads per
+// step instead of four) the slots fit the register budget of two wavefronts
+// per SIMD; five stages keep one.
+template <int T>
+struct Fused2Cfg {
+  static constexpr int PD = T >= 5 ? 1 : 2;
+  static constexpr int NS = 2 * PD;
+};
 
 template <int T>
 struct Fused2State {
   double w[T][4][2];
   double p0[T][2];
-  double hu[4][2], hd[4][2];  // u0 and dty0 of rows q-1-d
+  double hu[4][2], hd[4][2];  // u0 and dty0 of rows q-1-d (d = 0..3)
   double u0n[2];              // u0 of row q-1 (loaded one step ahead of hu)
-  double pf[kNS][3][2];       // prefetched u_{k0-1} (row q), u_{k0-2} (row q-1), u0 (row q)
+  double pf[Fused2Cfg<T>::NS][3][2];  // prefetched u_{k0-1} (row q), u_{k0-2} (row q-1), u0 (row q)
 };
 
 // Columns c0, c0+1 of a row (c0 even, c0 < nu <= pitch, pitch even): one
@@ -858,6 +859,7 @@ __device__ __forceinline__ void fused2_step(const FusedArgs &a, Fused2State<T> &
   const Mesh &m = a.m;
   const int nu = m.nu, ntm = m.ntm, pitch = m.pitch;
   constexpr int s0 = U & 3, s1 = (U + 3) & 3, s2 = (U + 2) & 3;
+  constexpr int kPD = Fused2Cfg<T>::PD, kNS = Fused2Cfg<T>::NS;
   constexpr int p0s = U % kNS, pls = (U + kPD) % kNS;  // slot read now, slot loaded now
   int q = lo0 + i;
   double a0[2], b1[2], u0q[2];
@@ -879,6 +881,9 @@ __device__ __forceinline__ void fused2_step(const FusedArgs &a, Fused2State<T> &
     load_pair(B + o, 0, S.pf[pls][1][0], S.pf[pls][1][1]);
     load_pair(U0 + oa, 0, S.pf[pls][2][0], S.pf[pls][2][1]);
   }
+  // u0 and dty0 of row q-5 (age 4: the slot that rows q-1 take now), for a
+  // fifth stage
+  double hu4[2] = {S.hu[s0][0], S.hu[s0][1]}, hd4[2] = {S.hd[s0][0], S.hd[s0][1]};
 #pragma unroll
   for (int e = 0; e < 2; e++) {
     S.hu[s0][e] = S.u0n[e];  // u0 of row q-1
@@ -1010,9 +1015,9 @@ __device__ __forceinline__ void fused2_step(const FusedArgs &a, Fused2State<T> &
         double ukm2 = (t == 1) ? b1[e] : S.w[t >= 2 ? t - 2 : 0][s2][e];
         double x = k[0] * xc;
         x = x + k[1] * ukm2;
-        x = x + k[2] * S.hu[ha][e];
+        x = x + k[2] * (t <= 4 ? S.hu[ha][e] : hu4[e]);
         x = x + k[3] * Lv[e];
-        x = x + k[4] * S.hd[ha][e];
+        x = x + k[4] * (t <= 4 ? S.hd[ha][e] : hd4[e]);
         v[e] = x;
       }
       if (R >= j0 && R < j1) {
@@ -1035,7 +1040,7 @@ __device__ __forceinline__ void fused2_step(const FusedArgs &a, Fused2State<T> &
 template <int T>
 __global__ __launch_bounds__(512) void k_rkg2_fused2(FusedArgs a) {
   extern __shared__ double lds[];  // [T + 1][3][2 * blockDim]: stage levels 0..T-1, then u0
-  __shared__ double cfs[4][5];
+  __shared__ double cfs[T][5];
   __shared__ double rowc[4 * kFusedMaxRows];
   const Mesh &m = a.m;
   const int nu = m.nu, ntm = m.ntm;
@@ -1121,7 +1126,7 @@ __global__ __launch_bounds__(512) void k_rkg2_fused2(FusedArgs a) {
   }
   int nsteps = (hi0 - lo0) + T;
 #pragma unroll
-  for (int i = 0; i < kPD; i++) {
+  for (int i = 0; i < Fused2Cfg<T>::PD; i++) {
     int q = min(lo0 + i, hi0 - 1), rn = min(max(lo0 + i - 1, 0), ntm - 1);
     size_t oa = (size_t)q * m.pitch + cl, o = (size_t)rn * m.pitch + cl;
     load_pair(A + oa, 0, S.pf[i][0][0], S.pf[i][0][1]);

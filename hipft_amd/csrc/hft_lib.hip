@@ -120,7 +120,7 @@ struct hft_ctx {
   // 1: each fused launch picks its band count against whole rounds of
   // co-resident blocks (fused_band_count); 0: bands of band_rows rows
   int fused_bands = 0;
-  int fused_cap[5] = {0, 0, 0, 0, 0};  // co-resident k_rkg2_fused2<T> blocks on the device
+  int fused_cap[6] = {0, 0, 0, 0, 0, 0};  // co-resident k_rkg2_fused2<T> blocks on the device
   int fused_kernel = 3;  // 1: k_rkg2_fused, 2: k_rkg2_march, 3: k_rkg2_fused2 (2 columns/thread)
   int adv_band = 1, adv_band_rows = 16;  // banded WENO3 stage kernel
   // independent mode: per-realization CFL sub-steps of the advection step
@@ -794,7 +794,8 @@ void clear_graphs(hft_ctx *c) {
 }
 
 bool fused_usable(const hft_ctx *c) {
-  return c->fused && c->nu <= 1024 && c->fused_T >= 2 && c->fused_T <= 4 &&
+  return c->fused && c->nu <= 1024 && c->fused_T >= 2 &&
+         (c->fused_T <= 4 || (c->fused_T == 5 && c->fused_kernel == 3)) &&
          c->band_rows + 2 * c->fused_T <= kFusedMaxRows;
 }
 
@@ -898,7 +899,7 @@ int fused_band_count(int ntm, int T, int active, int cap) {
 // stage k+t-1 also to W[t] where W[t] is not null.
 int launch_fused(hft_ctx *c, int64_t k, int active, const std::vector<int> &ord,
                  const std::vector<int64_t> &s, const double *pA, const double *pB,
-                 const double *dty0, double *oA, double *oB, double *const W[4]) {
+                 const double *dty0, double *oA, double *oB, double *const W[5]) {
   int nr = c->nr, T = c->fused_T;
   FusedArgs fa;
   fa.m = c->m;
@@ -916,6 +917,7 @@ int launch_fused(hft_ctx *c, int64_t k, int active, const std::vector<int> &ord,
     switch (T) {
       case 2: rc = fused2_capacity<2>(c, threads, sh2, &cap); break;
       case 3: rc = fused2_capacity<3>(c, threads, sh2, &cap); break;
+      case 5: rc = fused2_capacity<5>(c, threads, sh2, &cap); break;
       default: rc = fused2_capacity<4>(c, threads, sh2, &cap); break;
     }
     if (rc) return rc;
@@ -928,7 +930,7 @@ int launch_fused(hft_ctx *c, int64_t k, int active, const std::vector<int> &ord,
   fa.D = c->d_D; fa.coef = c->d_coef; fa.nr_table = nr; fa.s = c->d_s; fa.order = c->d_order;
   fa.rstride = c->rstride;
   fa.cdt = c->d_cdt;
-  for (int t = 0; t < 4; t++) fa.W[t] = W[t];
+  for (int t = 0; t < 5; t++) fa.W[t] = W[t];
   size_t shmem = (size_t)T * 3 * fa.ncol * sizeof(double);
   double units = 0;
   for (int z = 0; z < active; z++) {
@@ -936,7 +938,7 @@ int launch_fused(hft_ctx *c, int64_t k, int active, const std::vector<int> &ord,
     units += 5.0 * st;  // algorithmic passes: as if stage by stage
   }
   int blocks = active * fa.nbands;
-  bool store = W[0] || W[1] || W[2] || W[3];
+  bool store = W[0] || W[1] || W[2] || W[3] || W[4];
   if (c->fused_kernel == 3) {
     // two phi columns per thread: a multiple of 64 threads, LDS rows of 2 *
     // threads (3-row rings of the T stage levels and of u0)
@@ -945,6 +947,7 @@ int launch_fused(hft_ctx *c, int64_t k, int active, const std::vector<int> &ord,
     switch (T) {
       case 2: return launch_ex(c, kGroupFused, k_rkg2_fused2<2>, blocks, threads, sh2, fa, units);
       case 3: return launch_ex(c, kGroupFused, k_rkg2_fused2<3>, blocks, threads, sh2, fa, units);
+      case 5: return launch_ex(c, kGroupFused, k_rkg2_fused2<5>, blocks, threads, sh2, fa, units);
       default: return launch_ex(c, kGroupFused, k_rkg2_fused2<4>, blocks, threads, sh2, fa, units);
     }
   }
@@ -999,7 +1002,7 @@ int superstep_launches(hft_ctx *c, int n, const std::vector<int> &ord,
     if (fused_ok && active >= c->fused_min) {
       double *oA = pick_free(nullptr, nullptr);
       double *oB = pick_free(oA, nullptr);
-      double *none[4] = {nullptr, nullptr, nullptr, nullptr};
+      double *none[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
       rc = launch_fused(c, k, active, ord, s, pA, pB, c->dty0, oA, oB, none);
       if (rc) return rc;
       for (int z = 0; z < active; z++)
@@ -1183,6 +1186,7 @@ int hft_create(int ntm, int npm1, const double *t, const double *p, int nr, int 
     hipFuncSetAttribute((const void *)k_rkg2_fused2<2>, hipFuncAttributeMaxDynamicSharedMemorySize, 3 * row);
     hipFuncSetAttribute((const void *)k_rkg2_fused2<3>, hipFuncAttributeMaxDynamicSharedMemorySize, 4 * row);
     hipFuncSetAttribute((const void *)k_rkg2_fused2<4>, hipFuncAttributeMaxDynamicSharedMemorySize, 5 * row);
+    hipFuncSetAttribute((const void *)k_rkg2_fused2<5>, hipFuncAttributeMaxDynamicSharedMemorySize, 6 * row);
     set_adjoint_attributes();
   }
   *out = c;
@@ -2138,7 +2142,7 @@ int hft_set_tuning(hft_ctx *c, const char *key, int value) {
   clear_graphs(c);  // captured launch sequences depend on the tuning
   if (!strcmp(key, "fused")) c->fused = value != 0;
   else if (!strcmp(key, "fused_stages")) {
-    if (value < 2 || value > 4) return fail(HFT_E_ARG, "fused_stages must be 2..4");
+    if (value < 2 || value > 5) return fail(HFT_E_ARG, "fused_stages must be 2..5 (5: fused_kernel 3 only)");
     c->fused_T = value;
   } else if (!strcmp(key, "band_rows")) {
     if (value < 4 || value > 4096) return fail(HFT_E_ARG, "band_rows must be 4..4096");

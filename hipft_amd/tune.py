@@ -45,8 +45,14 @@ def candidates(nr):
                         out.append(dict(fused=1, fused_stages=T, band_rows=H,
                                         fused_min_realizations=m, fused_target_blocks=B,
                                         fused_kernel=K, fused_bands=0))
+    # five stages per launch: the two-column kernel only
+    for H in (32, 64, 128):
+        for m in sorted({1, 4, 8, 16, 32} & set(range(1, max(nr, 1) + 1))):
+            for B in (0, 256):
+                out.append(dict(fused=1, fused_stages=5, band_rows=H, fused_min_realizations=m,
+                                fused_target_blocks=B, fused_kernel=3, fused_bands=0))
     # band count chosen per launch (two-column kernel only; band_rows unused)
-    for T in (3, 4):
+    for T in (3, 4, 5):
         for m in sorted({1, 4, 8, 16, 32} & set(range(1, max(nr, 1) + 1))):
             out.append(dict(fused=1, fused_stages=T, band_rows=32, fused_min_realizations=m,
                             fused_target_blocks=0, fused_kernel=3, fused_bands=1))

@@ -221,7 +221,8 @@ int hft_stream(hft_ctx *ctx, void **stream);
 /* Launch tuning of one context (chosen per grid and batch by
  * hipft_amd/tune.py; the defaults suit 512x1024 batches):
  *   "fused"                  1 = run RKG2 stages several per launch (default)
- *   "fused_stages"           stages per fused launch, 2..4 (default 4)
+ *   "fused_stages"           stages per fused launch, 2..4 (default 4); 5 with
+ *                            fused_kernel 3
  *   "band_rows"              theta rows per block of a fused launch (32)
  *   "fused_min_realizations" fused launches only while at least this many
  *                            realizations are still stepping (8)

@@ -112,13 +112,15 @@ def test_frozen_weights_adjoint_matches_frozen_autograd(gpu_available):
     assert rel(grads[:, 2], gsp) < 1e-10
 
 
-@pytest.mark.parametrize("fused_min", [1, 8])
-def test_segmented_rkg2_adjoint_matches_stored(gpu_available, fused_min):
+@pytest.mark.parametrize("fused_min,T", [(1, 4), (8, 4), (1, 5)])
+def test_segmented_rkg2_adjoint_matches_stored(gpu_available, fused_min, T):
     """Segment recomputation of RKG2 stages (short segments, one-stage
     launches) gives the same adjoint as storing every stage during the
-    forward recomputation (fused multi-stage launches when fused_min = 1)."""
+    forward recomputation (fused multi-stage launches of T stages, each
+    stage also written out, when fused_min = 1)."""
     o, g, ts = build(41, 81, 3, [120.0, 500.0, 1000.0], flow=False)
-    g.set_tuning(fused=1, fused_stages=4, band_rows=16, fused_min_realizations=fused_min)
+    g.set_tuning(fused=1, fused_kernel=3, fused_stages=T, band_rows=16,
+                 fused_min_realizations=fused_min)
     x0 = dev_state(g)
     tgt = torch.zeros_like(x0)
     out = []

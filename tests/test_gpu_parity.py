@@ -448,7 +448,23 @@ def test_fused_rkg2_bitexact(gpu_available, T, H, B, kernel):
     assert_same(g.get_state("tp"), o.get_f(), f"fused kernel {kernel} T={T} H={H} B={B}")
 
 
-@pytest.mark.parametrize("T", [3, 4])
+@pytest.mark.parametrize("H,B", [(5, 0), (32, 0), (64, 0), (32, 40)])
+def test_fused_rkg2_five_stages_bitexact(gpu_available, H, B):
+    """Five RKG2 stages per launch (two-column kernel; the fifth stage takes
+    u0 and dty0 of its row from the slot the step reuses) give the oracle's
+    result bit for bit."""
+    o, g = pair(91, 181, nr=3)
+    eta = [150.0, 450.0, 950.0]
+    o.load_diffusion(eta)
+    g.set_diffusion(eta)
+    g.set_tuning(fused=1, fused_kernel=3, fused_stages=5, band_rows=H, fused_min_realizations=1,
+                 fused_target_blocks=B)
+    o.diffusion_step(6.0, subcycles=0)
+    g.diffusion_step(6.0, subcycles=0)
+    assert_same(g.get_state("tp"), o.get_f(), f"fused kernel 3 T=5 H={H} B={B}")
+
+
+@pytest.mark.parametrize("T", [3, 4, 5])
 def test_fused_round_aware_bands_bitexact(gpu_available, T):
     """Band counts chosen per launch against whole rounds of co-resident
     blocks (fused_bands=1): a 181-row grid (ragged last band, independent
