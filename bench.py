@@ -320,7 +320,19 @@ def main():
         per_real = el1 / k1
         out["single"] = dict(value=round(k1 / el1, 3), unit="realization-steps/s",
                              ms_per_step=round(1e3 * per_real, 3),
-                             batched_speedup_per_realization=round(value * per_real, 2))
+                             batched_speedup_per_realization=round(value * per_real, 2),
+                             note="one realization, launch shapes tuned for it (fused RKG2)")
+        # the plain path: one realization, one RKG2 stage per launch, one
+        # thread per point in the WENO3 stages
+        m1.set_tuning(fused=0, advection_band=0, graphs=0)
+        step(m1)
+        el1p = time_steps(m1, k1)
+        per_plain = el1p / k1
+        out["single_plain"] = dict(value=round(k1 / el1p, 3), unit="realization-steps/s",
+                                   ms_per_step=round(1e3 * per_plain, 3),
+                                   batched_speedup_per_realization=round(value * per_plain, 2),
+                                   note="one realization, one RKG2 stage per launch, pointwise "
+                                        "WENO3 stages")
         m1.close()
     if not args.no_adjoint:
         m.close()
