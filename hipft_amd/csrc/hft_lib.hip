@@ -120,6 +120,7 @@ struct hft_ctx {
   // 1: each fused launch picks its band count against whole rounds of
   // co-resident blocks (fused_band_count); 0: bands of band_rows rows
   int fused_bands = 0;
+  int fused_band_share = 32;  // percent: slowdown of a round from an idle to a full device
   int fused_cap[6] = {0, 0, 0, 0, 0, 0};  // co-resident k_rkg2_fused2<T> blocks on the device
   int fused_kernel = 3;  // 1: k_rkg2_fused, 2: k_rkg2_march, 3: k_rkg2_fused2 (2 columns/thread)
   int adv_band = 1, adv_band_rows = 16;  // banded WENO3 stage kernel
@@ -866,14 +867,14 @@ int fused2_capacity(hft_ctx *c, int threads, size_t shmem, int *cap) {
 // holds `cap` blocks at once.  A block marches its band's rows plus about 3T
 // rows of halo and pipeline fill, one block per slot, so a launch takes
 // rounds of cap blocks; a round's time grows with the rows per band and,
-// by about a third from an idle to a full device, with the blocks sharing
-// HBM (measured on 512 x 1024 bands of 32-128 rows).  Of the band counts
+// by kShare (about a third; tuning key "fused_band_share" in percent) from
+// an idle to a full device, with the blocks sharing the memory system
+// (measured on 512 x 1024 bands of 32-128 rows).  Of the band counts
 // whose bands have 16..(kFusedMaxRows - 2T) rows, the one with the least
 // modelled time; a launch whose last round is nearly empty (active * nbands
 // just above a multiple of cap) moves to fewer, taller bands or to more,
 // shorter ones that fill the rounds.
-int fused_band_count(int ntm, int T, int active, int cap) {
-  const double kShare = 0.32;
+int fused_band_count(int ntm, int T, int active, int cap, double kShare) {
   int hmax = kFusedMaxRows - 2 * T;
   int best = (ntm + hmax - 1) / hmax;
   double best_t = 1e300;
@@ -921,7 +922,7 @@ int launch_fused(hft_ctx *c, int64_t k, int active, const std::vector<int> &ord,
       default: rc = fused2_capacity<4>(c, threads, sh2, &cap); break;
     }
     if (rc) return rc;
-    int nb = fused_band_count(c->ntm, T, active, cap);
+    int nb = fused_band_count(c->ntm, T, active, cap, 0.01 * c->fused_band_share);
     fa.H = (c->ntm + nb - 1) / nb;
   }
   fa.nbands = (c->ntm + fa.H - 1) / fa.H;
@@ -2159,6 +2160,9 @@ int hft_set_tuning(hft_ctx *c, const char *key, int value) {
   } else if (!strcmp(key, "fused_bands")) {
     if (value < 0 || value > 1) return fail(HFT_E_ARG, "fused_bands must be 0 or 1");
     c->fused_bands = value;
+  } else if (!strcmp(key, "fused_band_share")) {
+    if (value < 0 || value > 200) return fail(HFT_E_ARG, "fused_band_share must be 0..200");
+    c->fused_band_share = value;
   } else if (!strcmp(key, "graphs")) {
     c->graphs_on = value != 0;
   } else if (!strcmp(key, "adjoint_segment")) {
@@ -2197,6 +2201,7 @@ int hft_get_tuning(hft_ctx *c, const char *key, int *value) {
   else if (!strcmp(key, "fused_min_realizations")) *value = c->fused_min;
   else if (!strcmp(key, "fused_target_blocks")) *value = c->fused_target_blocks;
   else if (!strcmp(key, "fused_bands")) *value = c->fused_bands;
+  else if (!strcmp(key, "fused_band_share")) *value = c->fused_band_share;
   else if (!strcmp(key, "fused_kernel")) *value = c->fused_kernel;
   else if (!strcmp(key, "graphs")) *value = c->graphs_on;
   else if (!strcmp(key, "graph_hits")) *value = (int)c->graph_hits;

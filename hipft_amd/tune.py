@@ -21,12 +21,12 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CACHE = os.path.join(HERE, "tuning_cache.json")
 
 DIFF_KEYS = ("fused", "fused_stages", "band_rows", "fused_min_realizations",
-             "fused_target_blocks", "fused_kernel", "fused_bands")
+             "fused_target_blocks", "fused_kernel", "fused_bands", "fused_band_share")
 ADV_KEYS = ("advection_band", "advection_band_rows")
 KEYS = DIFF_KEYS + ADV_KEYS
 DEFAULTS = dict(fused=1, fused_stages=4, band_rows=32, fused_min_realizations=8,
-                fused_target_blocks=0, fused_kernel=3, fused_bands=0, advection_band=1,
-                advection_band_rows=16)
+                fused_target_blocks=0, fused_kernel=3, fused_bands=0, fused_band_share=32,
+                advection_band=1, advection_band_rows=16)
 
 
 def adv_candidates():
@@ -51,12 +51,15 @@ def candidates(nr):
             for B in (0, 256):
                 out.append(dict(fused=1, fused_stages=5, band_rows=H, fused_min_realizations=m,
                                 fused_target_blocks=B, fused_kernel=3, fused_bands=0))
-    # band count chosen per launch (two-column kernel only; band_rows unused)
+    # band count chosen per launch (two-column kernel only; band_rows unused),
+    # with the band-count model's memory-sharing slowdown in percent
     for T in (3, 4, 5):
         for m in sorted({1, 4, 8, 16, 32} & set(range(1, max(nr, 1) + 1))):
-            out.append(dict(fused=1, fused_stages=T, band_rows=32, fused_min_realizations=m,
-                            fused_target_blocks=0, fused_kernel=3, fused_bands=1))
-    return out
+            for share in (16, 32, 64):
+                out.append(dict(fused=1, fused_stages=T, band_rows=32, fused_min_realizations=m,
+                                fused_target_blocks=0, fused_kernel=3, fused_bands=1,
+                                fused_band_share=share))
+    return [dict({"fused_band_share": DEFAULTS["fused_band_share"]}, **c) for c in out]
 
 
 def cache_key(model):

@@ -237,6 +237,9 @@ int hft_stream(hft_ctx *ctx, void **stream);
  *                            that its blocks fill whole rounds of the blocks
  *                            the device holds at once; band_rows and
  *                            fused_target_blocks are then unused (0)
+ *   "fused_band_share"       that choice's model of how much longer a round
+ *                            takes on a full device than on an idle one, in
+ *                            percent (32)
  *   "advection_band"         1 = WENO3 stages by theta bands computing each
  *                            face once (default); 0 = one thread per point
  *   "advection_band_rows"    theta rows per block of the banded WENO3 (16)

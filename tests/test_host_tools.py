@@ -55,6 +55,8 @@ def test_candidates_cover_knobs():
     assert {c["fused_bands"] for c in c64 if c["fused"]} == {0, 1}
     assert all(c["fused_kernel"] == 3 for c in c64 if c["fused_bands"])
     assert all(set(tune.DIFF_KEYS) <= set(c) for c in c64)
+    assert all(c["fused_kernel"] == 3 for c in c64 if c["fused"] and c["fused_stages"] == 5)
+    assert {c["fused_band_share"] for c in c64 if c["fused"] and c["fused_bands"]} == {16, 32, 64}
 
 
 def test_apply_sets_every_key_with_defaults():
