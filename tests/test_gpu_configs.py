@@ -130,6 +130,46 @@ def test_config3_batch_step_vs_single_runs(gpu_available):
     g.close()
 
 
+def test_config3_tuned_batch_conserves_signed_flux(gpu_available):
+    """The benchmark batch exactly as bench.py runs it (64 realizations at
+    512x1024, launch shapes from the tuning cache, PTL sub-cycling, Strang
+    steps): the signed flux of every realization, weighted as the flux-form
+    operators conserve it (interior cells sin(t) dt dp, each pole value
+    pi sin(th) dth / 2), stays at its initial value to round-off over three
+    steps."""
+    sys.path.insert(0, ROOT)
+    import bench
+    from hipft_amd import tune
+    wl = bench.make_workload(64)
+    g = bench.setup_model(wl)
+    cfg = tune.autotune(g)  # cached on the GPU this suite runs on
+    print(f"tuned launch shapes: {cfg}", flush=True)
+    o = Oracle(wl["t"], wl["p"], 1)
+    m = o.mesh
+    Wt = np.zeros((o.npm, o.ntm))
+    Wt[1:o.npm - 1, 1:o.ntm - 1] = (m["st"][1:-1] * m["dt"][1:-1])[None, :] * \
+        m["dp"][1:o.npm - 1][:, None]
+    npole = math.pi * m["sth"][1] * m["dth"][1] / 2.0
+    spole = math.pi * m["sth"][o.nt - 2] * m["dth"][o.nt - 2] / 2.0
+
+    def signed(f):
+        s = (Wt[None] * f).sum(axis=(1, 2)) + npole * f[:, 1, 0] + spole * f[:, 1, -1]
+        mag = np.abs(Wt[None] * f).sum(axis=(1, 2)) + np.abs(npole * f[:, 1, 0]) + \
+            np.abs(spole * f[:, 1, -1])
+        return s, mag
+
+    s0, mag0 = signed(g.get_state("tp"))
+    for _ in range(3):
+        g.flux_transport_step(1.0, strang=True, subcycles=0)
+    s3, mag3 = signed(g.get_state("tp"))
+    drift = np.abs(s3 - s0) / mag0
+    print(f"signed-flux drift over 3 steps: max {drift.max():.2e} (relative to the unsigned flux)",
+          flush=True)
+    assert drift.max() <= 1e-12
+    assert np.all(mag3 < mag0)  # diffusion cancels opposite polarities
+    g.close()
+
+
 def _window_model(ntm, npm1, nr, seed, clustered=False, nbip=200):
     from hipft_amd.adjoint import FlowModel
     t = W.clustered_theta(ntm, 2.0) if clustered else W.uniform_theta(ntm)
