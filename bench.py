@@ -16,10 +16,13 @@ A "step" is one time step of the whole batch: value = realizations x steps /
 wall time of the timed steps.
 
 Prints one JSON line.  Extra objects:
-  roofline     -- dominant kernel (the multi-stage RKG2 launch): algorithmic
-                  bytes / device time from HIP events on the library's stream
-                  over the timed steps; traffic = HBM bytes per launch from the
-                  committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary
+  roofline     -- dominant kernel (the multi-stage RKG2 launch): compulsory
+                  bytes per launch / average launch time from HIP events on
+                  the library's stream over the timed steps; traffic = HBM
+                  bytes per launch from the committed rocprofv3
+                  FETCH_SIZE/WRITE_SIZE summary; the stage-by-stage
+                  equivalent and the whole step's measured DRAM fraction
+                  beside it
   cpu_baseline -- the CPU oracle (1 core) on a bounded sample of the workload
   single       -- the same step for one realization per launch (plain path)
   adjoint      -- cost of the adjoint gradient relative to the forward run
@@ -91,9 +94,11 @@ def _cpu_model():
     return "unknown"
 
 
-def cpu_baseline(wl, budget_s=12.0, max_steps=20):
+def cpu_baseline(wl, budget_s=6.0, warmup=2, repeats=3):
     """The oracle (CPU restatement, OpenMP over all host threads, and one
-    thread) on realization 0 of the workload."""
+    thread) on realization 0 of the workload: warm-up steps, then the median
+    rate of `repeats` timed runs of n steps each, n sized from the warm-up so
+    that a run takes about budget_s / repeats."""
     from oracle.oracle import Oracle, get_threads, set_threads
     nthreads = get_threads()
 
@@ -108,26 +113,37 @@ def cpu_baseline(wl, budget_s=12.0, max_steps=20):
         o.flow_add_dr(*[[c] for c in wl["dr"][0]])
         o.set_lf_alpha()
         o.load_diffusion([wl["eta"][0]])
-        n = 0
-        t0 = time.perf_counter()
-        while n < max_steps:
+
+        def one():
             o.advection_step(0.5)
             o.diffusion_step(1.0, 0)
             o.advection_step(0.5)
             o.analysis()
-            n += 1
-            if time.perf_counter() - t0 > budget_s:
-                break
-        return n, time.perf_counter() - t0
 
-    n_all, el_all = run(nthreads)
-    n_one, el_one = run(1)
+        t0 = time.perf_counter()
+        for _ in range(warmup):
+            one()
+        per = (time.perf_counter() - t0) / warmup
+        n = max(1, int(budget_s / repeats / per))
+        rates = []
+        for _ in range(repeats):
+            t0 = time.perf_counter()
+            for _ in range(n):
+                one()
+            rates.append(n / (time.perf_counter() - t0))
+        return float(np.median(rates)), n, rates
+
+    v_all, n_all, r_all = run(nthreads)
+    v_one, n_one, r_one = run(1)
     set_threads(nthreads)
-    return dict(value=n_all / el_all, unit="realization-steps/s", cores=nthreads, kind="port",
-                sample=f"{n_all} steps of realization 0 (eta={wl['eta'][0]:.1f} km^2/s) of the "
-                       f"512x1024 workload, CPU oracle (C restatement, OpenMP over points) on "
-                       f"{nthreads} threads of {os.cpu_count()} CPUs ({_cpu_model()})",
-                single_core=dict(value=n_one / el_one, cores=1, steps=n_one))
+    return dict(value=round(v_all, 4), unit="realization-steps/s", cores=nthreads, kind="port",
+                sample=f"realization 0 (eta={wl['eta'][0]:.1f} km^2/s) of the 512x1024 workload: "
+                       f"{warmup} warm-up steps, then the median of {repeats} runs of {n_all} "
+                       f"steps; CPU oracle (C restatement, OpenMP over points) on {nthreads} "
+                       f"threads of {os.cpu_count()} CPUs ({_cpu_model()})",
+                repeats=[round(x, 4) for x in r_all],
+                single_core=dict(value=round(v_one, 4), cores=1, steps_per_run=n_one,
+                                 repeats=[round(x, 4) for x in r_one]))
 
 
 def adjoint_bench(nr=16, ntm=256, npm1=512, nsteps=200, every=20, seed=77):
@@ -204,17 +220,20 @@ def pmc_probe(m, nbytes=256 << 20):
 
 
 def load_traffic(kernel_sub):
-    """HBM bytes per launch of a kernel from the committed rocprofv3 --pmc
-    summary (FETCH_SIZE + WRITE_SIZE, scaled by a copy of known size)."""
+    """HBM bytes per launch of a kernel, and of a whole workload step (every
+    kernel of the step), from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE + WRITE_SIZE, scaled by a copy of known size)."""
     path = os.path.join(ROOT, "profiles", "step_pmc_batched64.json")
     if not os.path.exists(path):
-        return None
+        return None, None
     with open(path) as fh:
         d = json.load(fh)
+    per, step_total = None, 0.0
     for name, v in d.get("kernels", {}).items():
+        step_total += v["hbm_bytes_per_launch"] * v["dispatches"]
         if kernel_sub in name:
-            return v["hbm_bytes_per_launch"]
-    return None
+            per = v["hbm_bytes_per_launch"]
+    return per, (step_total or None)
 
 
 def main():
@@ -267,8 +286,11 @@ def main():
     groups = {}
     for g in ("rkg2", "rkg2_fused", "weno", "ptl", "analysis"):
         ms, n, b = m.kernel_time(g)
+        sb = m.kernel_stage_bytes(g)
         groups[g] = dict(ms=ms, launches=n, alg_GB=b / 1e9,
                          GBs=(b / 1e9) / (ms / 1e3) if ms > 0 else None)
+        if sb != b:
+            groups[g]["stage_equivalent_GB"] = sb / 1e9
     m.set_timing(False)
     s_last, cyc = m.diffusion_stats()
     value = nr * args.steps / el
@@ -276,26 +298,40 @@ def main():
     fused = groups["rkg2_fused"]["launches"] > 0
     dom = "rkg2_fused" if fused else "rkg2"
     ms_d, n_d, b_d = groups[dom]["ms"], groups[dom]["launches"], groups[dom]["alg_GB"] * 1e9
+    sb_d = groups[dom].get("stage_equivalent_GB", b_d / 1e9) * 1e9
     fk = {1: "k_rkg2_fused", 2: "k_rkg2_march", 3: "k_rkg2_fused2"}[
         tuning.get("fused_kernel", tune.DEFAULTS["fused_kernel"])]
     kname = f"{fk}<{tuning['fused_stages']}>" if fused else "k_rkg2_stage("
     avg_ms = ms_d / max(n_d, 1)
     achieved = (b_d / 1e9) / (ms_d / 1e3) if ms_d > 0 else 0.0
-    traffic = load_traffic(kname)
+    traffic, traffic_step = load_traffic(kname)
     roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
                 kernel=kname, avg_launch_ms=round(avg_ms, 5),
                 alg_bytes_per_launch=round(b_d / max(n_d, 1)), launches=n_d,
-                note="algorithmic bytes count 5 FP64 field passes per RKG2 stage and "
-                     "realization as if stage by stage; a fused launch keeps "
-                     "intermediate stages in LDS, so achieved can exceed the HBM peak")
+                note="algorithmic bytes = the compulsory FP64 field passes of a launch: read "
+                     "u_{k-1}, u_{k-2} and u0, write two stage levels, per realization it "
+                     "advances (8 x 512 x 1023 bytes a pass); traffic = HBM bytes per launch "
+                     "measured by rocprofv3 FETCH_SIZE + WRITE_SIZE (profiles/"
+                     "step_pmc_batched64.json)")
+    if fused and b_d > 0:
+        # the same stages one per launch would move 5 passes each
+        roof["stage_equivalent_GBs"] = round((sb_d / 1e9) / (ms_d / 1e3), 1)
+        roof["temporal_blocking_gain"] = round(sb_d / b_d, 3)
     if traffic:
         roof["hbm_achieved"] = round(traffic / 1e9 / avg_ms * 1e3, 1)
         roof["hbm_frac"] = round(roof["hbm_achieved"] / HBM_PEAK_GBS, 4)
+        roof["traffic_over_alg"] = round(traffic / (b_d / max(n_d, 1)), 3)
+    step_ms = 1e3 * el / args.steps
+    if traffic_step:
+        # whole step: measured DRAM bytes of one step (all kernels, the
+        # committed counter profile) over the measured step time
+        roof["step_hbm_GB"] = round(traffic_step / 1e9, 3)
+        roof["step_hbm_frac"] = round(traffic_step / 1e9 / (step_ms / 1e3) / HBM_PEAK_GBS, 4)
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "realization-steps/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": round(1e3 * el / args.steps, 3),
+        "ms_per_step": round(step_ms, 3),
         "ms_per_step_with_kernel_events": round(1e3 * el_ev / args.steps, 3),
         "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",

@@ -89,6 +89,7 @@ _SIGS = [
                                       ctypes.POINTER(ctypes.c_int)]),
     ("hft_set_timing", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("hft_kernel_time", ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, _D, _I64, _D]),
+    ("hft_kernel_stage_bytes", ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, _D]),
     ("hft_set_state_device", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]),
     ("hft_get_state_device", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]),
     ("hft_set_stream", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]),
@@ -477,3 +478,10 @@ class FluxTransport:
         _check(lib().hft_kernel_time(self._h, group.encode(), ctypes.byref(ms), ctypes.byref(n),
                                      ctypes.byref(b)), "hft_kernel_time")
         return ms.value, n.value, b.value
+
+    def kernel_stage_bytes(self, group):
+        """Bytes the group's work would move one RKG2 stage per launch."""
+        b = ctypes.c_double()
+        _check(lib().hft_kernel_stage_bytes(self._h, group.encode(), ctypes.byref(b)),
+               "hft_kernel_stage_bytes")
+        return b.value

@@ -561,18 +561,23 @@ int adjoint_check(hft_ctx *c, const double *lam_dev, int64_t ld, int subcycles, 
 
 }  // namespace
 
-void set_adjoint_attributes() {
-  hipFuncSetAttribute((const void *)k_rkg2_adj_fused<4, 1024>,
-                      hipFuncAttributeMaxDynamicSharedMemorySize, 4 * 3 * 1024 * (int)sizeof(double));
+int set_adjoint_attributes() {
   int lds = (19 * 1024 + 16) * (int)sizeof(double);
-  hipFuncSetAttribute((const void *)k_advection_vjp_band<true, true, 1024>,
-                      hipFuncAttributeMaxDynamicSharedMemorySize, lds);
-  hipFuncSetAttribute((const void *)k_advection_vjp_band<true, false, 1024>,
-                      hipFuncAttributeMaxDynamicSharedMemorySize, lds);
-  hipFuncSetAttribute((const void *)k_advection_vjp_band<false, true, 1024>,
-                      hipFuncAttributeMaxDynamicSharedMemorySize, lds);
-  hipFuncSetAttribute((const void *)k_advection_vjp_band<false, false, 1024>,
-                      hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+  struct {
+    const void *fn;
+    int bytes;
+  } req[] = {{(const void *)k_rkg2_adj_fused<4, 1024>, 4 * 3 * 1024 * (int)sizeof(double)},
+             {(const void *)k_advection_vjp_band<true, true, 1024>, lds},
+             {(const void *)k_advection_vjp_band<true, false, 1024>, lds},
+             {(const void *)k_advection_vjp_band<false, true, 1024>, lds},
+             {(const void *)k_advection_vjp_band<false, false, 1024>, lds}};
+  for (auto &q : req) {
+    hipError_t e = hipFuncSetAttribute(q.fn, hipFuncAttributeMaxDynamicSharedMemorySize, q.bytes);
+    if (e != hipSuccess)
+      return fail(HFT_E_HIP, "hipFuncSetAttribute(%d bytes of LDS, adjoint): %s", q.bytes,
+                  hipGetErrorString(e));
+  }
+  return 0;
 }
 
 extern "C" {

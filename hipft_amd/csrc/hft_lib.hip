@@ -80,7 +80,8 @@ int dalloc(T **p, size_t n) {
 struct TimedPair {
   hipEvent_t a, b;
   int group;
-  double units;  // algorithmic field passes x realizations of the launch
+  double units;   // compulsory field passes x realizations of the launch
+  double sunits;  // the same work done stage by stage (differs for fused launches)
 };
 
 // Kernel timing groups; launches timed under "rkg2_fused" also count in "rkg2".
@@ -208,6 +209,7 @@ struct hft_ctx {
   double t_ms[kNGroups] = {0};
   int64_t t_n[kNGroups] = {0};
   double t_units[kNGroups] = {0};
+  double t_sunits[kNGroups] = {0};
 };
 
 namespace {
@@ -231,10 +233,12 @@ int drain_timing(hft_ctx *c) {
     c->t_ms[tp.group] += ms;
     c->t_n[tp.group] += 1;
     c->t_units[tp.group] += tp.units;
+    c->t_sunits[tp.group] += tp.sunits;
     if (tp.group == kGroupFused) {
       c->t_ms[0] += ms;
       c->t_n[0] += 1;
       c->t_units[0] += tp.units;
+      c->t_sunits[0] += tp.sunits;
     }
     c->free_ev.push_back(tp.a);
     c->free_ev.push_back(tp.b);
@@ -246,9 +250,11 @@ int drain_timing(hft_ctx *c) {
 // Launch helper: records events around the launch when timing is enabled.
 // units: compulsory field passes (reads + writes of a realization's field)
 // the launch makes, times its realizations; used for algorithmic bytes.
+// sunits: the passes the same work takes one stage per launch (a fused
+// multi-stage launch keeps its intermediate stages on chip); < 0 = units.
 template <typename K, typename A>
 int launch_ex(hft_ctx *c, int group, K kern, int blocks, int threads, size_t shmem, const A &args,
-              double units) {
+              double units, double sunits = -1.0) {
   if (blocks <= 0) return 0;
   hipEvent_t a = nullptr, b = nullptr;
   if (c->timing) {
@@ -265,7 +271,7 @@ int launch_ex(hft_ctx *c, int group, K kern, int blocks, int threads, size_t shm
   HIPCHK(hipGetLastError());
   if (c->timing) {
     HIPCHK(hipEventRecord(b, c->stream));
-    c->pending.push_back({a, b, group, units});
+    c->pending.push_back({a, b, group, units, sunits < 0 ? units : sunits});
   }
   return 0;
 }
@@ -296,7 +302,7 @@ int phase_end(hft_ctx *c, hipEvent_t a, int group, int body_rc) {
   int rc = ev_get(c, &b);
   if (rc) return rc;
   HIPCHK(hipEventRecord(b, c->stream));
-  c->pending.push_back({a, b, group, 0.0});
+  c->pending.push_back({a, b, group, 0.0, 0.0});
   return 0;
 }
 
@@ -718,6 +724,12 @@ void sts_coefs(int64_t s, std::vector<double> &uj, std::vector<double> &vj,
   }
 }
 
+// Captured RKG2 super-steps (hipGraph execs) of the context.
+void clear_graphs(hft_ctx *c) {
+  for (auto &g : c->graphs) hipGraphExecDestroy(g.second);
+  c->graphs.clear();
+}
+
 struct CoefCache {
   int64_t s = -1;
   std::vector<double> u, v, ub, g;
@@ -736,7 +748,10 @@ int upload_rkg2(hft_ctx *c, const std::vector<int> &reals, const std::vector<dou
   int64_t smax = s[ord[0]];
   size_t need = (size_t)(smax + 1) * nr;
   if (need > c->coef_cap) {
-    if (c->d_coef) hipFree(c->d_coef);
+    // captured super-steps hold the old table's address: drop them
+    clear_graphs(c);
+    if (c->d_coef) HIPCHK(hipFree(c->d_coef));
+    c->d_coef = nullptr;
     size_t cap = std::max(need, (size_t)1024);
     int rc = dalloc(&c->d_coef, cap);
     if (rc) return rc;
@@ -785,15 +800,6 @@ int upload_rkg2(hft_ctx *c, const std::vector<int> &reals, const std::vector<dou
   return 0;
 }
 
-// One RKG2 super-step for the listed realizations, each with its own cycle
-// time step dt2[r] and stage count s[r] (diffusion_step_sts_cd,
-// src/hipft.f90:5163-5242).  use_af: dty0 already holds L(f) from the PTL
-// pass (same operator, so identical to recomputing it).
-void clear_graphs(hft_ctx *c) {
-  for (auto &g : c->graphs) hipGraphExecDestroy(g.second);
-  c->graphs.clear();
-}
-
 bool fused_usable(const hft_ctx *c) {
   return c->fused && c->nu <= 1024 && c->fused_T >= 2 &&
          (c->fused_T <= 4 || (c->fused_T == 5 && c->fused_kernel == 3)) &&
@@ -803,6 +809,10 @@ bool fused_usable(const hft_ctx *c) {
 int superstep_launches(hft_ctx *c, int n, const std::vector<int> &ord,
                        const std::vector<int64_t> &s, bool use_af);
 
+// One RKG2 super-step for the listed realizations, each with its own cycle
+// time step dt2[r] and stage count s[r] (diffusion_step_sts_cd,
+// src/hipft.f90:5163-5242).  use_af: dty0 already holds L(f) from the PTL
+// pass (same operator, so identical to recomputing it).
 int rkg2_superstep(hft_ctx *c, const std::vector<int> &reals, const std::vector<double> &dt2,
                    const std::vector<int64_t> &s, bool use_af) {
   int n = (int)reals.size();
@@ -933,10 +943,16 @@ int launch_fused(hft_ctx *c, int64_t k, int active, const std::vector<int> &ord,
   fa.cdt = c->d_cdt;
   for (int t = 0; t < 5; t++) fa.W[t] = W[t];
   size_t shmem = (size_t)T * 3 * fa.ncol * sizeof(double);
-  double units = 0;
+  // compulsory passes: read u_{k-1}, u_{k-2}, u0 (and dty0, except the
+  // two-column kernel, which forms it from u0) and write two stage levels
+  // per realization, plus each stored stage; stage by stage the same stages
+  // would move 5 passes each
+  int nstore = 0;
+  for (int t = 0; t < 5; t++) nstore += W[t] ? 1 : 0;
+  double units = ((c->fused_kernel == 3 ? 5.0 : 6.0) + nstore) * active, sunits = 0;
   for (int z = 0; z < active; z++) {
     int64_t st = std::min<int64_t>(T, s[ord[z]] - k + 1);
-    units += 5.0 * st;  // algorithmic passes: as if stage by stage
+    sunits += 5.0 * st;
   }
   int blocks = active * fa.nbands;
   bool store = W[0] || W[1] || W[2] || W[3] || W[4];
@@ -946,23 +962,23 @@ int launch_fused(hft_ctx *c, int64_t k, int active, const std::vector<int> &ord,
     int threads = ((fa.ncol / 2 + 63) / 64) * 64;
     size_t sh2 = (size_t)(T + 1) * 3 * 2 * threads * sizeof(double);
     switch (T) {
-      case 2: return launch_ex(c, kGroupFused, k_rkg2_fused2<2>, blocks, threads, sh2, fa, units);
-      case 3: return launch_ex(c, kGroupFused, k_rkg2_fused2<3>, blocks, threads, sh2, fa, units);
-      case 5: return launch_ex(c, kGroupFused, k_rkg2_fused2<5>, blocks, threads, sh2, fa, units);
-      default: return launch_ex(c, kGroupFused, k_rkg2_fused2<4>, blocks, threads, sh2, fa, units);
+      case 2: return launch_ex(c, kGroupFused, k_rkg2_fused2<2>, blocks, threads, sh2, fa, units, sunits);
+      case 3: return launch_ex(c, kGroupFused, k_rkg2_fused2<3>, blocks, threads, sh2, fa, units, sunits);
+      case 5: return launch_ex(c, kGroupFused, k_rkg2_fused2<5>, blocks, threads, sh2, fa, units, sunits);
+      default: return launch_ex(c, kGroupFused, k_rkg2_fused2<4>, blocks, threads, sh2, fa, units, sunits);
     }
   }
   if (c->fused_kernel == 2 && !store) {
     switch (T) {
-      case 2: return launch_ex(c, kGroupFused, k_rkg2_march<2>, blocks, fa.ncol, shmem, fa, units);
-      case 3: return launch_ex(c, kGroupFused, k_rkg2_march<3>, blocks, fa.ncol, shmem, fa, units);
-      default: return launch_ex(c, kGroupFused, k_rkg2_march<4>, blocks, fa.ncol, shmem, fa, units);
+      case 2: return launch_ex(c, kGroupFused, k_rkg2_march<2>, blocks, fa.ncol, shmem, fa, units, sunits);
+      case 3: return launch_ex(c, kGroupFused, k_rkg2_march<3>, blocks, fa.ncol, shmem, fa, units, sunits);
+      default: return launch_ex(c, kGroupFused, k_rkg2_march<4>, blocks, fa.ncol, shmem, fa, units, sunits);
     }
   }
   switch (T) {
-    case 2: return launch_ex(c, kGroupFused, k_rkg2_fused<2>, blocks, fa.ncol, shmem, fa, units);
-    case 3: return launch_ex(c, kGroupFused, k_rkg2_fused<3>, blocks, fa.ncol, shmem, fa, units);
-    default: return launch_ex(c, kGroupFused, k_rkg2_fused<4>, blocks, fa.ncol, shmem, fa, units);
+    case 2: return launch_ex(c, kGroupFused, k_rkg2_fused<2>, blocks, fa.ncol, shmem, fa, units, sunits);
+    case 3: return launch_ex(c, kGroupFused, k_rkg2_fused<3>, blocks, fa.ncol, shmem, fa, units, sunits);
+    default: return launch_ex(c, kGroupFused, k_rkg2_fused<4>, blocks, fa.ncol, shmem, fa, units, sunits);
   }
 }
 
@@ -1065,6 +1081,9 @@ int ptl_pass(hft_ctx *c, const std::vector<int> &reals, bool coupled,
              const std::vector<double> &dt_in, std::vector<double> &res) {
   int n = (int)reals.size();
   int nr = c->nr;
+  // The previous super-step's upload_rkg2 copies out of the same pinned
+  // buffers asynchronously: wait for those copies before overwriting them.
+  HIPCHK(hipEventSynchronize(c->coef_ev));
   for (int z = 0; z < n; z++) c->h_small[z] = reals[z];
   unsigned long long *hmax = (unsigned long long *)(c->h_smalld);
   unsigned long long *hmin = hmax + nr;
@@ -1075,10 +1094,10 @@ int ptl_pass(hft_ctx *c, const std::vector<int> &reals, bool coupled,
     hmax[r] = 0ull;
     hmin[r] = hugebits;
   }
-  HIPCHK(hipEventSynchronize(c->coef_ev));
   HIPCHK(hipMemcpyAsync(c->d_order, c->h_small, n * sizeof(int), hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(c->d_absmax, hmax, nr * 8, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(c->d_dtmin, hmin, nr * 8, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipEventRecord(c->coef_ev, c->stream));
   LapArgs la;
   la.m = c->m; la.g = tiles_for(c, n); la.x = c->f; la.y = c->dty0; la.D = c->d_D;
   la.order = c->d_order; la.rstride = c->rstride; la.absmax = c->d_absmax;
@@ -1129,7 +1148,7 @@ int ptl_pass(hft_ctx *c, const std::vector<int> &reals, bool coupled,
 // ============================================================================
 // C ABI
 // ============================================================================
-void set_adjoint_attributes();
+int set_adjoint_attributes();
 
 extern "C" {
 
@@ -1178,17 +1197,26 @@ int hft_create(int ntm, int npm1, const double *t, const double *p, int nr, int 
   {
     // the fused RKG2 kernels use T stages x 3 rows x up to 1024 doubles of LDS
     int row = 3 * 1024 * (int)sizeof(double);
-    hipFuncSetAttribute((const void *)k_rkg2_fused<2>, hipFuncAttributeMaxDynamicSharedMemorySize, 2 * row);
-    hipFuncSetAttribute((const void *)k_rkg2_fused<3>, hipFuncAttributeMaxDynamicSharedMemorySize, 3 * row);
-    hipFuncSetAttribute((const void *)k_rkg2_fused<4>, hipFuncAttributeMaxDynamicSharedMemorySize, 4 * row);
-    hipFuncSetAttribute((const void *)k_rkg2_march<2>, hipFuncAttributeMaxDynamicSharedMemorySize, 2 * row);
-    hipFuncSetAttribute((const void *)k_rkg2_march<3>, hipFuncAttributeMaxDynamicSharedMemorySize, 3 * row);
-    hipFuncSetAttribute((const void *)k_rkg2_march<4>, hipFuncAttributeMaxDynamicSharedMemorySize, 4 * row);
-    hipFuncSetAttribute((const void *)k_rkg2_fused2<2>, hipFuncAttributeMaxDynamicSharedMemorySize, 3 * row);
-    hipFuncSetAttribute((const void *)k_rkg2_fused2<3>, hipFuncAttributeMaxDynamicSharedMemorySize, 4 * row);
-    hipFuncSetAttribute((const void *)k_rkg2_fused2<4>, hipFuncAttributeMaxDynamicSharedMemorySize, 5 * row);
-    hipFuncSetAttribute((const void *)k_rkg2_fused2<5>, hipFuncAttributeMaxDynamicSharedMemorySize, 6 * row);
-    set_adjoint_attributes();
+    struct {
+      const void *fn;
+      int bytes;
+    } lds[] = {{(const void *)k_rkg2_fused<2>, 2 * row},  {(const void *)k_rkg2_fused<3>, 3 * row},
+               {(const void *)k_rkg2_fused<4>, 4 * row},  {(const void *)k_rkg2_march<2>, 2 * row},
+               {(const void *)k_rkg2_march<3>, 3 * row},  {(const void *)k_rkg2_march<4>, 4 * row},
+               {(const void *)k_rkg2_fused2<2>, 3 * row}, {(const void *)k_rkg2_fused2<3>, 4 * row},
+               {(const void *)k_rkg2_fused2<4>, 5 * row}, {(const void *)k_rkg2_fused2<5>, 6 * row}};
+    for (auto &q : lds) {
+      hipError_t e2 = hipFuncSetAttribute(q.fn, hipFuncAttributeMaxDynamicSharedMemorySize, q.bytes);
+      if (e2 != hipSuccess) rc = fail(HFT_E_HIP, "hipFuncSetAttribute(%d bytes of LDS): %s", q.bytes,
+                                      hipGetErrorString(e2));
+    }
+    if (!rc) rc = set_adjoint_attributes();
+    if (rc) {
+      std::string msg = g_err;
+      hft_destroy(c);
+      g_err = msg;
+      return rc;
+    }
   }
   *out = c;
   return 0;
@@ -1854,6 +1882,7 @@ static int assimilate_impl(hft_ctx *c, const double *map_dev, int per_real, int 
   }
   HIPCHK(hipMemcpyAsync(c->d_apar, ap.data(), ap.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(c->d_ipar, ip.data(), ip.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));  // ap and ip are pageable locals
   AssimArgs a;
   a.m = c->m; a.g = tiles_for(c, nr); a.f = c->f; a.delta = balance ? c->buf0 : nullptr;
   a.map = map_dev; a.map_rstride = per_real ? (size_t)3 * ntm * c->npm1 : 0; a.npm1 = c->npm1;
@@ -2112,7 +2141,7 @@ int hft_analysis(hft_ctx *c, const double *fval_pt, const double *fval_scale, do
   HIPCHK(hipGetLastError());
   if (c->timing) {
     HIPCHK(hipEventRecord(e1, c->stream));
-    c->pending.push_back({e0, e1, 3, 1.0 * nr});
+    c->pending.push_back({e0, e1, 3, 1.0 * nr, 1.0 * nr});
   }
   HIPCHK(hipMemcpyAsync(out, c->d_hist, (size_t)nr * 14 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -2226,6 +2255,7 @@ int hft_set_timing(hft_ctx *c, int enable) {
     c->t_ms[g] = 0;
     c->t_n[g] = 0;
     c->t_units[g] = 0;
+    c->t_sunits[g] = 0;
   }
   c->timing = (enable & 1) != 0;
   c->phase_timing = (enable & 2) != 0;
@@ -2242,6 +2272,18 @@ int hft_kernel_time(hft_ctx *c, const char *group, double *total_ms, int64_t *la
       if (total_ms) *total_ms = c->t_ms[g];
       if (launches) *launches = c->t_n[g];
       if (alg_bytes) *alg_bytes = c->t_units[g] * 8.0 * (double)c->ntm * (double)c->nu;
+      return 0;
+    }
+  return fail(HFT_E_ARG, "unknown kernel group '%s'", group);
+}
+
+int hft_kernel_stage_bytes(hft_ctx *c, const char *group, double *stage_bytes) {
+  if (!c || !group || !stage_bytes) return fail(HFT_E_ARG, "null argument");
+  int rc = drain_timing(c);
+  if (rc) return rc;
+  for (int g = 0; g < kNGroups; g++)
+    if (strcmp(group, kGroups[g]) == 0) {
+      *stage_bytes = c->t_sunits[g] * 8.0 * (double)c->ntm * (double)c->nu;
       return 0;
     }
   return fail(HFT_E_ARG, "unknown kernel group '%s'", group);

@@ -233,6 +233,13 @@ module hft
       real(c_double), intent(out) :: total_ms, alg_bytes
       integer(c_int64_t), intent(out) :: launches
     end function
+    integer(c_int) function hft_kernel_stage_bytes(ctx, group, stage_bytes) &
+        bind(c, name="hft_kernel_stage_bytes")
+      import :: c_int, c_double, c_ptr, c_char
+      type(c_ptr), value :: ctx
+      character(kind=c_char), intent(in) :: group(*)
+      real(c_double), intent(out) :: stage_bytes
+    end function
     ! device-resident state and the discrete adjoint of one step
     integer(c_int) function hft_set_state_device(ctx, src, ld) bind(c, name="hft_set_state_device")
       import :: c_int, c_ptr, c_int64_t

@@ -274,7 +274,9 @@ int hft_get_tuning(hft_ctx *ctx, const char *key, int *value);
  * "analysis", "assim", "flow", "adjoint", or "rkg2_fused" = the multi-stage
  * RKG2 launches, which "rkg2" includes) since the reset: total device time (ms), number of
  * launches, and the algorithmic bytes those launches move (compulsory
- * reads + writes of FP64 fields over the ntm x (npm-2) unique points).
+ * reads + writes of FP64 fields over the ntm x (npm-2) unique points; a
+ * multi-stage launch counts what it must move: its input levels, u0 and
+ * its two output levels, not its intermediate stages).
  * With phase events on, the groups "phase_advection", "phase_diffusion",
  * "phase_source" and "phase_flux_transport" hold the stream time of every
  * hft_advection_step, hft_diffusion_step, hft_source_step and
@@ -284,6 +286,13 @@ int hft_get_tuning(hft_ctx *ctx, const char *key, int *value);
 int hft_set_timing(hft_ctx *ctx, int enable);
 int hft_kernel_time(hft_ctx *ctx, const char *group, double *total_ms, int64_t *launches,
                     double *alg_bytes);
+/* The bytes the same group's work would move one RKG2 stage per launch (5
+ * field passes per stage and realization): equal to alg_bytes except for the
+ * multi-stage "rkg2_fused" launches (and "rkg2", which includes them), whose
+ * intermediate stages stay on chip.  stage_bytes / alg_bytes is the traffic
+ * that temporal blocking saves.  No counterpart in the reference (its
+ * wtime_* timers, src/hipft.f90:478-486, count time only). */
+int hft_kernel_stage_bytes(hft_ctx *ctx, const char *group, double *stage_bytes);
 
 /* ---- differentiation (no counterpart in the reference) ---------------------
  * Device-resident state for callers that keep maps in HBM (PyTorch): src/dst

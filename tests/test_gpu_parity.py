@@ -370,6 +370,29 @@ def test_assimilation_matches_oracle(gpu_available, add_mode, balance, lat):
     assert_same(g.get_state("tp"), o.get_f(), "assimilation", 1e-14)
 
 
+@pytest.mark.parametrize("power,limit,mult", [(4.0, 0.5, 1.0), (3.0, 0.5, 1.3)])
+def test_assimilation_closed_form(gpu_available, power, limit, mult):
+    """The GPU blend against the closed form f + mu^p (mult d - f), strict
+    cut mu > limit (mu exactly at the limit keeps the state), the seam column
+    taking its periodic twin's increment (data column 1 unused) and pole rows
+    left as blended; the same inputs as the oracle's closed-form pins."""
+    from test_oracle_math import _mu_field, assim_expected
+    o, g = pair(31, 61, nr=1, clustered=False)
+    t, p = o.mesh["t"], o.mesh["p"][: o.npm1]
+    f0 = g.get_state("tp")[0].copy()
+    data = 40.0 * W.bipole_map(t, p, 5, 77) + 7.0
+    data[:, 0] = 1.0e4  # a first meridian that disagrees with the last: never read
+    mu = _mu_field(o, t, p)
+    g.assimilate(np.stack([data, np.zeros_like(data), mu]), custom_from_mu=True,
+                 mu_power=power, mu_limit=limit, mult_fac=mult)
+    f = g.get_state("tp")[0]
+    exp = assim_expected(f0, data, mu, power, limit, mult)
+    assert np.abs(f - exp).max() <= 4e-15 * np.abs(exp).max()
+    assert np.array_equal(f[5, 1:-1], f0[5, 1:-1])
+    assert np.all(f[6, 1:-1] != f0[6, 1:-1])
+    assert np.ptp(f[1:-1, 0]) > 0.5 and np.ptp(f[1:-1, -1]) > 0.5
+
+
 @pytest.mark.parametrize("case", ["advect_gaussians_phi", "advect_gaussians_theta",
                                   "advect_gaussians_phi_theta", "diffuse_soccer",
                                   "diffuse_advect_soccer"])
@@ -540,6 +563,22 @@ def test_banded_advection_bitexact(gpu_available, H):
     assert_same(g.get_state("tp"), o.get_f(), f"SSPRK(4,3) band H={H}")
 
 
+@pytest.mark.parametrize("H", [64, 128])
+def test_banded_advection_tall_bands_512_rows(gpu_available, H):
+    """512x1024 clustered, bands of 64 and 128 theta rows (the tuned shapes
+    of the benchmark batch): operator and two SSPRK(4,3) steps equal the
+    oracle bit for bit."""
+    o, g = pair(512, 1024, nr=2)
+    _flows(o, g, 2)
+    g.set_tuning(advection_band=1, advection_band_rows=H)
+    x = o.get_f()
+    assert_same(g.apply_advection_operator(x), o.advection_operator(x), f"A x band H={H}")
+    for _ in range(2):
+        o.advection_step(0.5)
+        g.advection_step(0.5)
+    assert_same(g.get_state("tp"), o.get_f(), f"SSPRK(4,3) band H={H} at 512x1024")
+
+
 @pytest.mark.parametrize("atten", [False, True])
 def test_banded_advection_matches_pointwise_full_size(gpu_available, atten):
     """512x1024, 1-D flow profiles and attenuated (2-D) flows: banded and
@@ -594,6 +633,40 @@ def test_graph_replay_matches_direct_launches(gpu_available, fused):
         g.close()
     for a, b in zip(out[1], out[0]):
         assert_same(a, b, f"graph replay (fused={fused})")
+
+
+def test_graph_replay_after_coef_table_growth(gpu_available):
+    """A super-step whose stage table outgrows the device coefficient table
+    (8 realizations x 146 stages > 1024 entries) reallocates it; graphs
+    captured before hold the old table's address, so they must be dropped:
+    repeating the first step's stage counts afterwards captures anew and
+    matches direct launches bit for bit."""
+    t, p = grid(91, 181)
+    nr = 8
+    maps = np.stack([W.bipole_map(t, p, 20, 400 + r) for r in range(nr)])
+    eta = np.linspace(150.0, 900.0, nr)
+    dts = [1.0, 1.0, 200.0, 1.0, 1.0]
+    out, builds = [], None
+    for graphs in (0, 1):
+        g = FluxTransport(t, p, nr, mode="independent")
+        g.set_state(maps, layout="pt")
+        g.set_diffusion(eta)
+        g.set_tuning(fused_min_realizations=1, graphs=graphs)
+        states = []
+        for dt in dts:
+            g.diffusion_step(dt, subcycles=1)
+            states.append(g.get_state("tp"))
+        s, _ = g.diffusion_stats()
+        if graphs:
+            builds = g.get_tuning("graph_builds")
+            hits = g.get_tuning("graph_hits")
+        out.append(states)
+        g.close()
+    assert s.max() == 11
+    # captures: dt=1 (replayed once), dt=200, dt=1 again after the table grew
+    assert builds == 3 and hits == 2, (builds, hits)
+    for i, (a, b) in enumerate(zip(out[1], out[0])):
+        assert_same(a, b, f"graph replay after table growth, step {i}")
 
 
 def test_driver_writes_map_files(gpu_available, tmp_path):

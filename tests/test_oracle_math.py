@@ -301,6 +301,105 @@ def test_assimilation_add_mode_and_balance():
     assert abs(fp + fm) < 1e-12 * (fp - fm)
 
 
+def assim_expected(f, data, mu, power, limit, mult=1.0):
+    """Closed form of one μ-weighted blend (src/hipft.f90:4242-4250,
+    4269-4271, 4120-4124, 4138-4140) for a state f (npm, ntm) and PT slices
+    data, mu (ntm, npm1): w = mu**power where mu > limit (strictly), else 0;
+    reference column k (2..npm1) blends with data column k, the seam columns
+    take the increment of their periodic twin (column 1 <- npm-1, column npm
+    <- 2, so data column 1 is never used); no pole re-average."""
+    npm, ntm = f.shape
+    w = np.where(mu > limit, np.power(np.abs(mu), power), 0.0)  # (ntm, npm1)
+    d = mult * data
+    out = f.copy()
+    for k in range(1, npm):  # 0-based column k <-> reference column k+1
+        kk = k + 1
+        src = kk if kk <= npm - 1 else 2  # data column (1-based) of this column
+        out[k] = f[k] + w[:, src - 1] * (d[:, src - 1] - f[k])
+    out[0] = out[npm - 2]
+    return out
+
+
+def _mu_field(o, t, p):
+    """A μ slice with every regime: negative (far side), below, exactly at and
+    just above the limit 0.5, and values up to 1 that vary along phi on the
+    pole rows too."""
+    T, P = np.meshgrid(t, p, indexing="ij")
+    mu = 0.05 + 0.95 * np.abs(np.cos(0.5 * P + 0.3 * T)) * (0.6 + 0.4 * np.sin(T + 0.2))
+    mu[:, 5] = 0.5
+    mu[:, 6] = np.nextafter(0.5, 1.0)
+    mu[:, 7] = -0.7
+    mu[0, :] = np.linspace(0.55, 1.0, o.npm1)   # north pole row varies in phi
+    mu[-1, :] = np.linspace(1.0, 0.6, o.npm1)   # south pole row too
+    return mu
+
+
+@pytest.mark.parametrize("power,limit,mult", [(4.0, 0.5, 1.0), (3.0, 0.5, 1.3), (2.5, 0.2, 0.9)])
+def test_assimilation_weight_law_closed_form(power, limit, mult):
+    """f <- f + mu^p (mult*d - f) with the strict cut mu > limit: pins the power
+    (mu^4 vs mu or mu^2), the cut's direction and equality case, the data
+    scale factor, and that pole rows stay as blended (not re-averaged)."""
+    o, t, p = _assim_setup()
+    f0 = o.get_f()[0].copy()
+    data = 40.0 * W.bipole_map(t, p, 5, 77) + 7.0
+    mu = _mu_field(o, t, p)
+    d = np.stack([data, np.zeros_like(data), mu])[None]
+    o.assimilate(d, custom_from_mu=True, mu_power=power, mu_limit=limit, mult_fac=mult)
+    f = o.get_f()[0]
+    exp = assim_expected(f0, data, mu, power, limit, mult)
+    scale = np.abs(exp).max()
+    assert np.abs(f - exp).max() <= 4e-15 * scale
+    # the equality case keeps the state, the next double above it does not
+    # (limit 0.5 cases; column 5 is reference column 6 = 0-based f column 5)
+    if limit == 0.5:
+        assert np.array_equal(f[5, 1:-1], f0[5, 1:-1])
+        assert np.all(f[6, 1:-1] != f0[6, 1:-1])
+    assert np.array_equal(f[7, 1:-1], f0[7, 1:-1])  # far side (mu < 0)
+    # poles: not re-averaged after the blend
+    assert np.ptp(f[1:-1, 0]) > 0.5 and np.ptp(f[1:-1, -1]) > 0.5
+
+
+def test_assimilation_seam_takes_its_periodic_twin():
+    """Data column 1 (the first meridian of a PT map) is never read: the seam
+    column's increment is overwritten by that of column npm-1 (set_periodic_bc_3d
+    on deltaf, src/hipft.f90:4124), so a map whose first column disagrees with
+    its last gives the same state as a periodic one."""
+    o1, t, p = _assim_setup()
+    o2, _, _ = _assim_setup()
+    data = W.bipole_map(t, p, 5, 78) * 30.0
+    mu = _mu_field(o1, t, p)
+    bad = data.copy()
+    bad[:, 0] = 1.0e4  # disagrees with data[:, -1]
+    for o, dd in ((o1, data), (o2, bad)):
+        o.assimilate(np.stack([dd, np.zeros_like(dd), mu])[None], custom_from_mu=True,
+                     mu_power=4.0, mu_limit=0.5)
+    f1, f2 = o1.get_f()[0], o2.get_f()[0]
+    assert np.array_equal(f1, f2)
+    assert np.array_equal(f1[0], f1[o1.npm - 2]) and np.array_equal(f1[-1], f1[1])
+
+
+def test_flux_area_weights_converge_to_sphere_area():
+    """get_flux of a constant map c is c times the discrete sphere area. On a
+    uniform grid the interior rows are the trapezoid rule of the integral of
+    sin(theta) (Euler-Maclaurin: 2 - h^2/6) and the two quarter pole cells add
+    2 * (1/4) sin(h/2) h ~ h^2/4, so area/(4 pi) - 1 = h^2/24 + O(h^4). A
+    dropped pole cell (-h^2/12) or a half instead of a quarter (+h^2/6) fails."""
+    c = 3.0
+    ratios = []
+    for ntm in (41, 81, 161):
+        t, p = grid(ntm, 2 * (ntm - 1))
+        o = Oracle(t, p, 1)
+        fp, fm = o.get_flux(np.full((o.npm, o.ntm), c))
+        assert fm == 0.0
+        h = PI / (ntm - 1)
+        ratios.append((fp / (4.0 * PI * c) - 1.0) / h ** 2)
+    assert abs(ratios[-1] - 1.0 / 24.0) < 2e-3 / 24.0
+    assert abs(ratios[0] - 1.0 / 24.0) < 2e-2 / 24.0
+    # negative constant: all flux in the negative sum
+    fp, fm = o.get_flux(np.full((o.npm, o.ntm), -c))
+    assert fp == 0.0 and fm < 0.0
+
+
 # ---- validation fields ---------------------------------------------------------------------
 
 def test_tesseral_harmonic_closed_form():
