@@ -80,6 +80,8 @@ _SIGS = [
                                              ctypes.c_int, ctypes.c_int]),
     ("hft_set_validation_field", ctypes.c_int, [ctypes.c_void_p, _D]),
     ("hft_analysis", ctypes.c_int, [ctypes.c_void_p, _D, _D, _D]),
+    ("hft_analysis_noalloc", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
+                                            ctypes.c_void_p]),
     ("hft_state_device", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p),
                                         ctypes.POINTER(ctypes.c_int)]),
     ("hft_synchronize", ctypes.c_int, [ctypes.c_void_p]),
@@ -398,6 +400,13 @@ class FluxTransport:
         s = _vec(fval_scale, self.nr)
         _check(lib().hft_analysis(self._h, _p(a), _p(s), _p(out)), "hft_analysis")
         return out
+
+    def analysis_noalloc(self, out_ptr, f_ptr=None, ld=0):
+        """hft_analysis_noalloc: diagnostics of the state (or of a device batch
+        f_ptr with row length ld) into the device array out_ptr [nr][14], on
+        the context's stream, allocating nothing."""
+        _check(lib().hft_analysis_noalloc(self._h, f_ptr, int(ld), out_ptr),
+               "hft_analysis_noalloc")
 
     # ---- device access / timing -------------------------------------------------
     def state_device(self):

@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstring>
 #include <functional>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -204,6 +205,7 @@ struct hft_ctx {
   // timing
   bool timing = false;        // events around every launch
   bool phase_timing = false;  // events at the boundaries of the step's phases
+  double *scratch_dev = nullptr;  // this device's copy of `scratch` (hft_analysis_noalloc)
   std::vector<TimedPair> pending;
   std::vector<hipEvent_t> free_ev;
   double t_ms[kNGroups] = {0};
@@ -1145,6 +1147,35 @@ int ptl_pass(hft_ctx *c, const std::vector<int> &reals, bool coupled,
 
 }  // namespace
 
+// Partial sums of hft_analysis_noalloc: a file-scope device array, so that
+// entry point allocates nothing.  Every GPU the library runs on holds its own
+// copy (8 MiB) until the process ends; host code must use the per-device
+// address from hipGetSymbolAddress, never the array's name.
+constexpr size_t kScratchDoubles = (size_t)1 << 20;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+namespace {
+std::mutex g_scratch_mu;
+std::vector<double *> g_scratch_ptr;  // per device id
+
+// Address of this device's `scratch`, looked up once per device with that
+// device current (called from hft_create).
+int init_scratch(hft_ctx *c) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if ((int)g_scratch_ptr.size() <= c->device) g_scratch_ptr.resize(c->device + 1, nullptr);
+  if (!g_scratch_ptr[c->device]) {
+    void *ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return fail(HFT_E_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", c->device,
+                  hipGetErrorString(e));
+    g_scratch_ptr[c->device] = (double *)ptr;
+  }
+  c->scratch_dev = g_scratch_ptr[c->device];
+  return 0;
+}
+}  // namespace
+
 // ============================================================================
 // C ABI
 // ============================================================================
@@ -1186,6 +1217,7 @@ int hft_create(int ntm, int npm1, const double *t, const double *p, int nr, int 
   }
   build_mesh(c, t, p);
   int rc = setup_device(c);
+  if (!rc) rc = init_scratch(c);
   if (rc) {
     std::string msg = g_err;
     hft_destroy(c);
@@ -2145,6 +2177,40 @@ int hft_analysis(hft_ctx *c, const double *fval_pt, const double *fval_scale, do
   }
   HIPCHK(hipMemcpyAsync(out, c->d_hist, (size_t)nr * 14 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int hft_analysis_noalloc(hft_ctx *c, const double *f_dev, int64_t ld, double *out_dev) {
+  if (!c || !out_dev) return fail(HFT_E_ARG, "hft_analysis_noalloc: null argument");
+  if (!c->scratch_dev) return fail(HFT_E_STATE, "hft_analysis_noalloc: no scratch address");
+  int nr = c->nr;
+  size_t need = (size_t)nr * c->ana_nblk * kNPart;
+  if (need > kScratchDoubles)
+    return fail(HFT_E_ARG,
+                "hft_analysis_noalloc: %d realizations x %d row blocks x %d partials = %zu "
+                "doubles exceed the %zu-double scratch array",
+                nr, c->ana_nblk, kNPart, need, kScratchDoubles);
+  if (f_dev && ld < c->nu)
+    return fail(HFT_E_ARG, "hft_analysis_noalloc: ld %lld < %d unique phi columns", (long long)ld,
+                c->nu);
+  AnaArgs a;
+  a.f = f_dev ? f_dev : c->f;
+  a.fval = nullptr;
+  a.fval_scale = nullptr;
+  a.ntm = c->ntm; a.nu = c->nu; a.nr = nr;
+  a.pitch = f_dev ? (int)ld : c->pitch;
+  a.rstride = f_dev ? (size_t)c->ntm * (size_t)ld : c->rstride;
+  a.da_t = c->d_da_t; a.sn_t = c->d_sn_t; a.cs_t = c->d_cs_t;
+  a.tt = c->d_tt; a.da_p = c->d_da_p; a.cs_p = c->d_cs_p; a.sn_p = c->d_sn_p; a.pcol = c->d_pcol;
+  a.lat_n = 30.0 * d2r;
+  a.lat_s = pi - 30.0 * d2r;
+  a.rows_per_block = c->ana_rows; a.part = c->scratch_dev; a.nblk = c->ana_nblk;
+  // every partial is written before it is read: nothing to clear
+  hipLaunchKernelGGL(k_analysis_part, dim3(c->ana_nblk, nr), dim3(kBlock), 0, c->stream, a);
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(k_analysis_final, dim3((nr + 63) / 64), dim3(64), 0, c->stream,
+                     (const double *)c->scratch_dev, c->ana_nblk, nr, c->ntm, c->npm, out_dev);
+  HIPCHK(hipGetLastError());
   return 0;
 }
 

@@ -193,6 +193,14 @@ module hft
       type(c_ptr), value :: ctx, fval, fval_scale
       real(c_double), intent(out) :: out(*)
     end function
+    ! no counterpart in the reference: hft_analysis for callers that cannot
+    ! allocate (device input and output, partial sums in a fixed device array)
+    integer(c_int) function hft_analysis_noalloc(ctx, f_dev, ld, out_dev) &
+        bind(c, name="hft_analysis_noalloc")
+      import :: c_int, c_ptr, c_int64_t
+      type(c_ptr), value :: ctx, f_dev, out_dev
+      integer(c_int64_t), value :: ld
+    end function
     integer(c_int) function hft_state_device(ctx, f_dev, pitch) bind(c, name="hft_state_device")
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx

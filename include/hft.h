@@ -210,6 +210,18 @@ int hft_assimilate_device(hft_ctx *ctx, const double *map_dev, int per_realizati
  * used, multiplied per realization by fval_scale[r]. */
 int hft_set_validation_field(hft_ctx *ctx, const double *fval_pt);
 int hft_analysis(hft_ctx *ctx, const double *fval_pt, const double *fval_scale, double *out);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops): the history
+ * diagnostics of hft_analysis without a validation field (its
+ * VALIDATION_ERR column is 0), computed on the context's stream without
+ * any allocation, copy or synchronisation.  f_dev: a device batch laid out
+ * [r][j][c] with row length ld >= npm-2 (unique phi columns), or NULL for
+ * the context's state; out_dev: device array [nr][HFT_NHIST].  The block
+ * partial sums live in a fixed 8 MiB device array shared by every context
+ * on a GPU (nr x row blocks x 16 doubles, at most 2^20; larger batches get
+ * HFT_E_ARG), so calls that overlap on one GPU (other streams or host
+ * threads) race; calls on one stream are ordered. */
+int hft_analysis_noalloc(hft_ctx *ctx, const double *f_dev, int64_t ld, double *out_dev);
 
 /* ---- device access (Python/PyTorch interop, benchmarks) -------------------
  * Device pointer of the state and its layout: f_dev[(r*ntm + j)*pitch + c],
