@@ -79,6 +79,9 @@ _SIGS = [
                                              ctypes.c_int, _D, _D, _D, ctypes.c_double,
                                              ctypes.c_int, ctypes.c_int]),
     ("hft_set_validation_field", ctypes.c_int, [ctypes.c_void_p, _D]),
+    ("hft_set_weno_eps", ctypes.c_int, [ctypes.c_void_p, ctypes.c_double]),
+    ("hft_set_pole_flux_lat_limit", ctypes.c_int, [ctypes.c_void_p, ctypes.c_double]),
+    ("hft_balance_flux", ctypes.c_int, [ctypes.c_void_p, _D]),
     ("hft_analysis", ctypes.c_int, [ctypes.c_void_p, _D, _D, _D]),
     ("hft_analysis_noalloc", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
                                             ctypes.c_void_p]),
@@ -392,6 +395,24 @@ class FluxTransport:
         a = None if fval_pt is None else np.ascontiguousarray(
             np.broadcast_to(np.asarray(fval_pt, dtype=np.float64), self.pt_shape))
         _check(lib().hft_set_validation_field(self._h, _p(a)), "hft_set_validation_field")
+
+    def set_weno_eps(self, eps):
+        """weno_eps (load_weno): <= 0 uses the cell widths (the default)."""
+        _check(lib().hft_set_weno_eps(self._h, float(eps)), "hft_set_weno_eps")
+
+    def set_pole_flux_lat_limit(self, deg):
+        _check(lib().hft_set_pole_flux_lat_limit(self._h, float(deg)),
+               "hft_set_pole_flux_lat_limit")
+
+    def balance_flux(self, copy=False):
+        """balance_flux of every realization: in place, or (copy=True) a
+        balanced PT copy of the state, returned, with the state unchanged."""
+        if not copy:
+            _check(lib().hft_balance_flux(self._h, None), "hft_balance_flux")
+            return None
+        out = np.zeros(self.pt_shape)
+        _check(lib().hft_balance_flux(self._h, _p(out)), "hft_balance_flux")
+        return out
 
     def analysis(self, fval_pt=None, fval_scale=None):
         out = np.zeros((self.nr, NHIST))

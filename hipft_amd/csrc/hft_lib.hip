@@ -206,6 +206,7 @@ struct hft_ctx {
   bool timing = false;        // events around every launch
   bool phase_timing = false;  // events at the boundaries of the step's phases
   double *scratch_dev = nullptr;  // this device's copy of `scratch` (hft_analysis_noalloc)
+  double pole_lat = 30.0;         // pole_flux_lat_limit, degrees (src/hipft.f90:566)
   std::vector<TimedPair> pending;
   std::vector<hipEvent_t> free_ev;
   double t_ms[kNGroups] = {0};
@@ -519,6 +520,16 @@ void dev_to_tp(const hft_ctx *c, const std::vector<double> &dev, double *tp) {
         tp[j + (size_t)c->ntm * ((k - 1) + (size_t)c->npm * r)] =
             dev[fidx(r, j, cc, c->ntm, c->pitch)];
     }
+}
+
+// device layout -> PT map (npm1, ntm, nr), one-point phi overlap
+void dev_to_pt(const hft_ctx *c, const std::vector<double> &dev, double *f) {
+  for (int r = 0; r < c->nr; r++)
+    for (int j = 0; j < c->ntm; j++)
+      for (int l = 1; l <= c->npm1; l++) {
+        int cc = (l == 1) ? c->nu - 1 : l - 2;
+        f[(l - 1) + (size_t)c->npm1 * (j + (size_t)c->ntm * r)] = dev[fidx(r, j, cc, c->ntm, c->pitch)];
+      }
 }
 
 // Validation field PT (npm1,ntm,nr) -> device layout.  analysis_step pairs
@@ -1352,12 +1363,7 @@ int hft_get_state(hft_ctx *c, double *f, int layout) {
   if (layout == HFT_LAYOUT_TP) {
     dev_to_tp(c, dev, f);
   } else if (layout == HFT_LAYOUT_PT) {
-    for (int r = 0; r < c->nr; r++)
-      for (int j = 0; j < c->ntm; j++)
-        for (int l = 1; l <= c->npm1; l++) {
-          int cc = (l == 1) ? c->nu - 1 : l - 2;
-          f[(l - 1) + (size_t)c->npm1 * (j + (size_t)c->ntm * r)] = dev[fidx(r, j, cc, c->ntm, c->pitch)];
-        }
+    dev_to_pt(c, dev, f);
   } else {
     return fail(HFT_E_ARG, "hft_get_state: bad layout %d", layout);
   }
@@ -2156,8 +2162,8 @@ int hft_analysis(hft_ctx *c, const double *fval_pt, const double *fval_scale, do
   a.f = c->f; a.fval = fv; a.fval_scale = scale; a.ntm = c->ntm; a.nu = c->nu; a.pitch = c->pitch;
   a.nr = nr; a.rstride = c->rstride; a.da_t = c->d_da_t; a.sn_t = c->d_sn_t; a.cs_t = c->d_cs_t;
   a.tt = c->d_tt; a.da_p = c->d_da_p; a.cs_p = c->d_cs_p; a.sn_p = c->d_sn_p; a.pcol = c->d_pcol;
-  a.lat_n = 30.0 * d2r;
-  a.lat_s = pi - 30.0 * d2r;
+  a.lat_n = c->pole_lat * d2r;
+  a.lat_s = pi - c->pole_lat * d2r;
   a.rows_per_block = c->ana_rows; a.part = c->d_part; a.nblk = c->ana_nblk;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (c->timing) {
@@ -2202,8 +2208,8 @@ int hft_analysis_noalloc(hft_ctx *c, const double *f_dev, int64_t ld, double *ou
   a.rstride = f_dev ? (size_t)c->ntm * (size_t)ld : c->rstride;
   a.da_t = c->d_da_t; a.sn_t = c->d_sn_t; a.cs_t = c->d_cs_t;
   a.tt = c->d_tt; a.da_p = c->d_da_p; a.cs_p = c->d_cs_p; a.sn_p = c->d_sn_p; a.pcol = c->d_pcol;
-  a.lat_n = 30.0 * d2r;
-  a.lat_s = pi - 30.0 * d2r;
+  a.lat_n = c->pole_lat * d2r;
+  a.lat_s = pi - c->pole_lat * d2r;
   a.rows_per_block = c->ana_rows; a.part = c->scratch_dev; a.nblk = c->ana_nblk;
   // every partial is written before it is read: nothing to clear
   hipLaunchKernelGGL(k_analysis_part, dim3(c->ana_nblk, nr), dim3(kBlock), 0, c->stream, a);
@@ -2211,6 +2217,45 @@ int hft_analysis_noalloc(hft_ctx *c, const double *f_dev, int64_t ld, double *ou
   hipLaunchKernelGGL(k_analysis_final, dim3((nr + 63) / 64), dim3(64), 0, c->stream,
                      (const double *)c->scratch_dev, c->ana_nblk, nr, c->ntm, c->npm, out_dev);
   HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int hft_set_weno_eps(hft_ctx *c, double eps) {
+  if (!c) return fail(HFT_E_ARG, "null context");
+  if (!(eps == eps)) return fail(HFT_E_ARG, "hft_set_weno_eps: eps is NaN");
+  // load_weno, src/hipft.f90:6895-6901
+  for (int j = 1; j <= c->ntm; j++) c->wt[6][j] = eps <= zero ? c->dt[j] : eps;
+  for (int k = 1; k <= c->npm; k++) c->wp[6][k] = eps <= zero ? c->dp[k] : eps;
+  // launches already queued may still read the device copies
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipMemcpy((void *)c->m.teps, c->wt[6].data() + 1, (c->wt[6].size() - 1) * sizeof(double),
+                   hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy((void *)c->m.peps, c->wp[6].data() + 1, (c->wp[6].size() - 1) * sizeof(double),
+                   hipMemcpyHostToDevice));
+  return 0;
+}
+
+int hft_set_pole_flux_lat_limit(hft_ctx *c, double deg) {
+  if (!c) return fail(HFT_E_ARG, "null context");
+  if (!(deg == deg)) return fail(HFT_E_ARG, "hft_set_pole_flux_lat_limit: NaN");
+  c->pole_lat = deg;
+  return 0;
+}
+
+int hft_balance_flux(hft_ctx *c, double *out_pt) {
+  if (!c) return fail(HFT_E_ARG, "null context");
+  int rc;
+  if (!out_pt) {
+    if ((rc = balance_field(c, c->f))) return rc;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+  }
+  HIPCHK(hipMemcpyAsync(c->buf0, c->f, c->nfield * sizeof(double), hipMemcpyDeviceToDevice,
+                        c->stream));
+  if ((rc = balance_field(c, c->buf0))) return rc;
+  std::vector<double> dev;
+  if ((rc = get_field(c, c->buf0, dev))) return rc;
+  dev_to_pt(c, dev, out_pt);
   return 0;
 }
 

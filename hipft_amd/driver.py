@@ -64,17 +64,71 @@ DEFAULTS = dict(
     advance_source=False, source_rfe_model=0, source_rfe_total_unsigned_flux_per_hour=0.0,
     source_rfe_total_unsigned_flux_per_hours=None, source_rfe_seed=-9999,
     source_rfe_lifetime=0.3, source_from_file=False, source_filename=None, source_map=None,
+    initial_map_filename="", initial_map_flux_balance=False, output_map_idx_cadence=0,
+    output_map_flux_balance=False, output_flows=False, output_flows_directory="output_flows",
+    dt_min=1.0e-15, pole_flux_lat_limit=30.0, flow_rigid_omega=0.0, flow_num_method=4,
+    weno_eps=0.0, upwind=1.0, diffusion_coef_filename="", diffusion_coef_grid=False,
+    diffusion_num_method=3, run_directory=None,
 )
+
+MAX_REALIZATIONS = 2000  # src/hipft.f90:236
+
+# Reference options this build does not implement, with the only value it
+# accepts: a hipft.in asking for anything else is refused rather than run
+# with a different method.
+UNSUPPORTED = (
+    ("flow_num_method", 4, "only SSPRK(4,3) + WENO3 (flow_num_method = 4) is built"),
+    ("diffusion_num_method", 3,
+     "only RKG2 super time-stepping (diffusion_num_method = 3) is built"),
+    ("upwind", 1.0, "the WENO3 pole faces are built with full upwinding (upwind = 1)"),
+    ("flow_rigid_omega", 0.0,
+     "the rigid rotation through the poles (src/hipft.f90:3450-3487) is not built"),
+    ("diffusion_coef_filename", "",
+     "diffusivity maps from files (src/hipft.f90:6706-6800) are not built"),
+    ("diffusion_coef_grid", False,
+     "the grid-dependent diffusivity (src/hipft.f90:6804-6819) is not built"),
+    ("output_flows", False, "flow map output (write_flows) is not built"),
+)
+
+
+def check_config(c):
+    """process_input_parameters' checks (src/hipft.f90:8482-8540) and the
+    options this build refuses; raises ValueError."""
+    errs = []
+    for key, ok, why in UNSUPPORTED:
+        v = c.get(key, ok)
+        if isinstance(ok, str):
+            bad = str(v or "").strip() != ok
+        elif isinstance(ok, bool):
+            bad = bool(v) != ok
+        else:
+            bad = float(v) != ok
+        if bad:
+            errs.append(f"{key} = {v!r}: {why}")
+    if c["output_map_time_cadence"] > 0 and c["output_map_idx_cadence"] > 0:
+        errs.append("output_map_time_cadence and output_map_idx_cadence cannot both be set")
+    if int(c["n_realizations"]) > MAX_REALIZATIONS:
+        errs.append(f"n_realizations = {c['n_realizations']} > {MAX_REALIZATIONS}")
+    if c["initial_map"] is None and (c["res_nt"] < 3 or c["res_np"] < 3):
+        errs.append("no initial map and res_nt, res_np < 3")
+    if errs:
+        raise ValueError("hipft.in: " + "; ".join(errs))
 
 
 def per_realization(cfg, name, nr):
     """Per-realization values: the `<name>s` list where given, else `<name>`
-    (process_input_parameters defaults, src/hipft.f90:8542-8579)."""
+    (process_input_parameters defaults, src/hipft.f90:8542-8579): an entry
+    left unset or at its sentinel (below 0, below -1e5 for the flow
+    coefficients) takes the scalar value."""
     out = np.full(nr, float(cfg[name]))
     vals = cfg.get(name + "s")
+    low = -100000.0 if name.startswith(("flow_dr_coef", "flow_mf_coef")) else 0.0
     if vals is not None:
+        if not isinstance(vals, (list, tuple, np.ndarray)):
+            vals = [vals]
         for i, v in enumerate(list(vals)[:nr]):
-            out[i] = v
+            if v is not None and float(v) >= low:
+                out[i] = v
     return out
 
 
@@ -82,6 +136,7 @@ class Run:
     def __init__(self, cfg, mode="coupled"):
         c = dict(DEFAULTS)
         c.update(cfg)
+        check_config(c)
         self.c = c
         nr = int(c["n_realizations"])
         self.vfield = None
@@ -104,7 +159,20 @@ class Run:
             self.vfield = ("gauss", None)
         self.t, self.p = s2, s1
         self.model = FluxTransport(s2, s1, nr, mode=mode)
-        self.model.set_state(f_local, layout="pt", mult_fac=c["initial_map_mult_fac"])
+        if c["initial_map_flux_balance"]:
+            # load_initial_condition: balance_flux after the pole averaging,
+            # then the multiplication factor (2163-2173)
+            self.model.set_state(f_local, layout="pt")
+            self.model.balance_flux()
+            if c["initial_map_mult_fac"] != 1.0:
+                f_tp = self.model.get_state("tp")
+                self.model.set_state(c["initial_map_mult_fac"] * f_tp, layout="tp")
+        else:
+            self.model.set_state(f_local, layout="pt", mult_fac=c["initial_map_mult_fac"])
+        if c["weno_eps"] != 0.0:
+            self.model.set_weno_eps(c["weno_eps"])
+        if c["pole_flux_lat_limit"] != 30.0:
+            self.model.set_pole_flux_lat_limit(c["pole_flux_lat_limit"])
         self.nr = nr
         self.eta = per_realization(c, "diffusion_coef_constant", nr)
         if self.vfield is not None and self.vfield[0] == "soccer":
@@ -218,35 +286,67 @@ def run(cfg, max_steps=None, mode="coupled"):
                            st["sts_s"][r] if sts_s is None else sts_s[r],
                            st["subcycles"][r] if subcycles is None else subcycles[r]])
 
+    rundir = c["run_directory"]
+    cadence_on = c["output_map_idx_cadence"] > 0 or c["output_map_time_cadence"] > 0
+    first_map = [cadence_on]  # the initial condition is always output (1214-1218)
+    map_list = None
+    if rundir is not None and cadence_on:
+        # hipft_output_map_list.out, header (1654-1665)
+        map_list = os.path.join(rundir, "hipft_output_map_list.out")
+        with open(map_list, "w") as fh:
+            fh.write("IDX".rjust(10) + " " + "TIME".rjust(25) + " " + "FILENAME\n")
+
     def map_due(time):
+        # output_map (2726-2737): every output_map_idx_cadence steps, or at
+        # each multiple of output_map_time_cadence
+        due = first_map[0]
+        first_map[0] = False
+        icad = c["output_map_idx_cadence"]
+        if icad > 0 and st["ntime"] % icad == 0:
+            due = True
         cad = c["output_map_time_cadence"]
-        return cad > 0 and time >= c["time_start"] + (
-            st["idx_out"] - (c["output_map_idx_start"] - 1)) * cad
+        if cad > 0 and time >= c["time_start"] + (
+                st["idx_out"] - (c["output_map_idx_start"] - 1)) * cad:
+            due = True
+        return due
 
-    def write_map(name):
+    def write_map(name, time=None):
         with _span("io"):
-            _write_map(name)
+            base = _write_map(name, time is not None)
+            if base and map_list and time is not None:
+                with open(map_list, "a") as fh:  # (i10,a,1pe25.15,a,a), 2768
+                    fh.write(f"{st['idx_out']:10d}   {_fortran_pe(time, 25, 15)}    {base}\n")
 
-    def _write_map(name):
-        # write_map (src/hipft.f90:2801-2917): PT maps, one-point phi overlap
-        d = c["output_map_directory"]
-        if not d:
-            return
+    def _write_map(name, indexed):
+        # write_map (src/hipft.f90:2801-2917): PT maps, one-point phi overlap.
+        # As the executable (run_directory set): the initial and final maps in
+        # the run directory, the indexed ones in output_map_directory (2742-
+        # 2756); as a library call: every map in output_map_directory, if set.
+        if rundir is not None:
+            d = os.path.join(rundir, c["output_map_directory"] or "output_maps") if indexed \
+                else rundir
+        else:
+            d = c["output_map_directory"]
+            if not d:
+                return None
         from . import h5io
         os.makedirs(d, exist_ok=True)
-        maps = m.get_state("pt")
-        path = os.path.join(d, f"{c['output_map_root_filename']}{name}.h5")
+        # output_map_flux_balance: a balanced copy is written (2839-2843)
+        maps = m.balance_flux(copy=True) if c["output_map_flux_balance"] else m.get_state("pt")
+        base = f"{c['output_map_root_filename']}{name}.h5"
+        path = os.path.join(d, base)
         if c["output_map_2d"] and nr == 1:
             h5io.write_map(path, maps[0], [R.p, R.t], single=c["output_single_precision"])
         else:
             h5io.write_map(path, maps, [R.p, R.t, np.arange(1, nr + 1, dtype=np.float64)],
                            single=c["output_single_precision"])
+        return base
 
     write_map("_initial")  # load_initial_condition, 2185
     emit(st["time"])
     if map_due(st["time"]):
         st["idx_out"] += 1
-        write_map(f"_idx{st['idx_out']:06d}")
+        write_map(f"_idx{st['idx_out']:06d}", st["time"])
     steps = 0
     tm["setup"] = _clock.perf_counter() - wall0
     while True:
@@ -291,6 +391,8 @@ def run(cfg, max_steps=None, mode="coupled"):
                     st["dtime"] = st["dtmax_flow"]
             if c["dt_max"] < st["dtime"]:
                 st["dtime"] = c["dt_max"]
+            if st["dtime"] < c["dt_min"]:  # 4378-4380
+                print("   UPDATE_TIMESTEP: Warning! Time step is smaller than DTMIN!", flush=True)
             st["ts_needs"] = False
         cad = c["output_map_time_cadence"]
         if cad > 0:
@@ -355,7 +457,7 @@ def run(cfg, max_steps=None, mode="coupled"):
         emit(st["time"])
         if map_due(st["time"]):
             st["idx_out"] += 1
-            write_map(f"_idx{st['idx_out']:06d}")  # output_map_step, 2725-2760
+            write_map(f"_idx{st['idx_out']:06d}", st["time"])  # output_map, 2725-2760
         steps += 1
         if st["time"] >= c["time_end"] or (max_steps is not None and steps >= max_steps):
             break
@@ -365,6 +467,14 @@ def run(cfg, max_steps=None, mode="coupled"):
     m.set_timing(False)
     tm["total"] = _clock.perf_counter() - wall0
     return np.array(sol), np.array(num), R
+
+
+def _fortran_pe(x, width, digits):
+    """Fortran 1pe<width>.<digits> (exponent E+nn, or +nnn beyond 99)."""
+    mant, ex = f"{x:.{digits}E}".split("E")
+    ex = int(ex)
+    tail = f"E{ex:+03d}" if abs(ex) <= 99 else f"{ex:+04d}"
+    return f"{mant}{tail}".rjust(width)
 
 
 def _fortran_e(x, width, digits):
@@ -400,6 +510,83 @@ def write_realization_parameters(cfg, directory="."):
             fh.write(f"{i + 1:41d} " +
                      "".join(_fortran_e(cols[k][i], 40, 16) + " "
                              for k in REALIZATION_PARAMETERS) + "\n")
+    return path
+
+
+# Per-realization arrays of the namelist and the scalar each defaults to.
+ARRAY_DEFAULTS = {
+    "flow_attenuate_values": "flow_attenuate_value",
+    "flow_dr_coef_p0_values": "flow_dr_coef_p0_value",
+    "flow_dr_coef_p2_values": "flow_dr_coef_p2_value",
+    "flow_dr_coef_p4_values": "flow_dr_coef_p4_value",
+    "flow_mf_coef_p1_values": "flow_mf_coef_p1_value",
+    "flow_mf_coef_p3_values": "flow_mf_coef_p3_value",
+    "flow_mf_coef_p5_values": "flow_mf_coef_p5_value",
+    "diffusion_coef_constants": "diffusion_coef_constant",
+    "assimilate_data_mu_powers": "assimilate_data_mu_power",
+    "assimilate_data_lat_limits": "assimilate_data_lat_limit",
+    "assimilate_data_mu_limits": "assimilate_data_mu_limit",
+    "source_rfe_total_unsigned_flux_per_hours": "source_rfe_total_unsigned_flux_per_hour",
+}
+# namelist defaults of the reference (module input_parameters, 493-726) that
+# differ from the driver's own library defaults
+_NAMELIST_DEFAULTS = dict(output_map_directory="output_maps", flow_list_filename=" ",
+                          diffusion_coef_filename=" ", source_filename=" ",
+                          assimilate_data_map_list_filename=" ")
+
+
+def run_parameters_used(cfg):
+    """The namelist values as the run uses them, after process_input_parameters
+    (src/hipft.f90:8482-8579): every key of the group, the per-realization
+    arrays at their MAX_REALIZATIONS length with unset entries taken from the
+    scalars, res_nt/res_np cleared when a map file sets the grid."""
+    from .namelist import REFERENCE_KEYS
+    c = dict(DEFAULTS)
+    c.update(_NAMELIST_DEFAULTS)
+    c.update({k: v for k, v in cfg.items() if v is not None})
+    if c.get("initial_map_filename") and c["res_nt"] * c["res_np"] > 0:
+        c["res_nt"] = c["res_np"] = 0
+    out = {}
+    for k in REFERENCE_KEYS:
+        if k in ARRAY_DEFAULTS:
+            out[k] = list(per_realization(c, ARRAY_DEFAULTS[k], MAX_REALIZATIONS))
+        else:
+            v = c.get(k)
+            out[k] = "" if v is None else v
+    return out
+
+
+def write_run_parameters(cfg, directory="."):
+    """hipft_run_parameters_used.out (read_input_file, src/hipft.f90:8450-8454):
+    the &hipft_input_parameters group with the values the run used, one item
+    a line, arrays continued on the following lines.  The reference writes it
+    with the Fortran runtime's list-directed namelist output, whose number
+    and string layout depends on the compiler; this file carries the same
+    names and values (reals at 17 significant digits, strings quoted)."""
+    vals = run_parameters_used(cfg)
+
+    def fmt(v):
+        if isinstance(v, (bool, np.bool_)):
+            return "T" if v else "F"
+        if isinstance(v, (int, np.integer)):
+            return f"{int(v):12d}"
+        if isinstance(v, (float, np.floating)):
+            return _fortran_e(float(v), 25, 16)
+        return "'" + str(v).strip() + "'"
+
+    lines = [" &HIPFT_INPUT_PARAMETERS"]
+    for k, v in vals.items():
+        head = f" {k.upper()} ="
+        if isinstance(v, list):
+            lines.append(f"{head} {fmt(v[0])},")
+            pad = " " * (len(head) + 1)
+            lines += [f"{pad}{fmt(x)}," for x in v[1:]]
+        else:
+            lines.append(f"{head} {fmt(v)},")
+    lines.append(" /")
+    path = os.path.join(directory, "hipft_run_parameters_used.out")
+    with open(path, "w") as fh:
+        fh.write("\n".join(lines) + "\n")
     return path
 
 
@@ -475,7 +662,10 @@ def main(path="hipft.in", directory=None, mode="coupled"):
     t0 = _clock.perf_counter()
     base = os.path.dirname(os.path.abspath(path))
     directory = directory or "."
-    cfg = namelist.to_run_config(namelist.read(path), base)
+    nml = namelist.read(path)
+    cfg = namelist.to_run_config(nml, base)
+    cfg["run_directory"] = directory
+    write_run_parameters(dict(nml, **{k: v for k, v in cfg.items() if k in nml}), directory)
     write_realization_parameters(cfg, directory)
     sol, num, R = run(cfg, mode=mode)
     t1 = _clock.perf_counter()

@@ -79,6 +79,35 @@ def read(path):
         return parse(fh.read())
 
 
+# The reference's namelist group in its order (read_input_file,
+# src/hipft.f90:8324-8406); any other name is an error, as it is for a
+# Fortran namelist read.
+REFERENCE_KEYS = (
+    "verbose", "res_nt", "res_np", "n_realizations", "initial_map_filename",
+    "initial_map_flux_balance", "initial_map_mult_fac", "validation_run", "validation_run_width",
+    "time_start", "time_end", "output_map_idx_start", "output_map_directory",
+    "output_map_root_filename", "output_map_idx_cadence", "output_map_time_cadence",
+    "output_map_2d", "output_map_flux_balance", "output_flows", "output_flows_directory",
+    "output_history_time_cadence", "output_single_precision", "dt_min", "dt_max",
+    "strang_splitting", "pole_flux_lat_limit", "advance_flow", "flow_vp_rigid_omega",
+    "flow_rigid_omega", "flow_vt_const", "flow_attenuate", "flow_attenuate_value",
+    "flow_attenuate_values", "flow_dr_model", "flow_dr_coef_p0_value", "flow_dr_coef_p2_value",
+    "flow_dr_coef_p4_value", "flow_dr_coef_p0_values", "flow_dr_coef_p2_values",
+    "flow_dr_coef_p4_values", "flow_mf_model", "flow_mf_coef_p1_value", "flow_mf_coef_p3_value",
+    "flow_mf_coef_p5_value", "flow_mf_coef_p1_values", "flow_mf_coef_p3_values",
+    "flow_mf_coef_p5_values", "use_flow_from_files", "flow_list_filename", "flow_root_dir",
+    "flow_num_method", "weno_eps", "upwind", "advance_diffusion", "diffusion_coef_filename",
+    "diffusion_coef_grid", "diffusion_coef_factor", "diffusion_num_method",
+    "diffusion_subcycles", "diffusion_coef_constant", "diffusion_coef_constants",
+    "assimilate_data", "assimilate_data_balance_flux", "assimilate_data_mult_fac",
+    "assimilate_data_map_list_filename", "assimilate_data_map_root_dir",
+    "assimilate_data_custom_from_mu", "assimilate_data_mu_power", "assimilate_data_mu_powers",
+    "assimilate_data_lat_limit", "assimilate_data_lat_limits", "assimilate_data_mu_limit",
+    "assimilate_data_mu_limits", "assimilate_data_add", "advance_source", "source_from_file",
+    "source_filename", "source_rfe_model", "source_rfe_total_unsigned_flux_per_hour",
+    "source_rfe_total_unsigned_flux_per_hours", "source_rfe_lifetime", "source_rfe_seed",
+)
+
 # Namelist array names whose per-realization lists the driver takes as-is.
 LIST_KEYS = {
     "diffusion_coef_constants", "flow_dr_coef_p0_values", "flow_dr_coef_p2_values",
@@ -96,10 +125,29 @@ def to_run_config(nml, base_dir="."):
     relative paths are taken from base_dir."""
     import os
     cfg = {}
+    unknown = []
     for k, v in nml.items():
+        m = re.fullmatch(r"([a-z_0-9]+)\(\s*(\d+)\s*\)", k)
+        if m:  # one array element, name(i) = value
+            name, i = m.group(1), int(m.group(2))
+            if name not in LIST_KEYS:
+                unknown.append(k)
+                continue
+            cur = cfg.get(name)
+            cur = list(cur) if isinstance(cur, list) else ([] if cur is None else [cur])
+            cur += [None] * (i - len(cur))
+            cur[i - 1] = v[0] if isinstance(v, list) else v
+            cfg[name] = cur
+            continue
+        if k not in REFERENCE_KEYS:
+            unknown.append(k)
+            continue
         if k in LIST_KEYS and not isinstance(v, list):
             v = [v]
         cfg[k] = v
+    if unknown:
+        raise ValueError(f"hipft.in: unknown namelist item(s) {', '.join(sorted(unknown))} "
+                         f"(not in &hipft_input_parameters, src/hipft.f90:8324-8406)")
     cfg["base_dir"] = base_dir  # flow and assimilation lists are read relative to it
     fname = cfg.get("initial_map_filename")
     if fname:

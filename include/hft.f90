@@ -188,6 +188,22 @@ module hft
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx, fval
     end function
+    integer(c_int) function hft_set_weno_eps(ctx, eps) bind(c, name="hft_set_weno_eps")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: eps
+    end function
+    integer(c_int) function hft_set_pole_flux_lat_limit(ctx, deg) &
+        bind(c, name="hft_set_pole_flux_lat_limit")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: deg
+    end function
+    ! out_pt = c_null_ptr balances the state in place
+    integer(c_int) function hft_balance_flux(ctx, out_pt) bind(c, name="hft_balance_flux")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, out_pt
+    end function
     integer(c_int) function hft_analysis(ctx, fval, fval_scale, out) bind(c, name="hft_analysis")
       import :: c_int, c_double, c_ptr
       type(c_ptr), value :: ctx, fval, fval_scale

@@ -129,6 +129,11 @@ int hft_update_flow(hft_ctx *ctx);
  * limits in dt_r[nr] (may be NULL); *dt = min over realizations. */
 int hft_flow_dtmax(hft_ctx *ctx, int strang, double *dt, double *dt_r);
 
+/* weno_eps (load_weno, src/hipft.f90:6895-6901): the smoothness-weight
+ * epsilon of WENO3; eps <= 0 (the reference's default 0) uses the cell
+ * widths dt(j) in theta and dp(k) in phi. */
+int hft_set_weno_eps(hft_ctx *ctx, double eps);
+
 /* ---- diffusion -----------------------------------------------------------
  * load_diffusion with constant per-realization coefficient eta[nr] (km^2/s),
  * factor = diffusion_coef_factor (7.43162901307967e-09 converts km^2/s to
@@ -188,6 +193,13 @@ int hft_source_step(hft_ctx *ctx, double dt);
 int hft_apply_diffusion_operator(hft_ctx *ctx, const double *x, double *y);
 int hft_apply_advection_operator(hft_ctx *ctx, const double *x, double *y);
 
+/* balance_flux (src/hipft.f90:8752-8801) of every realization: with
+ * out_pt == NULL the state itself (initial_map_flux_balance, 2163-2167);
+ * otherwise a balanced copy of the state is written to out_pt as a PT map
+ * (npm1,ntm,nr) and the state is left as it is (output_map_flux_balance,
+ * 2839-2843). */
+int hft_balance_flux(hft_ctx *ctx, double *out_pt);
+
 /* ---- data assimilation -----------------------------------------------------
  * apply_data_assimilation + assimilate_new_data, src/hipft.f90:4148-4306,
  * 4083-4146.  map: (npm1,ntm,3) PT slices data/weight/mu, one map shared by
@@ -209,6 +221,9 @@ int hft_assimilate_device(hft_ctx *ctx, const double *map_dev, int per_realizati
  * if fval_scale[nr] is non-NULL the field set by hft_set_validation_field is
  * used, multiplied per realization by fval_scale[r]. */
 int hft_set_validation_field(hft_ctx *ctx, const double *fval_pt);
+/* pole_flux_lat_limit (degrees, src/hipft.f90:566; default 30): the polar
+ * caps of the analysis' polar flux and area columns (3276-3285). */
+int hft_set_pole_flux_lat_limit(hft_ctx *ctx, double deg);
 int hft_analysis(hft_ctx *ctx, const double *fval_pt, const double *fval_scale, double *out);
 /* No counterpart in the reference.  For callers that cannot allocate device
  * memory (code under HIP graph capture, tight serving loops): the history

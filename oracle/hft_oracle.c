@@ -45,7 +45,6 @@ static const double rsun_cm2 = 4.84416e21;
 static const double km_s_to_rs_hr = 0.005172413793103448;
 static const double m_s_to_rs_hr = 5.172413793103448e-06;
 static const double safety = 0.95;
-static const double pole_flux_lat_limit = 30.0; /* src/hipft.f90:566 */
 
 struct orc {
   int ntm, npm, nt, np, nr, npm1;
@@ -67,6 +66,7 @@ struct orc {
   double dt_euler;
   double *dt_euler_r;
   int have_diffusion;
+  double pole_flux_lat_limit; /* degrees, src/hipft.f90:566 (default 30) */
 };
 
 #define F3(a, n1, n2, j, k, i) (a)[((j) - 1) + (size_t)(n1) * (((k) - 1) + (size_t)(n2) * ((i) - 1))]
@@ -190,6 +190,7 @@ orc *orc_create(int ntm, int npm1, const double *t, const double *p, int nr) {
   o->eps_p = alloc1(npm);
   for (int i = 1; i <= ntm; i++) o->eps_t[i] = o->dt[i];
   for (int i = 1; i <= npm; i++) o->eps_p[i] = o->dp[i];
+  o->pole_flux_lat_limit = 30.0;
   const double *d = o->dt;
   for (int j = 2; j <= nt - 2; j++) {
     double tot = d[j - 1] + d[j] + d[j + 1];
@@ -923,14 +924,14 @@ void orc_analysis(const orc *o, const double *fval_pt, double *out) {
           fp = fp + fv;
         else
           fm = fm + fv;
-        if (t[j] <= pole_flux_lat_limit * d2r) {
+        if (t[j] <= o->pole_flux_lat_limit * d2r) {
           if (fv > 0.)
             fppn = fppn + fv;
           else
             fmpn = fmpn + fv;
           apn = apn + da_t * da_p;
         }
-        if (t[j] >= pi - pole_flux_lat_limit * d2r) {
+        if (t[j] >= pi - o->pole_flux_lat_limit * d2r) {
           if (fv > 0.)
             fpps = fpps + fv;
           else
@@ -955,6 +956,16 @@ void orc_analysis(const orc *o, const double *fval_pt, double *out) {
     r[11] = sqrt(eqd1 * eqd1 + eqd2 * eqd2);
   }
 }
+
+/* ---- weno_eps (load_weno, src/hipft.f90:6895-6901): eps <= 0 gives the
+ * cell widths dt(j), dp(k); otherwise every smoothness weight uses eps. */
+void orc_set_weno_eps(orc *o, double eps) {
+  for (int i = 1; i <= o->ntm; i++) o->eps_t[i] = eps <= 0. ? o->dt[i] : eps;
+  for (int i = 1; i <= o->npm; i++) o->eps_p[i] = eps <= 0. ? o->dp[i] : eps;
+}
+
+/* pole_flux_lat_limit in degrees (analysis_step, src/hipft.f90:3276-3285). */
+void orc_set_pole_flux_lat_limit(orc *o, double deg) { o->pole_flux_lat_limit = deg; }
 
 /* ---- get_flux / balance_flux: src/hipft.f90:8672-8807 ------------------- */
 void orc_get_flux(const orc *o, const double *map, double *flux_p, double *flux_m) {

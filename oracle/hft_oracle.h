@@ -81,6 +81,8 @@ void orc_analysis(const orc *o, const double *fval_pt, double *out);
 void orc_assimilate(orc *o, const double *data, int custom_from_mu, const double *mu_power,
                     const double *mu_limit, const double *lat_limit, double mult_fac,
                     int add_mode, int balance);
+void orc_set_weno_eps(orc *o, double eps);
+void orc_set_pole_flux_lat_limit(orc *o, double deg);
 void orc_get_flux(const orc *o, const double *map /*(ntm,npm)*/, double *fp, double *fm);
 void orc_balance_flux(const orc *o, double *map);
 

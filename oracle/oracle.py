@@ -87,6 +87,8 @@ def lib():
                                      ctypes.c_double, ctypes.c_int, ctypes.c_int]
         L.orc_get_flux.argtypes = [ctypes.c_void_p, _D, _D, _D]
         L.orc_balance_flux.argtypes = [ctypes.c_void_p, _D]
+        L.orc_set_weno_eps.argtypes = [ctypes.c_void_p, ctypes.c_double]
+        L.orc_set_pole_flux_lat_limit.argtypes = [ctypes.c_void_p, ctypes.c_double]
         L.orc_rfe_frame.argtypes = [ctypes.c_void_p, _D, ctypes.c_int, _D, _D]
         L.orc_source_step.argtypes = [ctypes.c_void_p, ctypes.c_double, _D]
         L.orc_tesseral.argtypes = [ctypes.c_int, ctypes.c_int, _D, _D, ctypes.c_int,
@@ -294,6 +296,19 @@ class Oracle:
         a = np.ascontiguousarray(np.broadcast_to(src, (self.nr, self.npm, self.ntm)),
                                  dtype=np.float64)
         lib().orc_source_step(self._h, float(dt), _p(a))
+
+    def set_weno_eps(self, eps):
+        """weno_eps (load_weno 6895-6901): <= 0 gives the cell widths."""
+        lib().orc_set_weno_eps(self._h, float(eps))
+
+    def set_pole_flux_lat_limit(self, deg):
+        lib().orc_set_pole_flux_lat_limit(self._h, float(deg))
+
+    def balance_flux(self, map_tp):
+        """balance_flux (8752-8801) of one map (npm, ntm), returned."""
+        a = np.array(map_tp, dtype=np.float64, order="C")
+        lib().orc_balance_flux(self._h, _p(a))
+        return a
 
     def get_flux(self, map_tp):
         """(positive, negative) flux of one map (npm, ntm), get_flux order."""

@@ -12,6 +12,7 @@ the system HDF5 C library (``libhdf5.so``) with ``ctypes``.
 Run once in a container where the reference tree is mounted:
 
     python tests/golden/make_fixtures.py /root/reference
+    python tests/golden/make_fixtures.py /root/reference --run-parameters
 """
 import ctypes
 import os
@@ -126,5 +127,23 @@ def main(ref_root):
         print(case, {k: v.shape for k, v in arrays.items()}, os.path.getsize(out))
 
 
+def run_parameters(ref_root, cases=("diffuse_soccer", "diffuse_advect_atten_map_1cr")):
+    """hipft_run_parameters_used.out of two cases (the namelist values a run
+    used, as the reference wrote them) into run_parameters_used.npz."""
+    arrays = {}
+    for case in cases:
+        path = os.path.join(ref_root, "testsuite", case, "reference",
+                            "hipft_run_parameters_used.out")
+        with open(path) as fh:
+            arrays[case] = np.array(fh.read())
+    out = os.path.join(HERE, "run_parameters_used.npz")
+    np.savez_compressed(out, **arrays)
+    print("run_parameters_used", list(arrays), os.path.getsize(out))
+
+
 if __name__ == "__main__":
-    main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")
+    root = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
+    if len(sys.argv) > 2 and sys.argv[2] == "--run-parameters":
+        run_parameters(root)
+    else:
+        main(root)

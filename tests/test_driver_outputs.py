@@ -13,8 +13,14 @@
 """
 import numpy as np
 
+import os
+
 from hipft_amd import driver, namelist
-from conftest import golden
+from conftest import GOLDEN, golden
+
+
+def golden_path(name):
+    return os.path.join(GOLDEN, name)
 
 
 def _read_table(text):
@@ -68,3 +74,71 @@ def test_fortran_e_format():
     assert driver._fortran_e(-262.0, 40, 16) == "-2.6200000000000000E+002".rjust(40)
     assert driver._fortran_e(1.0e-300, 23, 15) == "1.000000000000000E-300".rjust(23)
     assert driver._fortran_e(0.1, 40, 16) == "1.0000000000000001E-001".rjust(40)
+
+
+def test_run_parameters_used_matches_reference(tmp_path):
+    """hipft_run_parameters_used.out against the files two testsuite cases
+    hold: every item the reference wrote has the same value here.  Those
+    files come from an earlier release that wrote the per-realization arrays
+    before filling them (sentinels -1 and -1e6); the current source fills
+    unset entries with the scalar first (process_input_parameters,
+    src/hipft.f90:8542-8579, before the write at 8450-8454), so array entries
+    are compared through that rule."""
+    from hipft_amd.namelist import REFERENCE_KEYS
+    fx = np.load(golden_path("run_parameters_used.npz"))
+    for case in ("diffuse_soccer", "diffuse_advect_atten_map_1cr"):
+        nml = namelist.parse(str(golden(case)["hipft_in"]))
+        path = driver.write_run_parameters(nml, str(tmp_path))
+        ours = namelist.parse(open(path).read())
+        assert list(ours) == list(REFERENCE_KEYS)
+        ref = namelist.parse(str(fx[case]))
+        for k, rv in ref.items():
+            ov = ours[k]
+            if k in driver.ARRAY_DEFAULTS:
+                low = -100000.0 if k.startswith(("flow_dr", "flow_mf")) else 0.0
+                scalar = ref[driver.ARRAY_DEFAULTS[k]]
+                want = [v if v >= low else scalar for v in rv]
+                assert len(ov) == len(rv) == driver.MAX_REALIZATIONS, k
+                np.testing.assert_array_equal(ov, want, err_msg=k)
+            elif isinstance(rv, float):
+                assert ov == rv, (k, ov, rv)
+            else:
+                assert ov == rv, (k, ov, rv)
+
+
+def test_namelist_refuses_unknown_and_unsupported_items():
+    """An item outside &hipft_input_parameters is an error (as for a Fortran
+    namelist read); reference options this build does not implement are
+    refused instead of silently running another method."""
+    import pytest
+    base = "&hipft_input_parameters\n res_nt=31\n res_np=61\n time_end=2.0\n"
+    with pytest.raises(ValueError, match="unknown namelist item"):
+        namelist.to_run_config(namelist.parse(base + " no_such_item = 3\n/\n"))
+    for item, msg in (("diffusion_num_method = 1", "diffusion_num_method"),
+                      ("flow_num_method = 3", "flow_num_method"),
+                      ("upwind = 0.5", "upwind"), ("flow_rigid_omega = 0.2", "flow_rigid_omega"),
+                      ("diffusion_coef_filename = 'eta.h5'", "diffusion_coef_filename"),
+                      ("diffusion_coef_grid = .true.", "diffusion_coef_grid"),
+                      ("output_flows = .true.", "output_flows"),
+                      ("output_map_time_cadence = 1.0\n output_map_idx_cadence = 2",
+                       "cannot both")):
+        cfg = namelist.to_run_config(namelist.parse(base + f" {item}\n/\n"))
+        with pytest.raises(ValueError, match=msg):
+            driver.check_config(dict(driver.DEFAULTS, **cfg))
+    # supported values pass
+    cfg = namelist.to_run_config(namelist.parse(
+        base + " diffusion_num_method = 3\n flow_num_method = 4\n upwind = 1.0\n/\n"))
+    driver.check_config(dict(driver.DEFAULTS, **cfg))
+
+
+def test_per_realization_arrays_follow_the_sentinel_rule():
+    """diffusion_coef_constants(2) = 300 sets one element; entries left at
+    the sentinel (< 0; < -1e5 for flow coefficients) take the scalar."""
+    text = ("&hipft_input_parameters\n n_realizations = 4\n diffusion_coef_constant = 50.\n"
+            " diffusion_coef_constants(2) = 300.\n flow_dr_coef_p0_values = 10., -1000000., 30.\n"
+            " flow_dr_coef_p0_value = 7.\n/\n")
+    cfg = namelist.to_run_config(namelist.parse(text))
+    np.testing.assert_array_equal(driver.per_realization(cfg, "diffusion_coef_constant", 4),
+                                  [50.0, 300.0, 50.0, 50.0])
+    np.testing.assert_array_equal(driver.per_realization(cfg, "flow_dr_coef_p0_value", 4),
+                                  [10.0, 7.0, 30.0, 7.0])
