@@ -201,6 +201,84 @@ def adjoint_bench(nr=16, ntm=256, npm1=512, nsteps=200, every=20, seed=77):
                                 v0=float(v.grad.norm()), omega=float(o.grad.norm())))
 
 
+def cfg5_bench(days=30, nr=128, ntm=512, npm1=1024, ckpt=24, adjoint_mem_gb=0):
+    """Batched gradient assimilation at its workload: 512x1024, 128
+    realizations, a window of `days` daily observations (truth run with eta =
+    350 km^2/s, 5 G noise, mu^4 weights within 60 degrees of disk centre),
+    Adam on eta, v0 and the initial maps.  One warm-up iteration on a one-day
+    window (allocations), then one timed iteration of the full window: the
+    forward pass with checkpoints, the adjoint pass and the optimiser step.
+    Peak device memory = memory held after the adjoint pass (the library's
+    buffers stay allocated) plus torch's peak above its current use."""
+    import torch
+    from hipft_amd.variational import synthetic_window, window_misfit
+    torch.cuda.init()
+    free_start, total = torch.cuda.mem_get_info()
+    t0 = time.perf_counter()
+    win = synthetic_window(ntm, npm1, nr, days)
+    setup_s = time.perf_counter() - t0
+    print(f"cfg5: window of {days} observations ready ({setup_s:.1f} s)", file=sys.stderr,
+          flush=True)
+    model, obs, wts, spo = win["model"], win["obs"], win["weights"], win["steps_per_obs"]
+    if adjoint_mem_gb > 0:  # device memory the adjoint may use for stored RKG2 stages
+        model.ft.set_tuning(adjoint_mem_mb=int(adjoint_mem_gb * 1024))
+
+    def progress(msg):
+        print(f"cfg5: {msg} ({time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
+
+    le = torch.log(win["eta0"]).requires_grad_(True)
+    lv = torch.log(win["v0_0"]).requires_grad_(True)
+    om = win["omega0"]
+    x0 = win["f0"].clone().requires_grad_(True)
+    opt = torch.optim.Adam([{"params": [le, lv], "lr": 0.05}, {"params": [x0], "lr": 1.0}])
+
+    def iteration(o, w):
+        opt.zero_grad()
+        torch.cuda.synchronize()
+        a = time.perf_counter()
+        J = window_misfit(model, x0, torch.exp(le), torch.exp(lv), om, o, w, spo,
+                          checkpoint_every=ckpt, log=progress)
+        total_J = J.sum()
+        torch.cuda.synchronize()
+        b = time.perf_counter()
+        print(f"cfg5: forward over {len(o)} observations {b - a:.1f} s", file=sys.stderr,
+              flush=True)
+        total_J.backward()
+        torch.cuda.synchronize()
+        c = time.perf_counter()
+        opt.step()
+        torch.cuda.synchronize()
+        return float(total_J.detach()), b - a, c - b, time.perf_counter() - c
+
+    iteration(obs[:1], wts[:1])  # warm-up: allocations on a one-day window
+    torch.cuda.reset_peak_memory_stats()
+    J0, fwd, bwd, stp = iteration(obs, wts)
+    torch.cuda.synchronize()
+    held = free_start - torch.cuda.mem_get_info()[0]
+    peak = held + torch.cuda.max_memory_allocated() - torch.cuda.memory_allocated()
+    with torch.no_grad():
+        J1 = float(window_misfit(model, x0, torch.exp(le), torch.exp(lv), om, obs, wts,
+                                 spo, checkpoint_every=ckpt).sum())
+    s_r = model.ft.diffusion_stats()[0]
+    model_mem_mb = model.ft.get_tuning("adjoint_mem_mb")
+    model.ft.close()
+    it_s = fwd + bwd + stp
+    steps = days * spo
+    return dict(config=f"{ntm}x{npm1} clustered x2, {nr} realizations, {days} daily mu^4-"
+                       f"weighted observations (5 G noise) of a truth run with eta = 350, Adam "
+                       f"on eta, v0 and the initial maps, dt = 1 h, {steps} steps, checkpoints "
+                       f"every {ckpt} steps, adjoint stage memory "
+                       f"{model_mem_mb / 1024:.0f} GiB, RKG2 stages "
+                       f"{int(s_r.min())}-{int(s_r.max())} (last step)",
+                setup_s=round(setup_s, 2), forward_s=round(fwd, 3), adjoint_s=round(bwd, 3),
+                optimizer_s=round(stp, 4), iteration_s=round(it_s, 3),
+                adjoint_over_forward=round(bwd / fwd, 3),
+                forward_realization_steps_per_s=round(nr * steps / fwd, 1),
+                iterations_50_s=round(50 * it_s, 1),
+                misfit_before=J0, misfit_after_one_step=J1,
+                peak_device_GiB=round(peak / 2 ** 30, 2), device_GiB=round(total / 2 ** 30, 1))
+
+
 def pmc_probe(m, nbytes=256 << 20):
     """Counter calibration copy of a known size, then one workload step."""
     import ctypes
@@ -248,6 +326,13 @@ def main():
     ap.add_argument("--adjoint-only", action="store_true",
                     help="only the adjoint cost measurement (for profiling)")
     ap.add_argument("--retune", action="store_true", help="ignore the cached launch tuning")
+    ap.add_argument("--cfg5", action="store_true",
+                    help="only the batched gradient-assimilation measurement (512x1024, 128 "
+                         "realizations, 30-day window; one Adam iteration, several minutes)")
+    ap.add_argument("--cfg5-days", type=int, default=30)
+    ap.add_argument("--cfg5-adjoint-mem-gb", type=float, default=210.0,
+                    help="device memory for stored RKG2 stages in the cfg5 adjoint (0: the "
+                         "library default, 32 GiB)")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="for rocprofv3 --pmc runs: a 256 MiB device-to-device copy (counter "
                          "calibration) followed by one step of the workload")
@@ -264,6 +349,10 @@ def main():
         torch.cuda.init()
     if args.adjoint_only:
         print(json.dumps({"adjoint": adjoint_bench()}), flush=True)
+        return
+    if args.cfg5:
+        print(json.dumps({"gradient_assimilation": cfg5_bench(days=args.cfg5_days,
+                                                              adjoint_mem_gb=args.cfg5_adjoint_mem_gb)}), flush=True)
         return
     nr = args.nr
     wl = make_workload(nr)

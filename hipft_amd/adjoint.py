@@ -57,7 +57,9 @@ class FlowModel:
                 ft.set_flow_analytic()
         if self.autotune and self._params is None:
             from .tune import autotune
-            autotune(ft)  # launch shapes for this grid and batch (cached)
+            # launch shapes for this grid and batch (cached; a batch size not
+            # measured yet borrows the nearest one's)
+            autotune(ft, borrow=True)
         self._params = key
 
     def bind_stream(self):

@@ -17,6 +17,8 @@ import json
 import os
 import time
 
+import numpy as np
+
 HERE = os.path.dirname(os.path.abspath(__file__))
 CACHE = os.path.join(HERE, "tuning_cache.json")
 
@@ -79,13 +81,35 @@ def apply(model, cfg):
     model.set_tuning(**{k: cfg.get(k, DEFAULTS[k]) for k in KEYS})
 
 
-def autotune(model, dt=1.0, reps=3, use_cache=True, write_cache=True, cands=None, log=None):
-    """Pick and apply the fastest launch configuration for this batch."""
+def nearest_cached(model, cache):
+    """The cached choice for the same grid whose batch size is nearest (in
+    ratio) to this one's, or None."""
+    grid = f"gfx950|{model.ntm}x{model.npm1}|nr="
+    best, best_d = None, None
+    for k, v in cache.items():
+        if k.startswith(grid):
+            d = abs(np.log(int(k[len(grid):]) / model.nr))
+            if best_d is None or d < best_d:
+                best, best_d = v, d
+    return best
+
+
+def autotune(model, dt=1.0, reps=3, use_cache=True, write_cache=True, cands=None, log=None,
+             borrow=False):
+    """Pick and apply the fastest launch configuration for this batch.
+    borrow: with no cached choice for this batch size, apply the one cached
+    for the nearest batch size on the same grid instead of measuring
+    (results do not depend on the choice, only the speed does)."""
     key = cache_key(model)
     cache = load_cache()
     if use_cache and key in cache:
         apply(model, cache[key])
         return cache[key]
+    if use_cache and borrow:
+        near = nearest_cached(model, cache)
+        if near is not None:
+            apply(model, near)
+            return dict(near, borrowed=True)
     state = model.get_state("tp")
 
     def measure(cands, run, what):
