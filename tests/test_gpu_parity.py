@@ -424,11 +424,12 @@ def test_diffuse_dipole_vs_oracle_and_reference(gpu_available):
 
     Against the oracle (tests/golden/oracle_diffuse_dipole.npz): cycle times,
     stage counts and cycle counts identical, histories to summation order.
-    Against the reference fixture: stage and cycle counts identical; from the
-    4th PTL cycle on the cycle length differs by ~1.6e-8 relative, because
-    the PTL step is taken at the points where |L f| equals its maximum
-    exactly (src/hipft.f90:5116) and that tie-break follows the last bit of
-    the reference build's rounding.
+    Against the reference fixture: stage and cycle counts identical, the
+    first three cycle times to 5e-12; from the 4th PTL cycle on the cycle
+    length differs by ~2e-8 relative because that cycle's PTL step amplifies
+    last-bit differences of the state ~1e4-fold (one unit in the last place
+    of the input map moves it by 2-4e-8; test_diffuse_dipole_ptl_conditioning
+    shows this on the CPU), so the remaining rows are held to 1e-7.
     """
     case = "diffuse_dipole"
     sol, num, R = driver.run(case_cfg(case))
@@ -438,7 +439,11 @@ def test_diffuse_dipole_vs_oracle_and_reference(gpu_available):
     assert err <= 1e-13, cols
     ref = golden(case)
     assert np.array_equal(num[..., 6:8], ref["num"][..., 6:8])
+    # rows: initial, cycles 1..7, final; cycles 1-3 before the ill-conditioned 4th
+    assert np.allclose(num[:, 1:4, 1], ref["num"][:, 1:4, 1], rtol=5e-12, atol=0)
     assert np.allclose(num[..., 1], ref["num"][..., 1], rtol=1e-7)
+    err, _, cols = history_error(sol[:, :4], ref["sol"][:, :4])
+    assert err <= 1e-11, cols
     err, _, cols = history_error(sol, ref["sol"])
     assert err <= 1e-7, cols
 
