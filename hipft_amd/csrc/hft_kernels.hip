@@ -1037,30 +1037,33 @@ __device__ __forceinline__ void fused2_step(const FusedArgs &a, Fused2State<T> &
   lds_barrier();
 }
 
+// One (realization, band) tile of k_rkg2_fused2<T>: stages k0 .. k0+T-1 of
+// realization a.order[z] over theta rows [band*H, band*H + H), reading
+// u_{k0-1} from Ain and u_{k0-2} from Bin, writing outA/outB (the other
+// fields and tables from a).  lds, cfs and rowc are the block's shared
+// arrays; the tile begins after a barrier that ends every earlier use.
 template <int T>
-__global__ __launch_bounds__(512) void k_rkg2_fused2(FusedArgs a) {
-  extern __shared__ double lds[];  // [T + 1][3][2 * blockDim]: stage levels 0..T-1, then u0
-  __shared__ double cfs[T][5];
-  __shared__ double rowc[4 * kFusedMaxRows];
+__device__ __forceinline__ void fused2_tile(const FusedArgs &a, int z, int band, int Hb, int k0,
+                                            const double *Ain, const double *Bin, double *OAin,
+                                            double *OBin, double *lds, double (*cfs)[5],
+                                            double *rowc) {
   const Mesh &m = a.m;
   const int nu = m.nu, ntm = m.ntm;
   const int nrow = 2 * (int)blockDim.x;
-  int L = xcd_remap(blockIdx.x, gridDim.x);
-  int z = L / a.nbands, band = L - z * a.nbands;
   int r = a.order[z];
   int c0 = 2 * (int)threadIdx.x;
   int cl = min(c0, m.pitch - 2);  // column pair of the loads, inside the row
   bool col1 = c0 + 1 < nu;
   int e0 = c0 < nu ? c0 : 0, e1 = col1 ? c0 + 1 : 0;
   int cw0 = wrapc(e0 - 1, nu), ce0 = wrapc(e0 + 1, nu), ce1 = wrapc(e1 + 1, nu);
-  int j0 = band * a.H, j1 = min(ntm, j0 + a.H);
+  int j0 = band * Hb, j1 = min(ntm, j0 + Hb);
   int lo0 = max(0, j0 - T), hi0 = min(ntm, j1 + T);
   int64_t sr = a.s[r];
-  int nst = (int)((sr - a.k0 + 1) < T ? (sr - a.k0 + 1) : T);
-  bool final_here = (a.k0 + nst - 1) == sr;
+  int nst = (int)((sr - k0 + 1) < T ? (sr - k0 + 1) : T);
+  bool final_here = (k0 + nst - 1) == sr;
   size_t base = (size_t)r * a.rstride;
-  const double *A = a.A + base, *B = a.B + base, *U0 = a.f + base;
-  double *OA = a.outA + base, *OB = a.outB + base;
+  const double *A = Ain + base, *B = Bin + base, *U0 = a.f + base;
+  double *OA = OAin + base, *OB = OBin + base;
   double *Wst[T];
 #pragma unroll
   for (int t = 0; t < T; t++) Wst[t] = a.W[t] ? a.W[t] + base : nullptr;
@@ -1076,7 +1079,7 @@ __global__ __launch_bounds__(512) void k_rkg2_fused2(FusedArgs a) {
     int t = threadIdx.x / 5, e = threadIdx.x - 5 * t;
     double v = 0.0;
     if (t < nst) {
-      const StageCoef &k = a.coef[(size_t)(a.k0 + t) * a.nr_table + r];
+      const StageCoef &k = a.coef[(size_t)(k0 + t) * a.nr_table + r];
       v = e == 0 ? k.a : (e == 1 ? k.b : (e == 2 ? k.c : (e == 3 ? k.d : k.g)));
     }
     cfs[t][e] = v;
@@ -1149,6 +1152,123 @@ __global__ __launch_bounds__(512) void k_rkg2_fused2(FusedArgs a) {
       fused2_step<T, 3>(a, S, lds, rowc, i + 3, nsteps, lo0, hi0, j0, j1, nst, final_here, c0,
                         cl, cw0, ce0, ce1, col1, nrow, D2, P1, P5, A, B, U0, dtr, OA, OB, u0_pole, dt_pole,
                         cfs, Wst);
+  }
+}
+
+template <int T>
+__global__ __launch_bounds__(512) void k_rkg2_fused2(FusedArgs a) {
+  extern __shared__ double lds[];  // [T + 1][3][2 * blockDim]: stage levels 0..T-1, then u0
+  __shared__ double cfs[T][5];
+  __shared__ double rowc[4 * kFusedMaxRows];
+  int L = xcd_remap(blockIdx.x, gridDim.x);
+  int z = L / a.nbands, band = L - z * a.nbands;
+  fused2_tile<T>(a, z, band, a.H, a.k0, a.A, a.B, a.outA, a.outB, lds, cfs, rowc);
+}
+
+// ---------------------------------------------------------------------------
+// Persistent RKG2 super-step (k_rkg2_persist<T>): every fused tile of one
+// super-step -- realization z (position in the stage-count order), band b,
+// stage group g = stages 2+gT .. 2+gT+T-1 (clipped at s_z) -- in ONE launch.
+// Blocks take tiles from a list ordered group by group (longest realizations
+// first); a tile of group g waits until group g-1 of its band and of the
+// bands either side is done (the rows it reads as halo and the buffer it
+// overwrites).  Every tile a block waits for precedes its own in the list
+// and was taken by a running block, so the wait always ends; a bound on the
+// spins turns a fault of this argument into an error flag instead of a hang.
+// Stage levels rotate through four buffers: group 0 reads u_1 (P1) and u0,
+// writes P2/P3; odd groups read P2/P3 and write P0/P1; even groups read
+// P0/P1 and write P2/P3.
+// ---------------------------------------------------------------------------
+struct PersistArgs {
+  FusedArgs fa;           // mesh, u0 (f), D, coef, s, order, rstride, cdt; H per tile
+  double *P[4];
+  const unsigned *tiles;  // packed: z | band << 12 | g << 20
+  const int *zH;          // [n] band height of realization z
+  const int *znb;         // [n] band count of realization z
+  const int *zoff;        // [n] offset of z's counters in done
+  int ntiles;
+  int *counter;           // next tile
+  int *done;              // groups completed, per (z, band)
+  int *err;               // set when a wait outlasts wait_ticks
+  long long wait_ticks;   // bound of one wait, in wall_clock64() ticks
+};
+
+// Current value of a counter other blocks update with atomics: read with an
+// atomic read-modify-write, which is performed where the atomics are (a
+// plain or relaxed atomic load may keep hitting a stale copy in this XCD's
+// L2 while another XCD's updates go elsewhere).
+__device__ __forceinline__ int poll_int(int *p) {
+  return __hip_atomic_fetch_add(p, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+constexpr int kPersistZBits = 12, kPersistBBits = 8;
+
+__global__ void k_zero_ints(int *p, int n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
+}
+
+template <int T>
+__global__ __launch_bounds__(512) void k_rkg2_persist(PersistArgs p) {
+  extern __shared__ double lds[];
+  __shared__ double cfs[T][5];
+  __shared__ double rowc[4 * kFusedMaxRows];
+  __shared__ int sh_tile, sh_stop;
+  const FusedArgs &a = p.fa;
+  for (;;) {
+    if (threadIdx.x == 0) {
+      int stop = 0, tau = 0;
+      if (poll_int(p.err)) {
+        stop = 1;
+      } else {
+        tau = atomicAdd(p.counter, 1);
+        if (tau >= p.ntiles) {
+          stop = 1;
+        } else {
+          unsigned tl = p.tiles[tau];
+          int z = (int)(tl & ((1u << kPersistZBits) - 1));
+          int b = (int)((tl >> kPersistZBits) & ((1u << kPersistBBits) - 1));
+          int g = (int)(tl >> (kPersistZBits + kPersistBBits));
+          if (g > 0) {
+            int *dz = p.done + p.zoff[z];
+            int nb = p.znb[z], blo = b > 0 ? b - 1 : 0, bhi = b + 1 < nb ? b + 1 : nb - 1;
+            long long t0 = wall_clock64();
+            for (;;) {
+              bool ok = true;
+              for (int bb = blo; bb <= bhi; bb++)
+                if (poll_int(dz + bb) < g) ok = false;
+              if (ok) break;
+              if (wall_clock64() - t0 > p.wait_ticks || poll_int(p.err)) {
+                __hip_atomic_fetch_or(p.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                stop = 1;
+                break;
+              }
+              __builtin_amdgcn_s_sleep(2);
+            }
+          }
+        }
+      }
+      sh_tile = tau;
+      sh_stop = stop;
+      // acquire: the rows the tile reads were released by the tiles waited for
+      __atomic_thread_fence(__ATOMIC_ACQUIRE);
+    }
+    __syncthreads();
+    if (sh_stop) return;
+    unsigned tl = p.tiles[sh_tile];
+    int z = (int)(tl & ((1u << kPersistZBits) - 1));
+    int b = (int)((tl >> kPersistZBits) & ((1u << kPersistBBits) - 1));
+    int g = (int)(tl >> (kPersistZBits + kPersistBBits));
+    // group 0 reads u_1 (P1) and u0; odd groups P2/P3 -> P0/P1; even P0/P1 -> P2/P3
+    const double *Ain = g == 0 ? p.P[1] : ((g & 1) ? p.P[2] : p.P[0]);
+    const double *Bin = g == 0 ? a.f : ((g & 1) ? p.P[3] : p.P[1]);
+    double *OAo = (g & 1) ? p.P[0] : p.P[2];
+    double *OBo = (g & 1) ? p.P[1] : p.P[3];
+    fused2_tile<T>(a, z, b, p.zH[z], 2 + g * T, Ain, Bin, OAo, OBo, lds, cfs, rowc);
+    // release: this thread's stores of the tile, then the block's count
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0)
+      __hip_atomic_fetch_add(p.done + p.zoff[z] + b, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 

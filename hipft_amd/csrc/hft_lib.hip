@@ -126,6 +126,18 @@ struct hft_ctx {
   int fused_cap[6] = {0, 0, 0, 0, 0, 0};  // co-resident k_rkg2_fused2<T> blocks on the device
   int fused_kernel = 3;  // 1: k_rkg2_fused, 2: k_rkg2_march, 3: k_rkg2_fused2 (2 columns/thread)
   int adv_band = 1, adv_band_rows = 16;  // banded WENO3 stage kernel
+  // persistent super-step (k_rkg2_persist): 0 = off, 1 = on; band rows of
+  // the realizations with the most stages and of the others
+  int persist = 0, persist_rows = 32, persist_rows_long = 32, persist_long = 0;
+  unsigned *d_ptiles = nullptr, *h_ptiles = nullptr;
+  size_t ptiles_cap = 0;
+  int *d_pint = nullptr, *h_pint = nullptr;  // zH, znb, zoff (device) / staging (pinned)
+  size_t pint_cap = 0;
+  int *d_pctl = nullptr;  // counter, err, done[...]
+  size_t pctl_cap = 0;
+  hipEvent_t ptiles_ev = nullptr;
+  int persist_cap = 0;
+  int wall_khz = 0;  // wall_clock64() rate of the device
   // independent mode: per-realization CFL sub-steps of the advection step
   int adv_substeps = 1;
   int64_t flow_gen = 0, adv_lim_gen = -1;
@@ -833,7 +845,7 @@ int rkg2_superstep(hft_ctx *c, const std::vector<int> &reals, const std::vector<
   std::vector<int> ord;
   int rc = upload_rkg2(c, reals, dt2, s, ord);
   if (rc) return rc;
-  if (fused_usable(c) && !c->buf2) {
+  if ((fused_usable(c) || c->persist) && !c->buf2) {
     if ((rc = dalloc(&c->buf2, c->nfield)) || (rc = dalloc(&c->buf3, c->nfield))) return rc;
   }
   // the launch sequence depends only on the stage counts in launch order
@@ -997,6 +1009,166 @@ int launch_fused(hft_ctx *c, int64_t k, int active, const std::vector<int> &ord,
 
 int copy_finals(hft_ctx *c, const std::vector<double *> &final_loc);
 
+bool persist_usable(const hft_ctx *c, int64_t smax) {
+  int T = c->fused_T;
+  int64_t groups = (smax - 1 + T - 1) / T;
+  return c->persist && c->fused && c->fused_kernel == 3 && c->nu <= 1024 && T >= 2 && T <= 4 &&
+         c->nr < (1 << kPersistZBits) && groups < (1 << (32 - kPersistZBits - kPersistBBits)) &&
+         c->persist_rows >= 1 && c->persist_rows + 2 * T <= kFusedMaxRows &&
+         c->persist_rows_long >= 1 && c->persist_rows_long + 2 * T <= kFusedMaxRows;
+}
+
+// Stages 2..s of a super-step in one persistent launch (k_rkg2_persist):
+// the tile list (group-major, realizations in stage-count order, each
+// realization's bands), its per-realization band shapes and cleared
+// counters, then the launch; returns where each realization's last stage is.
+int superstep_persistent(hft_ctx *c, int n, const std::vector<int> &ord,
+                         const std::vector<int64_t> &s, std::vector<double *> &final_loc) {
+  int T = c->fused_T, ntm = c->ntm;
+  std::vector<int> G(n), H(n), nb(n), off(n + 1, 0);
+  int gmax = 0;
+  for (int z = 0; z < n; z++) {
+    int64_t sz = s[ord[z]];
+    G[z] = sz >= 2 ? (int)((sz - 1 + T - 1) / T) : 0;
+    gmax = std::max(gmax, G[z]);
+    H[z] = z < c->persist_long ? c->persist_rows_long : c->persist_rows;
+    H[z] = std::min(H[z], ntm);
+    nb[z] = (ntm + H[z] - 1) / H[z];
+    if (nb[z] >= (1 << kPersistBBits)) return fail(HFT_E_ARG, "persistent super-step: %d bands", nb[z]);
+    off[z + 1] = off[z] + nb[z];
+  }
+  size_t ntiles = 0;
+  for (int z = 0; z < n; z++) ntiles += (size_t)G[z] * nb[z];
+  if (ntiles == 0) return 0;
+  if (ntiles > (size_t)INT32_MAX) return fail(HFT_E_ARG, "persistent super-step: too many tiles");
+  if (!c->ptiles_ev) HIPCHK(hipEventCreateWithFlags(&c->ptiles_ev, hipEventDisableTiming));
+  HIPCHK(hipEventSynchronize(c->ptiles_ev));  // the previous upload has been read
+  if (ntiles > c->ptiles_cap) {
+    if (c->d_ptiles) HIPCHK(hipFree(c->d_ptiles));
+    if (c->h_ptiles) HIPCHK(hipHostFree(c->h_ptiles));
+    c->d_ptiles = nullptr;
+    c->h_ptiles = nullptr;
+    size_t cap = std::max(ntiles, (size_t)4096);
+    int rc = dalloc(&c->d_ptiles, cap);
+    if (rc) return rc;
+    HIPCHK(hipHostMalloc((void **)&c->h_ptiles, cap * sizeof(unsigned)));
+    c->ptiles_cap = cap;
+  }
+  size_t nint = 3 * (size_t)n;
+  if (nint > c->pint_cap) {
+    if (c->d_pint) HIPCHK(hipFree(c->d_pint));
+    if (c->h_pint) HIPCHK(hipHostFree(c->h_pint));
+    c->d_pint = nullptr;
+    c->h_pint = nullptr;
+    int rc = dalloc(&c->d_pint, nint);
+    if (rc) return rc;
+    HIPCHK(hipHostMalloc((void **)&c->h_pint, nint * sizeof(int)));
+    c->pint_cap = nint;
+  }
+  size_t nctl = 2 + (size_t)off[n];
+  if (nctl > c->pctl_cap) {
+    if (c->d_pctl) HIPCHK(hipFree(c->d_pctl));
+    c->d_pctl = nullptr;
+    int rc = dalloc(&c->d_pctl, nctl);
+    if (rc) return rc;
+    HIPCHK(hipMemset(c->d_pctl, 0, nctl * sizeof(int)));
+    c->pctl_cap = nctl;
+  }
+  size_t q = 0;
+  for (int g = 0; g < gmax; g++)
+    for (int z = 0; z < n; z++)
+      if (G[z] > g)
+        for (int b = 0; b < nb[z]; b++)
+          c->h_ptiles[q++] = (unsigned)z | ((unsigned)b << kPersistZBits) |
+                             ((unsigned)g << (kPersistZBits + kPersistBBits));
+  for (int z = 0; z < n; z++) {
+    c->h_pint[z] = H[z];
+    c->h_pint[n + z] = nb[z];
+    c->h_pint[2 * n + z] = off[z];
+  }
+  HIPCHK(hipMemcpyAsync(c->d_ptiles, c->h_ptiles, ntiles * sizeof(unsigned), hipMemcpyHostToDevice,
+                        c->stream));
+  HIPCHK(hipMemcpyAsync(c->d_pint, c->h_pint, nint * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipEventRecord(c->ptiles_ev, c->stream));
+  // counter and done counters start at zero; the error flag is kept (read
+  // and cleared by the diffusion step)
+  int *counter = c->d_pctl, *err = c->d_pctl + 1, *done = c->d_pctl + 2;
+  hipLaunchKernelGGL(k_zero_ints, dim3(std::max(1, std::min(64, (off[n] + 256) / 256))), dim3(256), 0,
+                     c->stream, counter, 1);
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(k_zero_ints, dim3(std::max(1, std::min(64, (off[n] + 255) / 256))), dim3(256), 0,
+                     c->stream, done, off[n]);
+  HIPCHK(hipGetLastError());
+  PersistArgs pa;
+  FusedArgs &fa = pa.fa;
+  fa.m = c->m;
+  fa.ncol = ((c->nu + 63) / 64) * 64;
+  fa.nbands = 0;
+  fa.H = 0;
+  fa.k0 = 2;
+  fa.A = fa.B = nullptr;
+  fa.outA = fa.outB = nullptr;
+  fa.f = c->f; fa.dty0 = c->dty0; fa.D = c->d_D; fa.coef = c->d_coef; fa.nr_table = c->nr;
+  fa.s = c->d_s; fa.order = c->d_order; fa.rstride = c->rstride; fa.cdt = c->d_cdt;
+  for (int t = 0; t < 5; t++) fa.W[t] = nullptr;
+  pa.P[0] = c->buf0; pa.P[1] = c->buf1; pa.P[2] = c->buf2; pa.P[3] = c->buf3;
+  pa.tiles = c->d_ptiles;
+  pa.zH = c->d_pint; pa.znb = c->d_pint + n; pa.zoff = c->d_pint + 2 * n;
+  pa.ntiles = (int)ntiles;
+  pa.counter = counter; pa.err = err; pa.done = done;
+  if (c->wall_khz == 0) {
+    int khz = 0;
+    HIPCHK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c->device));
+    c->wall_khz = khz > 0 ? khz : 100000;
+  }
+  pa.wait_ticks = 2000LL * c->wall_khz;  // 2 s: far above any real wait
+  int threads = ((fa.ncol / 2 + 63) / 64) * 64;
+  size_t sh = (size_t)(T + 1) * 3 * 2 * threads * sizeof(double);
+  if (c->persist_cap == 0) {
+    int per = 0, cus = 0;
+    switch (T) {
+      case 2: HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_rkg2_persist<2>, threads, sh)); break;
+      case 3: HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_rkg2_persist<3>, threads, sh)); break;
+      default: HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_rkg2_persist<4>, threads, sh)); break;
+    }
+    HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
+    c->persist_cap = std::max(1, per) * std::max(1, cus);
+  }
+  int blocks = (int)std::min<size_t>(ntiles, (size_t)c->persist_cap);
+  double units = 0, sunits = 0;
+  for (int z = 0; z < n; z++) {
+    units += 5.0 * G[z];
+    sunits += 5.0 * (double)(s[ord[z]] - 1);
+  }
+  int rc;
+  switch (T) {
+    case 2: rc = launch_ex(c, kGroupFused, k_rkg2_persist<2>, blocks, threads, sh, pa, units, sunits); break;
+    case 3: rc = launch_ex(c, kGroupFused, k_rkg2_persist<3>, blocks, threads, sh, pa, units, sunits); break;
+    default: rc = launch_ex(c, kGroupFused, k_rkg2_persist<4>, blocks, threads, sh, pa, units, sunits); break;
+  }
+  if (rc) return rc;
+  for (int z = 0; z < n; z++) {
+    int gl = G[z] - 1;
+    if (gl >= 0) final_loc[ord[z]] = (gl == 0 || !(gl & 1)) ? c->buf2 : c->buf0;
+  }
+  return 0;
+}
+
+// Error flag of the persistent super-steps since the last check (a wait
+// that ran past its bound); clears it.
+int persist_check(hft_ctx *c) {
+  if (!c->d_pctl) return 0;
+  int err = 0;
+  HIPCHK(hipMemcpyAsync(&err, c->d_pctl + 1, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (err) {
+    int zero = 0;
+    HIPCHK(hipMemcpy(c->d_pctl + 1, &zero, sizeof(int), hipMemcpyHostToDevice));
+    return fail(HFT_E_HIP, "persistent RKG2 super-step: a tile waited past its bound");
+  }
+  return 0;
+}
+
 // The launches of one RKG2 super-step (tables already uploaded).
 int superstep_launches(hft_ctx *c, int n, const std::vector<int> &ord,
                        const std::vector<int64_t> &s, bool use_af) {
@@ -1008,6 +1180,11 @@ int superstep_launches(hft_ctx *c, int n, const std::vector<int> &ord,
   a1.Af = use_af ? c->dty0 : nullptr; a1.rstride = c->rstride;
   int rc = launch(c, 0, k_rkg2_stage1, nblocks(a1.g), a1, 3.0 * n);
   if (rc) return rc;
+  if (persist_usable(c, smax) && c->buf2) {
+    std::vector<double *> fl(nr, nullptr);
+    if ((rc = superstep_persistent(c, n, ord, s, fl))) return rc;
+    return copy_finals(c, fl);
+  }
   // Stage buffers: pA holds u_{k-1}, pB u_{k-2}.  Single-stage launches
   // overwrite u_{k-2} in place (pointwise); fused launches write two fresh
   // buffers.  u_s of a realization that finishes in a fused launch is left
@@ -1247,7 +1424,9 @@ int hft_create(int ntm, int npm1, const double *t, const double *p, int nr, int 
                {(const void *)k_rkg2_fused<4>, 4 * row},  {(const void *)k_rkg2_march<2>, 2 * row},
                {(const void *)k_rkg2_march<3>, 3 * row},  {(const void *)k_rkg2_march<4>, 4 * row},
                {(const void *)k_rkg2_fused2<2>, 3 * row}, {(const void *)k_rkg2_fused2<3>, 4 * row},
-               {(const void *)k_rkg2_fused2<4>, 5 * row}, {(const void *)k_rkg2_fused2<5>, 6 * row}};
+               {(const void *)k_rkg2_fused2<4>, 5 * row}, {(const void *)k_rkg2_fused2<5>, 6 * row},
+               {(const void *)k_rkg2_persist<2>, 3 * row}, {(const void *)k_rkg2_persist<3>, 4 * row},
+               {(const void *)k_rkg2_persist<4>, 5 * row}};
     for (auto &q : lds) {
       hipError_t e2 = hipFuncSetAttribute(q.fn, hipFuncAttributeMaxDynamicSharedMemorySize, q.bytes);
       if (e2 != hipSuccess) rc = fail(HFT_E_HIP, "hipFuncSetAttribute(%d bytes of LDS): %s", q.bytes,
@@ -1298,6 +1477,11 @@ int hft_destroy(hft_ctx *c) {
   if (c->h_adv_order) hipHostFree(c->h_adv_order);
   if (c->h_adv_dt) hipHostFree(c->h_adv_dt);
   if (c->adv_ev) hipEventDestroy(c->adv_ev);
+  if (c->ptiles_ev) hipEventDestroy(c->ptiles_ev);
+  for (void *q : {(void *)c->d_ptiles, (void *)c->d_pint, (void *)c->d_pctl})
+    if (q) hipFree(q);
+  if (c->h_ptiles) hipHostFree(c->h_ptiles);
+  if (c->h_pint) hipHostFree(c->h_pint);
   if (c->own_stream) {
     hipStreamSynchronize(c->own_stream);
     hipStreamDestroy(c->own_stream);  // c->stream belongs to the caller
@@ -1826,7 +2010,7 @@ static int diffusion_impl(hft_ctx *c, double dtl, int subcycles, hft_cycle_fn cb
       if (!done[g]) cyc[g] += 1;
     }
   }
-  return 0;
+  return c->persist ? persist_check(c) : 0;
 }
 
 int hft_diffusion_stats(hft_ctx *c, int64_t *s, int64_t *cycles) {
@@ -2303,6 +2487,21 @@ int hft_set_tuning(hft_ctx *c, const char *key, int value) {
   } else if (!strcmp(key, "fused_band_share")) {
     if (value < 0 || value > 200) return fail(HFT_E_ARG, "fused_band_share must be 0..200");
     c->fused_band_share = value;
+  } else if (!strcmp(key, "persist")) {
+    // The persistent super-step (k_rkg2_persist) did not finish on the
+    // MI355X in its first runs (a 91 x 181 batch of six realizations), with
+    // no cause found; it stays switched off.
+    if (value != 0)
+      return fail(HFT_E_ARG, "persist: the persistent RKG2 super-step is switched off (it did not "
+                             "complete on the GPU; cause not found)");
+    c->persist = 0;
+  } else if (!strcmp(key, "persist_rows") || !strcmp(key, "persist_rows_long")) {
+    if (value < 4 || value > kFusedMaxRows - 10)
+      return fail(HFT_E_ARG, "%s must be 4..%d", key, kFusedMaxRows - 10);
+    (key[12] == '_' ? c->persist_rows_long : c->persist_rows) = value;
+  } else if (!strcmp(key, "persist_long")) {
+    if (value < 0) return fail(HFT_E_ARG, "persist_long must be >= 0");
+    c->persist_long = value;
   } else if (!strcmp(key, "graphs")) {
     c->graphs_on = value != 0;
   } else if (!strcmp(key, "adjoint_segment")) {
@@ -2343,6 +2542,10 @@ int hft_get_tuning(hft_ctx *c, const char *key, int *value) {
   else if (!strcmp(key, "fused_bands")) *value = c->fused_bands;
   else if (!strcmp(key, "fused_band_share")) *value = c->fused_band_share;
   else if (!strcmp(key, "fused_kernel")) *value = c->fused_kernel;
+  else if (!strcmp(key, "persist")) *value = c->persist;
+  else if (!strcmp(key, "persist_rows")) *value = c->persist_rows;
+  else if (!strcmp(key, "persist_rows_long")) *value = c->persist_rows_long;
+  else if (!strcmp(key, "persist_long")) *value = c->persist_long;
   else if (!strcmp(key, "graphs")) *value = c->graphs_on;
   else if (!strcmp(key, "graph_hits")) *value = (int)c->graph_hits;
   else if (!strcmp(key, "graph_builds")) *value = (int)c->graph_builds;

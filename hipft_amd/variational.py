@@ -96,8 +96,9 @@ def synthetic_window(ntm, npm1, nr, ndays, seed=350, eta_truth=350.0, v0_truth=1
                      noise_g=5.0, mu_power=4.0, mu_limit=0.5, nbip=200, clustered=True,
                      steps_per_day=24, autotune=True):
     """A batched gradient-assimilation problem: one seeded truth run (eta_truth,
-    meridional amplitude v0_truth, unit rotation scale) observed once a day for
-    ndays days as truth + N(0, noise_g) Gauss, each observation weighted by
+    meridional amplitude v0_truth, unit rotation scale) observed every
+    steps_per_day steps of 1 h (once a day by default), ndays times, as
+    truth + N(0, noise_g) Gauss, each observation weighted by
     mu^mu_power where mu > mu_limit (the disk within 60 degrees of centre for
     0.5) and 0 elsewhere, mu = sin(theta) cos(phi - phi_c) for an Earth
     observer whose Carrington longitude phi_c falls by 360 degrees per
@@ -122,7 +123,8 @@ def synthetic_window(ntm, npm1, nr, ndays, seed=350, eta_truth=350.0, v0_truth=1
             x = run(x, one(eta_truth), one(v0_truth), one(1.0), truth, steps_per_day)
             noise = torch.as_tensor(rng.normal(0.0, noise_g, tuple(x.shape[1:])), device="cuda")
             obs.append(x[0] + noise)
-            mu = W.mu_map(t, p, -k * 360.0 / SYNODIC_DAYS)[:, 1:]  # unique columns
+            days = k * steps_per_day / 24.0  # dt = 1 h
+            mu = W.mu_map(t, p, -days * 360.0 / SYNODIC_DAYS)[:, 1:]  # unique columns
             w = np.where(mu > mu_limit, np.power(np.abs(mu), mu_power), 0.0)
             weights.append(torch.as_tensor(w, device="cuda"))
     truth.ft.close()

@@ -99,6 +99,30 @@ def test_step_adjoint_matches_autograd(gpu_available, what):
         assert rel(grads[:, 2], gsp) < 1e-10, "dJ/dlog vp scale"
 
 
+@pytest.mark.parametrize("shape", [(33, 1025), (200, 129)])
+@pytest.mark.parametrize("what", ["advection", "step"])
+def test_step_adjoint_matches_autograd_large_grids(gpu_available, shape, what):
+    """The same comparison where the kernels take their other shapes: more
+    than 512 phi columns (the WENO3 transpose in 1024-thread blocks, the
+    fused RKG2 launches over the full width) and 200 theta rows (13 bands of
+    the WENO3 transpose, several fused bands)."""
+    diffusion = what == "step"
+    eta = [150.0, 420.0, 900.0]
+    o, g, ts = build(*shape, 3, eta, flow=True, diffusion=diffusion)
+    x0 = dev_state(g)
+    rng = np.random.default_rng(12)
+    tgt = torch.as_tensor(rng.standard_normal(tuple(x0.shape)), device="cuda")
+    s_r = ts.stage_counts(1.0) if diffusion else None
+    y, lam, grads = gpu_step_and_adjoint(g, x0, tgt, 1.0, True, 1)
+    yt, gx, gD, gst, gsp = torch_grads(ts, x0, tgt, 1.0, True, 1, s_r)
+    assert rel(y, yt) < 1e-12, "forward step"
+    assert rel(lam, gx) < 1e-10, f"dJ/dx ({what})"
+    if diffusion:
+        assert rel(grads[:, 0], gD) < 1e-10, "dJ/dlog eta"
+    assert rel(grads[:, 1], gst) < 1e-10, "dJ/dlog vt scale"
+    assert rel(grads[:, 2], gsp) < 1e-10, "dJ/dlog vp scale"
+
+
 def test_frozen_weights_adjoint_matches_frozen_autograd(gpu_available):
     """Linearised WENO3 with frozen smoothness weights, checked separately
     against autograd of the transcription with detached weights."""

@@ -1061,3 +1061,64 @@ def test_driver_index_cadence_map_list_and_balanced_output(gpu_available, tmp_pa
     assert np.allclose(sol[:, -1, 11:13], ha[:, 9:11], rtol=1e-13, atol=0)
     o.set_pole_flux_lat_limit(30.0)
     assert not np.allclose(sol[:, -1, 11], o.analysis()[:, 9], rtol=1e-3)
+
+
+PERSIST_OFF = ("the persistent super-step did not complete on the MI355X (the 91x181 six-"
+               "realization case ran past its time limit twice, also with atomic polling and a "
+               "wall-clock bound on every wait); cause not found, path switched off")
+
+
+@pytest.mark.skip(reason=PERSIST_OFF)
+@pytest.mark.parametrize("T,rows,long_rows,nlong", [(4, 32, 32, 0), (4, 64, 16, 2), (3, 16, 16, 0),
+                                                    (2, 24, 8, 1)])
+def test_persistent_superstep_matches_launches(gpu_available, T, rows, long_rows, nlong):
+    """One persistent launch per RKG2 super-step (tiles of realization x band
+    x stage group taken from a device list, each waiting for the previous
+    group of its band and the bands either side) gives the multi-launch
+    path's states bit for bit: Strang steps with PTL sub-cycling, six
+    realizations with different stage counts, uniform and per-realization
+    band heights, 2-4 stages per group."""
+    t, p = grid(91, 181)
+    nr = 6
+    maps = np.stack([W.bipole_map(t, p, 20, 500 + r) for r in range(nr)])
+    eta = np.array([900.0, 150.0, 420.0, 260.0, 700.0, 330.0])
+    vt = np.stack([W.meridional_profile(t, 11.0 + 2.0 * r) for r in range(nr)])
+    out = []
+    for persist in (0, 1):
+        g = FluxTransport(t, p, nr, mode="independent")
+        g.set_state(maps, layout="pt")
+        g.set_diffusion(eta)
+        g.set_flow_profiles(vt, None)
+        g.set_flow_analytic(*W.dr_coefficients())
+        g.set_tuning(fused_stages=T, persist=persist, persist_rows=rows,
+                     persist_rows_long=long_rows, persist_long=nlong)
+        states = []
+        for _ in range(3):
+            g.flux_transport_step(1.0, strang=True, subcycles=0)
+            states.append(g.get_state("tp"))
+        s, _ = g.diffusion_stats()
+        out.append(states)
+        g.close()
+    assert len(set(s.tolist())) > 2
+    for i, (a, b) in enumerate(zip(out[1], out[0])):
+        assert_same(a, b, f"persistent super-step, step {i}")
+
+
+@pytest.mark.skip(reason=PERSIST_OFF)
+def test_persistent_superstep_full_size(gpu_available):
+    """512x1024, eight realizations (stage counts from ~110 to ~350): the
+    persistent super-step equals the multi-launch path bit for bit."""
+    t, p = grid(512, 1024)
+    nr = 8
+    maps = np.stack([W.bipole_map(t, p, 40, 600 + r) for r in range(nr)])
+    eta = np.exp(np.linspace(np.log(100.0), np.log(1000.0), nr))
+    out = []
+    for persist in (0, 1):
+        g = FluxTransport(t, p, nr, mode="independent")
+        g.set_state(maps, layout="pt")
+        g.set_diffusion(eta)
+        g.set_tuning(persist=persist, persist_rows=64, persist_rows_long=32, persist_long=2)
+        g.diffusion_step(1.0, subcycles=0)
+        out.append(g.get_state("tp"))
+        g.close()
+    assert_same(out[1], out[0], "persistent super-step at 512x1024")
