@@ -51,6 +51,8 @@ def build_lib(verbose=False):
 
 
 FORTRAN_DRIVER = os.path.join(HERE, "fortran", "hft_run")
+# the reference's executable over the C ABI: reads hipft.in, writes histories
+FORTRAN_HIPFT = os.path.join(HERE, "fortran", "hft_hipft")
 
 
 def amdflang():
@@ -62,24 +64,26 @@ def amdflang():
 
 
 def build_fortran(verbose=False):
-    """The Fortran driver (include/hft.f90 + fortran/hft_run.f90) linked to
-    libhft.so; skipped when no Fortran compiler is installed."""
+    """The Fortran drivers (include/hft.f90 + fortran/hft_run.f90, and
+    fortran/hft_hipft.f90) linked to libhft.so; skipped when no Fortran
+    compiler is installed."""
     fc = amdflang()
     if fc is None:
         if verbose:
-            print("no Fortran compiler: hft_run not built", flush=True)
+            print("no Fortran compiler: Fortran drivers not built", flush=True)
         return None
     import tempfile
-    with tempfile.TemporaryDirectory() as mod_dir:
-        cmd = [fc, "-O2", "-J", mod_dir, os.path.join(ROOT, "include", "hft.f90"),
-               os.path.join(HERE, "fortran", "hft_run.f90"), "-L", HERE, "-lhft",
-               "-Wl,-rpath,$ORIGIN/..", "-o", FORTRAN_DRIVER]
-        if verbose:
-            print(" ".join(cmd), flush=True)
-        res = subprocess.run(cmd, capture_output=True, text=True, cwd=mod_dir)
-    if res.returncode != 0:
-        sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError("Fortran driver build failed")
+    for src, exe in (("hft_run.f90", FORTRAN_DRIVER), ("hft_hipft.f90", FORTRAN_HIPFT)):
+        with tempfile.TemporaryDirectory() as mod_dir:
+            cmd = [fc, "-O2", "-J", mod_dir, os.path.join(ROOT, "include", "hft.f90"),
+                   os.path.join(HERE, "fortran", src), "-L", HERE, "-lhft",
+                   "-Wl,-rpath,$ORIGIN/..", "-o", exe]
+            if verbose:
+                print(" ".join(cmd), flush=True)
+            res = subprocess.run(cmd, capture_output=True, text=True, cwd=mod_dir)
+        if res.returncode != 0:
+            sys.stderr.write(res.stdout + res.stderr)
+            raise RuntimeError(f"Fortran driver build failed ({src})")
     return FORTRAN_DRIVER
 
 

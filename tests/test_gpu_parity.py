@@ -1122,3 +1122,54 @@ def test_persistent_superstep_full_size(gpu_available):
         out.append(g.get_state("tp"))
         g.close()
     assert_same(out[1], out[0], "persistent super-step at 512x1024")
+
+
+@pytest.mark.parametrize("case,hours", [("diffuse_soccer", 48.0),
+                                        ("advect_gaussians_phi_theta", 24.0),
+                                        ("diffuse_advect_soccer", 24.0)])
+def test_fortran_hipft_executable_matches_python_driver(gpu_available, tmp_path, case, hours):
+    """The Fortran executable (hipft_amd/fortran/hft_hipft.f90: reads the
+    testsuite's hipft.in, runs the time loop through include/hft.f90, writes
+    the history files) against driver.main on the same namelist cut to a
+    day or two: the same rows (cycle rows at verbose = 2 included), the same
+    header and line layout, values to 1e-13 (the validation fields are
+    evaluated by Fortran and numpy, whose cos may differ in the last bit);
+    the soccer-ball case also against the reference's own rows."""
+    import subprocess
+    from hipft_amd import build as B
+    exe = B.FORTRAN_HIPFT
+    if not os.path.exists(exe):
+        pytest.skip("Fortran executable not built (no Fortran compiler)")
+    ref = golden(case)
+    text = str(ref["hipft_in"]).replace("time_end = 672.0", f"time_end = {hours}")
+    assert f"time_end = {hours}" in text
+    (tmp_path / "hipft.in").write_text(text)
+    dpy, dfo = tmp_path / "py", tmp_path / "fo"
+    dpy.mkdir()
+    dfo.mkdir()
+    driver.main(str(tmp_path / "hipft.in"), str(dpy))
+    r = subprocess.run([exe, str(tmp_path / "hipft.in")], cwd=str(dfo), capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    nr = int(ref["sol"].shape[0])
+    for k in range(1, nr + 1):
+        for kind in ("sol", "num"):
+            name = f"hipft_history_{kind}_r{k:06d}.out"
+            a = (dfo / name).read_text().split("\n")
+            b = (dpy / name).read_text().split("\n")
+            assert a[0] == b[0] and len(a) == len(b), (name, a[0], b[0], len(a), len(b))
+            assert [len(x) for x in a] == [len(x) for x in b], name
+            fa = np.loadtxt(dfo / name, skiprows=1)
+            fb = np.loadtxt(dpy / name, skiprows=1)
+            if kind == "num":
+                assert np.array_equal(fa[:, [0, 6, 7]], fb[:, [0, 6, 7]]), name
+                assert np.allclose(fa, fb, rtol=1e-14, atol=0), name
+            else:
+                err, verr, _ = history_error(fa[None], fb[None])
+                assert err <= 1e-13 and verr <= 1e-9, (name, err, verr)
+    if case == "diffuse_soccer":
+        sol = np.stack([np.loadtxt(dfo / f"hipft_history_sol_r{k:06d}.out", skiprows=1)
+                        for k in range(1, nr + 1)])
+        n = sol.shape[1]
+        err, verr, _ = history_error(sol, ref["sol"][:, :n])
+        assert err <= 5e-12 and verr <= 1e-6
