@@ -125,6 +125,7 @@ struct hft_ctx {
   int fused_band_share = 32;  // percent: slowdown of a round from an idle to a full device
   int fused_cap[6] = {0, 0, 0, 0, 0, 0};  // co-resident k_rkg2_fused2<T> blocks on the device
   int fused_kernel = 3;  // 1: k_rkg2_fused, 2: k_rkg2_march, 3: k_rkg2_fused2 (2 columns/thread)
+  int fused_body = 1;    // k_rkg2_fused2: interior march steps without row tests
   int adv_band = 1, adv_band_rows = 16;  // banded WENO3 stage kernel
   // persistent super-step (k_rkg2_persist): 0 = off, 1 = on; band rows of
   // the realizations with the most stages and of the others
@@ -937,7 +938,7 @@ int launch_fused(hft_ctx *c, int64_t k, int active, const std::vector<int> &ord,
                  const std::vector<int64_t> &s, const double *pA, const double *pB,
                  const double *dty0, double *oA, double *oB, double *const W[5]) {
   int nr = c->nr, T = c->fused_T;
-  FusedArgs fa;
+  FusedArgs fa{};
   fa.m = c->m;
   fa.H = c->band_rows;
   if (c->fused_target_blocks > 0 &&
@@ -986,6 +987,17 @@ int launch_fused(hft_ctx *c, int64_t k, int active, const std::vector<int> &ord,
     // threads (3-row rings of the T stage levels and of u0)
     int threads = ((fa.ncol / 2 + 63) / 64) * 64;
     size_t sh2 = (size_t)(T + 1) * 3 * 2 * threads * sizeof(double);
+    // interior steps without row tests when the buffers' byte offsets fit
+    // the 32-bit buffer-resource range
+    fa.body = (c->fused_body && c->rstride * sizeof(double) < (size_t)0x7fff0000) ? 1 : 0;
+    if (fa.body && 2 * threads == 1024) {
+      switch (T) {
+        case 2: return launch_ex(c, kGroupFused, k_rkg2_fused2<2, 1024>, blocks, threads, sh2, fa, units, sunits);
+        case 3: return launch_ex(c, kGroupFused, k_rkg2_fused2<3, 1024>, blocks, threads, sh2, fa, units, sunits);
+        case 5: return launch_ex(c, kGroupFused, k_rkg2_fused2<5, 1024>, blocks, threads, sh2, fa, units, sunits);
+        default: return launch_ex(c, kGroupFused, k_rkg2_fused2<4, 1024>, blocks, threads, sh2, fa, units, sunits);
+      }
+    }
     switch (T) {
       case 2: return launch_ex(c, kGroupFused, k_rkg2_fused2<2>, blocks, threads, sh2, fa, units, sunits);
       case 3: return launch_ex(c, kGroupFused, k_rkg2_fused2<3>, blocks, threads, sh2, fa, units, sunits);
@@ -1099,7 +1111,7 @@ int superstep_persistent(hft_ctx *c, int n, const std::vector<int> &ord,
   hipLaunchKernelGGL(k_zero_ints, dim3(std::max(1, std::min(64, (off[n] + 255) / 256))), dim3(256), 0,
                      c->stream, done, off[n]);
   HIPCHK(hipGetLastError());
-  PersistArgs pa;
+  PersistArgs pa{};
   FusedArgs &fa = pa.fa;
   fa.m = c->m;
   fa.ncol = ((c->nu + 63) / 64) * 64;
@@ -1425,6 +1437,10 @@ int hft_create(int ntm, int npm1, const double *t, const double *p, int nr, int 
                {(const void *)k_rkg2_march<3>, 3 * row},  {(const void *)k_rkg2_march<4>, 4 * row},
                {(const void *)k_rkg2_fused2<2>, 3 * row}, {(const void *)k_rkg2_fused2<3>, 4 * row},
                {(const void *)k_rkg2_fused2<4>, 5 * row}, {(const void *)k_rkg2_fused2<5>, 6 * row},
+               {(const void *)k_rkg2_fused2<2, 1024>, 3 * row},
+               {(const void *)k_rkg2_fused2<3, 1024>, 4 * row},
+               {(const void *)k_rkg2_fused2<4, 1024>, 5 * row},
+               {(const void *)k_rkg2_fused2<5, 1024>, 6 * row},
                {(const void *)k_rkg2_persist<2>, 3 * row}, {(const void *)k_rkg2_persist<3>, 4 * row},
                {(const void *)k_rkg2_persist<4>, 5 * row}};
     for (auto &q : lds) {
@@ -2481,6 +2497,9 @@ int hft_set_tuning(hft_ctx *c, const char *key, int value) {
   } else if (!strcmp(key, "fused_kernel")) {
     if (value < 1 || value > 3) return fail(HFT_E_ARG, "fused_kernel must be 1, 2 or 3");
     c->fused_kernel = value;
+  } else if (!strcmp(key, "fused_body")) {
+    if (value < 0 || value > 1) return fail(HFT_E_ARG, "fused_body must be 0 or 1");
+    c->fused_body = value;
   } else if (!strcmp(key, "fused_bands")) {
     if (value < 0 || value > 1) return fail(HFT_E_ARG, "fused_bands must be 0 or 1");
     c->fused_bands = value;
@@ -2540,6 +2559,7 @@ int hft_get_tuning(hft_ctx *c, const char *key, int *value) {
   else if (!strcmp(key, "fused_min_realizations")) *value = c->fused_min;
   else if (!strcmp(key, "fused_target_blocks")) *value = c->fused_target_blocks;
   else if (!strcmp(key, "fused_bands")) *value = c->fused_bands;
+  else if (!strcmp(key, "fused_body")) *value = c->fused_body;
   else if (!strcmp(key, "fused_band_share")) *value = c->fused_band_share;
   else if (!strcmp(key, "fused_kernel")) *value = c->fused_kernel;
   else if (!strcmp(key, "persist")) *value = c->persist;

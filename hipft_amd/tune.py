@@ -23,12 +23,13 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CACHE = os.path.join(HERE, "tuning_cache.json")
 
 DIFF_KEYS = ("fused", "fused_stages", "band_rows", "fused_min_realizations",
-             "fused_target_blocks", "fused_kernel", "fused_bands", "fused_band_share")
+             "fused_target_blocks", "fused_kernel", "fused_bands", "fused_band_share",
+             "fused_body")
 ADV_KEYS = ("advection_band", "advection_band_rows")
 KEYS = DIFF_KEYS + ADV_KEYS
 DEFAULTS = dict(fused=1, fused_stages=4, band_rows=32, fused_min_realizations=8,
                 fused_target_blocks=0, fused_kernel=3, fused_bands=0, fused_band_share=32,
-                advection_band=1, advection_band_rows=16)
+                fused_body=1, advection_band=1, advection_band_rows=16)
 
 
 def adv_candidates():
@@ -61,7 +62,9 @@ def candidates(nr):
                 out.append(dict(fused=1, fused_stages=T, band_rows=32, fused_min_realizations=m,
                                 fused_target_blocks=0, fused_kernel=3, fused_bands=1,
                                 fused_band_share=share))
-    return [dict({"fused_band_share": DEFAULTS["fused_band_share"]}, **c) for c in out]
+    # interior march steps without row tests: same results, never slower
+    return [dict({"fused_band_share": DEFAULTS["fused_band_share"], "fused_body": 1}, **c)
+            for c in out]
 
 
 def cache_key(model):

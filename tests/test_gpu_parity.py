@@ -529,6 +529,39 @@ def test_fused_round_aware_bands_bitexact(gpu_available, T):
     g.close()
 
 
+@pytest.mark.parametrize("T", [2, 3, 4, 5])
+def test_fused_interior_steps_bitexact(gpu_available, T):
+    """The two-column kernel's interior march steps (no row tests, buffer
+    loads and stores, compile-time ring slots) against the oracle on a
+    91x181 grid with bands tall enough for them (runtime LDS row length),
+    and against the same kernel without them on the 512x1024 batch (LDS row
+    length 1024), bit for bit."""
+    o, g = pair(91, 181, nr=3)
+    eta = [150.0, 450.0, 950.0]
+    o.load_diffusion(eta)
+    g.set_diffusion(eta)
+    g.set_tuning(fused=1, fused_stages=T, band_rows=40, fused_min_realizations=1,
+                 fused_kernel=3, fused_body=1)
+    o.diffusion_step(6.0, subcycles=0)
+    g.diffusion_step(6.0, subcycles=0)
+    assert_same(g.get_state("tp"), o.get_f(), f"interior steps T={T} 91x181")
+    g.close()
+    t, p = grid(512, 1024)
+    nr = 6
+    maps = np.stack([W.bipole_map(t, p, 40, 300 + r) for r in range(nr)])
+    out = []
+    for body in (0, 1):
+        m = FluxTransport(t, p, nr, mode="independent")
+        m.set_state(maps, layout="pt")
+        m.set_diffusion(np.array([100.0, 180.0, 320.0, 560.0, 800.0, 1000.0]))
+        m.set_tuning(fused=1, fused_stages=T, fused_min_realizations=1, fused_kernel=3,
+                     fused_bands=1, fused_body=body)
+        m.diffusion_step(1.0, subcycles=0)
+        out.append(m.get_state("tp"))
+        m.close()
+    assert_same(out[1], out[0], f"512x1024 interior steps T={T}")
+
+
 @pytest.mark.parametrize("kernel", [1, 2, 3])
 def test_fused_matches_single_stage_full_size(gpu_available, kernel):
     """512x1024 batch, independent stage counts: fused and one-stage-per-launch
