@@ -28,6 +28,10 @@ struct Mesh {
   const double *pCP, *pMC, *pMT, *pCPT, *pPT, *pMCT, *peps;  // phi, length np
   double bc_diff_n, bc_diff_s;  // bc_diffusion_{n,s}pole_fac
   double bc_flow_n, bc_flow_s;  // bc_flow_{n,s}pole_fac
+  // every WENO3 numerator weight (PT, MCT, MT, CPT) and eps lies in
+  // [2^-100, 2^100]: weno_face may take its short divide where the
+  // smoothness terms allow (set by the host)
+  int fastdiv;
 };
 
 // u_k = a*u_{k-1} + b*u_{k-2} + c*u0 + d*L(u_{k-1}) + g*dty0

@@ -1632,10 +1632,44 @@ __device__ inline WenoW weno_w(const double *CP, const double *MC, const double 
   return w;
 }
 
+// Correctly rounded FP64 quotients without the divide's scaling and fix-up
+// steps.  The compiler's FP64 divide is div_scale (numerator, denominator),
+// a hardware reciprocal refined by two Newton steps, the quotient, its
+// residual and div_fmas (a fused multiply-add, scaled only when div_scale
+// scaled), then div_fixup (special values).  When numerator and
+// denominator are finite, normal, within 2^767 of each other's exponent
+// and the quotient is normal, div_scale leaves both operands unchanged,
+// div_fmas is a plain fma and div_fixup returns its input, so this sequence
+// gives the same bits.  Two quotients by one denominator share its refined
+// reciprocal.
+struct RefinedRecip {
+  double b, y;
+};
+
+__device__ __forceinline__ RefinedRecip refined_recip(double b) {
+  double y = __builtin_amdgcn_rcp(b);
+  double e = fma(-b, y, 1.0);
+  y = fma(y, e, y);
+  e = fma(-b, y, 1.0);
+  y = fma(y, e, y);
+  return {b, y};
+}
+
+__device__ __forceinline__ double quotient(double a, const RefinedRecip &r) {
+  double q = a * r.y;
+  double res = fma(-r.b, q, a);
+  return fma(res, r.y, q);
+}
+
 // One face flux from the split fluxes at the six surrounding cells
-// (src/hipft.f90:7775-7801, 7852-7878).
+// (src/hipft.f90:7775-7801, 7852-7878).  fastdiv (Mesh::fastdiv): the
+// weights' numerators and eps lie in [2^-100, 2^100], so where the four
+// smoothness terms e do too (e >= eps holds; e <= 2^100 is tested), every
+// quotient below has operands in [2^-300, 2^301], no more than 2^601
+// apart, and a normal result: the short divides give the divide's bits.
+// Elsewhere (large or non-finite terms) the full divide runs.
 __device__ inline double weno_face(double LNm2, double LNm1, double LN0, double LPm1, double LP0,
-                                   double LPp1, const WenoW &w) {
+                                   double LPp1, const WenoW &w, int fastdiv = 0) {
   const double one = 1.0, four = 4.0;
   double p0m = (one + w.mcm) * LNm1 - w.mcm * LNm2;
   double p1m = w.mc0 * LNm1 + w.cpm * LN0;
@@ -1646,13 +1680,29 @@ __device__ inline double weno_face(double LNm2, double LNm1, double LN0, double 
   double B0m = four * (t0 * t0), B1m = four * (t1 * t1);
   double B0p = four * (t2 * t2), B1p = four * (t3 * t3);
   double e0 = w.em + B0m, e1 = w.em + B1m, e2 = w.e0 + B0p, e3 = w.e0 + B1p;
-  double w0m = w.ptm / (e0 * e0);
-  double w1m = w.mctm / (e1 * e1);
-  double w0p = w.mt0 / (e2 * e2);
-  double w1p = w.cpt0 / (e3 * e3);
-  double wm_sum = w0m + w1m, wp_sum = w0p + w1p;
-  double OM0m = w0m / wm_sum, OM1m = w1m / wm_sum;
-  double OM0p = w0p / wp_sum, OM1p = w1p / wp_sum;
+  double OM0m, OM1m, OM0p, OM1p;
+  const double kHi = 0x1p100;
+  if (fastdiv && e0 <= kHi && e1 <= kHi && e2 <= kHi && e3 <= kHi) {
+    double w0m = quotient(w.ptm, refined_recip(e0 * e0));
+    double w1m = quotient(w.mctm, refined_recip(e1 * e1));
+    double w0p = quotient(w.mt0, refined_recip(e2 * e2));
+    double w1p = quotient(w.cpt0, refined_recip(e3 * e3));
+    RefinedRecip rm = refined_recip(w0m + w1m), rp = refined_recip(w0p + w1p);
+    OM0m = quotient(w0m, rm);
+    OM1m = quotient(w1m, rm);
+    OM0p = quotient(w0p, rp);
+    OM1p = quotient(w1p, rp);
+  } else {
+    double w0m = w.ptm / (e0 * e0);
+    double w1m = w.mctm / (e1 * e1);
+    double w0p = w.mt0 / (e2 * e2);
+    double w1p = w.cpt0 / (e3 * e3);
+    double wm_sum = w0m + w1m, wp_sum = w0p + w1p;
+    OM0m = w0m / wm_sum;
+    OM1m = w1m / wm_sum;
+    OM0p = w0p / wp_sum;
+    OM1p = w1p / wp_sum;
+  }
   double um = OM0m * p0m + OM1m * p1m;
   double up = OM0p * p0p + OM1p * p1p;
   return up + um;
@@ -1743,7 +1793,7 @@ __device__ inline double theta_face_weno(const Mesh &m, const FlowAcc<F2D> &fa, 
     LP[q] = half * xv * (fa.vt(jr, c) - a);
   }
   WenoW w = weno_w(m.tCP, m.tMC, m.tMT, m.tCPT, m.tPT, m.tMCT, m.teps, jf - 2, jf - 1);
-  return weno_face(LN[0], LN[1], LN[2], LP[0], LP[1], LP[2], w);
+  return weno_face(LN[0], LN[1], LN[2], LP[0], LP[1], LP[2], w, m.fastdiv);
 }
 
 template <bool F2D>
@@ -1766,7 +1816,7 @@ __device__ inline double phi_face_weno(const Mesh &m, const FlowAcc<F2D> &fa, co
     LP[d] = half * xv * (fa.vp(j, wrapc(cc + 1, nu)) - a);  // east face
   }
   WenoW w = weno_w(m.pCP, m.pMC, m.pMT, m.pCPT, m.pPT, m.pMCT, m.peps, q, q + 1);
-  return weno_face(LN[0], LN[1], LN[2], LP[0], LP[1], LP[2], w);
+  return weno_face(LN[0], LN[1], LN[2], LP[0], LP[1], LP[2], w, m.fastdiv);
 }
 
 template <bool F2D>
@@ -1963,7 +2013,7 @@ __global__ __launch_bounds__(1024) void k_advection_band(AdvBandArgs a) {
     w.ptm = T_(11, j); w.mctm = T_(12, j); w.mt0 = T_(13, j); w.cpt0 = T_(14, j);
     w.em = T_(15, j); w.e0 = T_(16, j);
     // LN at rows j-1, j, j+1; LP at rows j, j+1, j+2 (slots 0..3 = rows j-1..j+2)
-    return weno_face(LN4[0], LN4[1], LN4[2], LP4[1], LP4[2], LP4[3], w);
+    return weno_face(LN4[0], LN4[1], LN4[2], LP4[1], LP4[2], LP4[3], w, m.fastdiv);
   };
   // rows j-1..j+2 of the own column
   double X[4], LN4[4], LP4[4];
@@ -2068,7 +2118,7 @@ __global__ __launch_bounds__(1024) void k_advection_band(AdvBandArgs a) {
         double2 q3 = sW[3 * ncol + c], q4 = sW[4 * ncol + c];
         w.mcm = q0.x; w.cpm = q0.y; w.mc0 = q1.x; w.cp0 = q1.y; w.ptm = q2.x;
         w.mctm = q2.y; w.mt0 = q3.x; w.cpt0 = q3.y; w.em = q4.x; w.e0 = q4.y;
-        fpw = weno_face(sLNb[c2], sLNb[c1], sLNb[cc], sLPb[c1], sLPb[cc], sLPb[cp], w);
+        fpw = weno_face(sLNb[c2], sLNb[c1], sLNb[cc], sLPb[c1], sLPb[cc], sLPb[cp], w, m.fastdiv);
       }
       sFPb[c] = fpw;
       pend = true;

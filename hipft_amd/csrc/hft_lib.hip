@@ -126,6 +126,7 @@ struct hft_ctx {
   int fused_cap[6] = {0, 0, 0, 0, 0, 0};  // co-resident k_rkg2_fused2<T> blocks on the device
   int fused_kernel = 3;  // 1: k_rkg2_fused, 2: k_rkg2_march, 3: k_rkg2_fused2 (2 columns/thread)
   int fused_body = 1;    // k_rkg2_fused2: interior march steps without row tests
+  int weno_fastdiv = 1;  // WENO3 weights: short divides where the operands allow
   int adv_band = 1, adv_band_rows = 16;  // banded WENO3 stage kernel
   // persistent super-step (k_rkg2_persist): 0 = off, 1 = on; band rows of
   // the realizations with the most stages and of the others
@@ -413,6 +414,19 @@ void build_mesh(hft_ctx *c, const double *tin, const double *pin) {
   }
 }
 
+// Mesh::fastdiv: every WENO3 weight that is a numerator of the weights'
+// divides (MT, CPT, PT, MCT) and every eps, over the faces the kernels use
+// (theta reference indices 2..nt-2, phi 1..np-1), lies in [2^-100, 2^100].
+void set_fastdiv(hft_ctx *c) {
+  auto in_range = [](double v) { return v >= 0x1p-100 && v <= 0x1p100; };
+  bool ok = c->weno_fastdiv != 0;
+  for (int q = 2; q <= 6 && ok; q++) {
+    for (int j = 2; j <= c->nt - 2; j++) ok = ok && in_range(c->wt[q][j]);
+    for (int k = 1; k <= c->np - 1; k++) ok = ok && in_range(c->wp[q][k]);
+  }
+  c->m.fastdiv = ok ? 1 : 0;
+}
+
 int upload_vec(hft_ctx *c, const std::vector<double> &v, const double **out) {
   double *d;
   int rc = dalloc(&d, v.size());
@@ -455,6 +469,7 @@ int setup_device(hft_ctx *c) {
   m.bc_diff_s = pi_i * c->dth_i[c->nt - 1] * c->dth_i[c->nt - 1];
   m.bc_flow_n = two * c->dth_i[2] * pi_i;  // src/hipft.f90:2568-2569
   m.bc_flow_s = two * c->dth_i[c->nt - 1] * pi_i;
+  set_fastdiv(c);
   size_t nf = c->nfield;
   if ((rc = dalloc(&c->f, nf)) || (rc = dalloc(&c->buf0, nf)) || (rc = dalloc(&c->buf1, nf)) ||
       (rc = dalloc(&c->dty0, nf)))
@@ -2432,6 +2447,7 @@ int hft_set_weno_eps(hft_ctx *c, double eps) {
                    hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy((void *)c->m.peps, c->wp[6].data() + 1, (c->wp[6].size() - 1) * sizeof(double),
                    hipMemcpyHostToDevice));
+  set_fastdiv(c);
   return 0;
 }
 
@@ -2497,6 +2513,10 @@ int hft_set_tuning(hft_ctx *c, const char *key, int value) {
   } else if (!strcmp(key, "fused_kernel")) {
     if (value < 1 || value > 3) return fail(HFT_E_ARG, "fused_kernel must be 1, 2 or 3");
     c->fused_kernel = value;
+  } else if (!strcmp(key, "weno_fastdiv")) {
+    if (value < 0 || value > 1) return fail(HFT_E_ARG, "weno_fastdiv must be 0 or 1");
+    c->weno_fastdiv = value;
+    set_fastdiv(c);
   } else if (!strcmp(key, "fused_body")) {
     if (value < 0 || value > 1) return fail(HFT_E_ARG, "fused_body must be 0 or 1");
     c->fused_body = value;
@@ -2560,6 +2580,7 @@ int hft_get_tuning(hft_ctx *c, const char *key, int *value) {
   else if (!strcmp(key, "fused_target_blocks")) *value = c->fused_target_blocks;
   else if (!strcmp(key, "fused_bands")) *value = c->fused_bands;
   else if (!strcmp(key, "fused_body")) *value = c->fused_body;
+  else if (!strcmp(key, "weno_fastdiv")) *value = c->weno_fastdiv;
   else if (!strcmp(key, "fused_band_share")) *value = c->fused_band_share;
   else if (!strcmp(key, "fused_kernel")) *value = c->fused_kernel;
   else if (!strcmp(key, "persist")) *value = c->persist;

@@ -601,6 +601,37 @@ def test_banded_advection_bitexact(gpu_available, H):
     assert_same(g.get_state("tp"), o.get_f(), f"SSPRK(4,3) band H={H}")
 
 
+@pytest.mark.parametrize("band", [0, 1])
+def test_weno_short_divides_and_their_fallback(gpu_available, band):
+    """WENO3 weights: the short divides (shared refined reciprocal, no
+    scaling or fix-up steps) where the smoothness terms lie in [eps, 2^100],
+    the full divide elsewhere.  A map with spikes of 1e40 (smoothness terms
+    far above 2^100 next to ordinary ones, so lanes of one wavefront take
+    both paths) and a smooth map give the oracle's operator and SSPRK(4,3)
+    steps bit for bit, with the short divides on and off."""
+    for spikes in (False, True):
+        o, g = pair(61, 121, nr=2, seed=5)
+        if spikes:
+            maps = g.get_state("pt").copy()
+            rng = np.random.default_rng(7)
+            idx = rng.integers(0, maps.size, 40)
+            maps.reshape(-1)[idx] = 1e40 * rng.standard_normal(40)
+            o.ingest_pt(maps)
+            g.set_state(maps, layout="pt")
+            assert_same(g.get_state("tp"), o.get_f(), "spiked state")
+        _flows(o, g, 2)
+        for fast in (1, 0):
+            g.set_tuning(advection_band=band, advection_band_rows=16, weno_fastdiv=fast)
+            assert g.get_tuning("weno_fastdiv") == fast
+            x = o.get_f()
+            assert_same(g.apply_advection_operator(x), o.advection_operator(x),
+                        f"A x band={band} spikes={spikes} fastdiv={fast}")
+        o.advection_step(0.5)
+        g.advection_step(0.5)
+        assert_same(g.get_state("tp"), o.get_f(), f"SSPRK(4,3) band={band} spikes={spikes}")
+        g.close()
+
+
 @pytest.mark.parametrize("H", [64, 128])
 def test_banded_advection_tall_bands_512_rows(gpu_available, H):
     """512x1024 clustered, bands of 64 and 128 theta rows (the tuned shapes
