@@ -356,6 +356,8 @@ def main():
     ap.add_argument("--adjoint-only", action="store_true",
                     help="only the adjoint cost measurement (for profiling)")
     ap.add_argument("--retune", action="store_true", help="ignore the cached launch tuning")
+    ap.add_argument("--no-step-graph", action="store_true",
+                    help="host-controlled steps (no device-planned steps replayed as hipGraphs)")
     ap.add_argument("--cfg5", action="store_true",
                     help="only the batched gradient-assimilation measurement (512x1024, 128 "
                          "realizations, 30-day window; one Adam iteration, several minutes)")
@@ -391,6 +393,8 @@ def main():
     from hipft_amd import tune
     tuning = tune.autotune(m, use_cache=not args.retune,
                            log=(lambda msg: print(msg, file=sys.stderr, flush=True)))
+    if args.no_step_graph:
+        m.set_tuning(step_graph=0)
     if args.pmc_probe:
         pmc_probe(m)
         return
@@ -398,6 +402,9 @@ def main():
         step(m)
     # timed region: throughput of the plain step loop
     el = time_steps(m, args.steps)
+    planned = dict(on=m.get_tuning("step_graph"), builds=m.get_tuning("step_graph_builds"),
+                   replays=m.get_tuning("step_graph_hits"),
+                   overflows=m.get_tuning("step_graph_overflows"))
     # second pass over as many steps with HIP events around every launch on
     # the library's stream (per-kernel times for the roofline; the events
     # themselves cost time, so this pass does not give the throughput)
@@ -461,6 +468,7 @@ def main():
                    "ntm": 512, "npm1": 1024, "realizations": nr,
                    "stages_last_step_min_max": [int(s_last.min()), int(s_last.max())],
                    "parallelism": "realization batch on 1 GPU",
+                   "planned_steps": planned,
                    "launch_tuning": {k: tuning.get(k, tune.DEFAULTS[k]) for k in tune.KEYS}},
         "roofline": roof,
         "kernels": groups,

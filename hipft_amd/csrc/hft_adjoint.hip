@@ -585,6 +585,7 @@ extern "C" {
 int hft_set_stream(hft_ctx *c, void *stream, int own) {
   if (!c) return fail(HFT_E_ARG, "null context");
   HIPCHK(hipStreamSynchronize(c->stream));
+  c->epoch++;
   if (!own) {
     if (!c->own_stream) c->own_stream = c->stream;
     c->stream = (hipStream_t)stream;  // NULL: the null stream

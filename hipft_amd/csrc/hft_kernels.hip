@@ -131,10 +131,26 @@ struct Stage1Args {
   const int *order;      // launch index -> realization
   const double *Af;      // optional precomputed L(f) (PTL pass), may be null
   size_t rstride;
+  // device count of the launch's realizations (g.nreal is then the most
+  // there may be): blocks past that count return at once
+  const int *dn = nullptr;
 };
 
+// First tile index of a block, with the XCD remap over the tiles that exist:
+// n realizations x tiles per realization when a device count is given (the
+// blocks beyond return), else the whole grid.  Returns -1 for such a block.
+__device__ inline int tile_index(const Tiles &g, const int *dn) {
+  int nt = (int)gridDim.x;
+  if (dn) {
+    nt = *dn * g.ntm * g.ctiles;
+    if ((int)blockIdx.x >= nt) return -1;
+  }
+  return xcd_remap(blockIdx.x, nt);
+}
+
 __global__ __launch_bounds__(kBlock) void k_rkg2_stage1(Stage1Args a) {
-  int t = xcd_remap(blockIdx.x, gridDim.x);
+  int t = tile_index(a.g, a.dn);
+  if (t < 0) return;
   int z, j, ct;
   tile_of(t, a.g, z, j, ct);
   int r = a.order[z];
@@ -265,6 +281,10 @@ struct FusedArgs {
   double *W[5];        // optional: stage k0+t-1 rows into W[t-1] (null: not kept)
   const double *cdt;   // [nr] cycle time step (k_rkg2_fused2 forms dty0 = dt L(u0) itself)
   int body;            // k_rkg2_fused2: interior steps without row tests (fused2_body_step)
+  // k_rkg2_fused2 launched from a device plan: {tiles, H, nbands} of this
+  // launch in device memory (nbands, H and the grid size above are then
+  // ignored; blocks past the tile count return at once)
+  const int *slot = nullptr;
 };
 
 constexpr int kFusedMaxRows = 160;  // H + 2T rows of per-row coefficients
@@ -1391,12 +1411,19 @@ __global__ __launch_bounds__(512) void k_rkg2_fused2(FusedArgs a) {
   extern __shared__ double lds[];  // [T + 1][3][2 * blockDim]: stage levels 0..T-1, then u0
   __shared__ double cfs[T][5];
   __shared__ __align__(16) double rowc[4 * kFusedMaxRows];
-  int L = xcd_remap(blockIdx.x, gridDim.x);
-  int z = L / a.nbands, band = L - z * a.nbands;
+  int nt = (int)gridDim.x, nb = a.nbands, H = a.H;
+  if (a.slot) {
+    nt = a.slot[0];
+    H = a.slot[1];
+    nb = a.slot[2];
+    if ((int)blockIdx.x >= nt) return;
+  }
+  int L = xcd_remap(blockIdx.x, nt);
+  int z = L / nb, band = L - z * nb;
   bool stored = false;
 #pragma unroll
   for (int t = 0; t < T; t++) stored = stored || a.W[t] != nullptr;
-  fused2_tile<T, NR>(a, z, band, a.H, a.k0, a.A, a.B, a.outA, a.outB, lds, cfs, rowc,
+  fused2_tile<T, NR>(a, z, band, H, a.k0, a.A, a.B, a.outA, a.outB, lds, cfs, rowc,
                      a.body != 0 && !stored);
 }
 
@@ -1518,10 +1545,12 @@ struct LapArgs {
   size_t rstride;
   // PTL: per-realization max |Lx| over points with |x| > 1e-14
   unsigned long long *absmax;  // may be null
+  const int *dn = nullptr;     // device realization count (as Stage1Args)
 };
 
 __global__ __launch_bounds__(kBlock) void k_laplacian(LapArgs a) {
-  int t = xcd_remap(blockIdx.x, gridDim.x);
+  int t = tile_index(a.g, a.dn);
+  if (t < 0) return;
   int z, j, ct;
   tile_of(t, a.g, z, j, ct);
   int r = a.order[z];
@@ -1565,10 +1594,12 @@ struct PtlArgs {
   int coupled;      // target = max over the launch's realizations
   int nlaunch;
   size_t rstride;
+  const int *dn = nullptr;  // device realization count (replaces nlaunch)
 };
 
 __global__ __launch_bounds__(kBlock) void k_ptl_min(PtlArgs a) {
-  int t = xcd_remap(blockIdx.x, gridDim.x);
+  int t = tile_index(a.g, a.dn);
+  if (t < 0) return;
   int z, j, ct;
   tile_of(t, a.g, z, j, ct);
   int r = a.order[z];
@@ -1577,7 +1608,8 @@ __global__ __launch_bounds__(kBlock) void k_ptl_min(PtlArgs a) {
   double target;
   if (a.coupled) {
     unsigned long long mx = 0;
-    for (int q = 0; q < a.nlaunch; q++) {
+    int nl = a.dn ? *a.dn : a.nlaunch;
+    for (int q = 0; q < nl; q++) {
       unsigned long long v = a.absmax[a.order[q]];
       mx = v > mx ? v : mx;
     }
@@ -1752,6 +1784,7 @@ struct AdvArgs {
   const int *order;
   size_t rstride;
   int write_aop;       // operator-only mode: out = A(x)
+  const int *skip = nullptr;  // when set and *skip != 0 the launch does nothing
 };
 
 template <bool F2D>
@@ -1849,6 +1882,7 @@ __device__ inline double advection_op(const Mesh &m, const FlowAcc<F2D> &fa, con
 
 template <bool F2D, bool HAS_Y>
 __global__ __launch_bounds__(kBlock) void k_advection_stage(AdvArgs a) {
+  if (a.skip && *a.skip) return;
   int t = xcd_remap(blockIdx.x, gridDim.x);
   int z, j, ct;
   tile_of(t, a.g, z, j, ct);
@@ -1910,6 +1944,7 @@ struct AdvBandArgs {
   int nbands, H, ncol;
   const int *order;
   size_t rstride;
+  const int *skip = nullptr;  // when set and *skip != 0 the launch does nothing
 };
 
 // Row-uniform values per row of the band table, and the most rows a band
@@ -1927,6 +1962,7 @@ __global__ __launch_bounds__(1024) void k_advection_band(AdvBandArgs a) {
   const Mesh &m = a.m;
   const int ncol = a.ncol, nu = m.nu, ntm = m.ntm, nt = m.nt, pitch = m.pitch;
   double2 *sW = reinterpret_cast<double2 *>(sh + 7 * ncol);
+  if (a.skip && *a.skip) return;
   int L = xcd_remap(blockIdx.x, gridDim.x);
   int z = L / a.nbands, band = L - z * a.nbands;
   int r = a.order[z];
@@ -2581,6 +2617,388 @@ __global__ __launch_bounds__(kBlock) void k_flow_dtmax(const double *vt, const d
   }
   for (int h = 32; h >= 1; h >>= 1) best = fmin(best, __shfl_xor(best, h, 64));
   if ((threadIdx.x & 63) == 0) atomicMin(&out[r], (unsigned long long)__double_as_longlong(best));
+}
+
+
+// ---------------------------------------------------------------------------
+// Device-planned diffusion step: the controller of diffusion_step
+// (src/hipft.f90:4589-4690), the PTL step rule of get_dtime_diffusion_ptl
+// (5077-5156) and the RKG2 tables of load_sts_rkg2 (5399-5461) evaluated on
+// the device, so that a whole time step is one fixed launch sequence that
+// can be replayed as a hipGraph without a host round trip:
+//   k_plan_init;
+//   per round:  [k_plan_prep, k_laplacian, k_ptl_min]   (PTL sub-cycling)
+//               k_plan_round, k_plan_coefs, k_rkg2_stage1,
+//               gcap x k_rkg2_fused2 (launch q: stages 2+qT .. 2+qT+T-1),
+//               k_copy_finals_plan;
+//   k_plan_end.
+// k_plan_round is one block: every controller (one per realization, or one
+// for the batch in coupled mode) takes the host controller's steps in the
+// same floating-point order, so step sizes, stage counts and coefficients
+// are the host's bit for bit.  A round whose super-step would need more
+// fused launches than the sequence holds (or a step that needs more rounds)
+// sets the overflow flag instead: that round and every later kernel of the
+// sequence do nothing, and the host resumes the step from the controller
+// state in device memory.
+// ---------------------------------------------------------------------------
+struct PlanArgs {
+  // controller state, one entry per controller g (ng = 1 coupled, nr independent)
+  double *stepped, *dt2;
+  int64_t *sdiag;
+  int *cyc, *done;
+  int *status;        // [0] overflow, [1] round to resume at, [2] realizations this round
+  int64_t *smax;      // [rounds] largest stage count a round needed (0: none cycled)
+  int64_t *last_s, *last_cycles;  // [nr] hft_diffusion_stats
+  // super-step tables read by the round's kernels
+  int *order;         // launch index -> realization (PTL pass: ascending; super-step: by s)
+  int64_t *s;         // [nr]
+  double *cdt, *ub1;  // [nr] cycle time step, ubj(1)
+  StageCoef *coef;    // [k][nr]
+  int *slots;         // [gcap][3]: tiles, H, nbands of fused launch q
+  const int *htab;    // [nr + 1] band rows of a fused launch over a realizations
+  const double *dte;  // [ng] explicit Euler limit
+  unsigned long long *absmax, *dtmin;  // [nr] PTL results (bits of non-negative doubles)
+  double dtl;         // step length
+  double dt2_fixed;   // dtl / subcycles when subcycles > 0
+  int subcycles, coupled, nr, ng, T, ntm, round, gcap, kspan;
+};
+
+// Exclusive count of set flags over the block (thread order), and the total.
+__device__ inline int block_prefix(bool flag, int *wsum, int *total) {
+  unsigned long long mk = __ballot(flag);
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+  int before = __popcll(mk & ((1ull << lane) - 1ull));
+  if (lane == 0) wsum[w] = __popcll(mk);
+  __syncthreads();
+  int off = 0, tot = 0;
+  for (int q = 0; q < nw; q++) {
+    if (q < w) off += wsum[q];
+    tot += wsum[q];
+  }
+  __syncthreads();
+  *total = tot;
+  return off + before;
+}
+
+// load_sts_rkg2 (src/hipft.f90:5399-5461): stage count of a cycle of length
+// dtl for the explicit limit dte, and the coefficients of stage k >= 2.
+__device__ inline int64_t plan_sts_stages(double dtl, double dte) {
+  double s_real = 0.5 * (sqrt(25.0 + 24.0 * (dtl / dte)) - 3.0);
+  double sc = ceil(s_real);
+  int64_t s = sc > 9.0e15 ? (int64_t)9.0e15 : (int64_t)sc;
+  if (s < 3) s = 3;
+  if (s % 2 == 0) s = s + 1;
+  return s;
+}
+
+__device__ inline double plan_sts_b(int64_t j) {
+  if (j == 1) return 1.0 / 3.0;
+  if (j == 2) return 1.0 / 15.0;
+  double dj = (double)j;
+  return (4.0 * (double)(j - 1) * (double)(j + 4)) /
+         (3.0 * dj * (double)(j + 1) * (double)(j + 2) * (double)(j + 3));
+}
+
+__device__ inline double plan_sts_w(int64_t s) { return 6.0 / (double)((s + 4) * (s - 1)); }
+
+__device__ inline StageCoef plan_sts_coef(int64_t s, int64_t k, double dtl) {
+  double w = plan_sts_w(s);
+  double u, v, ub, gm;
+  if (k == 2) {
+    u = 1.0 / 2.0;
+    v = -1.0 / 10.0;
+    ub = u * w;
+    gm = 0.0;
+  } else {
+    double dj = (double)k;
+    double bj = plan_sts_b(k), bjm1 = plan_sts_b(k - 1), bjm2 = plan_sts_b(k - 2);
+    double bj_bjm1 = bj / bjm1;
+    double bj_bjm2 = bj / bjm2;
+    u = bj_bjm1 * (2.0 + 1.0 / dj);
+    v = -bj_bjm2 * (1.0 / dj + 1.0);
+    ub = u * w;
+    gm = (0.5 * dj * (dj + 1.0) * bjm1 - 1.0) * ub;
+  }
+  StageCoef cf;
+  cf.a = u;
+  cf.b = v;
+  cf.c = 1.0 - u - v;
+  cf.d = ub * dtl;
+  cf.g = gm;
+  cf.w = nullptr;
+  cf.pad[0] = cf.pad[1] = 0.0;
+  return cf;
+}
+
+__global__ __launch_bounds__(1024) void k_plan_init(PlanArgs p) {
+  for (int g = threadIdx.x; g < p.ng; g += blockDim.x) {
+    p.stepped[g] = 0.0;
+    p.dt2[g] = p.subcycles > 0 ? p.dt2_fixed : 0.0;
+    p.sdiag[g] = p.subcycles;
+    p.cyc[g] = 1;
+    p.done[g] = 0;
+  }
+  if (threadIdx.x == 0) {
+    p.status[0] = 0;
+    p.status[1] = -1;
+    p.status[2] = 0;
+  }
+}
+
+// PTL pass set-up of a round (ptl_pass): the realizations whose controller
+// still cycles, ascending, into order / status[2]; the max and min
+// accumulators reset.  One block, nr <= blockDim.
+__global__ __launch_bounds__(1024) void k_plan_prep(PlanArgs p) {
+  __shared__ int wsum[16];
+  int r = threadIdx.x, total;
+  bool ovf = p.status[0] != 0;
+  bool act = !ovf && r < p.nr && !p.done[p.coupled ? 0 : r];
+  int idx = block_prefix(act, wsum, &total);
+  if (act) p.order[idx] = r;
+  if (r < p.nr) {
+    p.absmax[r] = 0ull;
+    p.dtmin[r] = (unsigned long long)__double_as_longlong(1.7976931348623157e308);
+  }
+  if (threadIdx.x == 0) p.status[2] = total;
+}
+
+// One round of the controller (diffusion_step's loop body, with ptl_pass's
+// step rule) and the super-step's tables.  One block, nr <= blockDim.
+__global__ __launch_bounds__(1024) void k_plan_round(PlanArgs p) {
+  __shared__ int wsum[16];
+  __shared__ long long s_of[1024];   // stage count per realization (-1: not in the round)
+  __shared__ long long s_sorted[1024];
+  __shared__ unsigned long long red_mx[16], red_mn[16];
+  __shared__ long long red_s[16];
+  __shared__ double b_d2, b_stp;
+  __shared__ long long b_s, b_sd;
+  __shared__ int b_act, b_cyc, b_ovf;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, nw = (blockDim.x + 63) >> 6;
+  const bool auto_sc = p.subcycles == 0;
+  const double kBig = 1.7976931348623157e308;
+  if (p.status[0]) {
+    for (int q = tid; q < p.gcap; q += blockDim.x) {
+      p.slots[3 * q] = 0;
+      p.slots[3 * q + 1] = 1;
+      p.slots[3 * q + 2] = 1;
+    }
+    if (tid == 0) p.status[2] = 0;
+    return;
+  }
+  // coupled PTL: one step for the batch, from the max of |L f| and the min
+  // of the step limits over every realization
+  unsigned long long mx = 0ull, mn = (unsigned long long)__double_as_longlong(kBig);
+  if (p.coupled && auto_sc) {
+    for (int r = tid; r < p.nr; r += blockDim.x) {
+      mx = p.absmax[r] > mx ? p.absmax[r] : mx;
+      mn = p.dtmin[r] < mn ? p.dtmin[r] : mn;
+    }
+    for (int h = 32; h >= 1; h >>= 1) {
+      unsigned long long a = __shfl_xor(mx, h, 64), b = __shfl_xor(mn, h, 64);
+      mx = a > mx ? a : mx;
+      mn = b < mn ? b : mn;
+    }
+    if (lane == 0) {
+      red_mx[w] = mx;
+      red_mn[w] = mn;
+    }
+    __syncthreads();
+    mx = 0ull;
+    mn = (unsigned long long)__double_as_longlong(kBig);
+    for (int q = 0; q < nw; q++) {
+      mx = red_mx[q] > mx ? red_mx[q] : mx;
+      mn = red_mn[q] < mn ? red_mn[q] : mn;
+    }
+  }
+  // controller g = tid: this round's cycle time step, stage count and
+  // diagnostic sub-cycle count
+  int g = tid;
+  bool act = g < p.ng && !p.done[g];
+  double d2 = 0.0, stp = 0.0;
+  int dn = 0, cyc = 0;
+  long long sd = 0, sg = 0;
+  if (act) {
+    stp = p.stepped[g];
+    d2 = p.dt2[g];
+    cyc = p.cyc[g];
+    if (auto_sc) {
+      unsigned long long axb = p.coupled ? mx : p.absmax[g];
+      unsigned long long dmb = p.coupled ? mn : p.dtmin[g];
+      double ax = __longlong_as_double((long long)axb), dmin = __longlong_as_double((long long)dmb);
+      double dtp = kBig;
+      if (ax > 0.0) {
+        dtp = 0.95 * dmin;
+        if (dtp < d2) dtp = d2;
+      }
+      d2 = dtp;
+    }
+    if (auto_sc && stp + d2 >= p.dtl) {
+      d2 = p.dtl - stp;
+      dn = 1;
+    }
+    if (auto_sc && cyc == 60 - 1) {
+      d2 = p.dtl - stp;
+      dn = 1;
+    }
+    sd = p.sdiag[g];
+    if (auto_sc) {
+      double q = ceil((p.dtl - stp) / d2);
+      sd = q > 2e9 ? 2000000000LL : (long long)q;
+    }
+    sg = plan_sts_stages(d2, p.dte[g]);
+  }
+  // largest stage count of the round -> fused launches it needs
+  long long sm = act ? sg : 0;
+  for (int h = 32; h >= 1; h >>= 1) {
+    long long o = __shfl_xor(sm, h, 64);
+    sm = o > sm ? o : sm;
+  }
+  if (lane == 0) red_s[w] = sm;
+  if (p.coupled && tid == 0) {
+    b_act = act ? 1 : 0;
+    b_d2 = d2;
+    b_stp = stp;
+    b_s = sg;
+    b_sd = sd;
+    b_cyc = cyc;
+  }
+  __syncthreads();
+  sm = 0;
+  for (int q = 0; q < nw; q++) sm = red_s[q] > sm ? red_s[q] : sm;
+  int need = sm > 0 ? (int)((sm - 1 + p.T - 1) / p.T) : 0;
+  if (tid == 0) {
+    p.smax[p.round] = sm;
+    b_ovf = need > p.gcap ? 1 : 0;
+    if (b_ovf) {
+      p.status[0] = 1;
+      p.status[1] = p.round;
+    }
+  }
+  __syncthreads();
+  if (b_ovf || sm == 0) {
+    for (int q = tid; q < p.gcap; q += blockDim.x) {
+      p.slots[3 * q] = 0;
+      p.slots[3 * q + 1] = 1;
+      p.slots[3 * q + 2] = 1;
+    }
+    if (tid == 0) p.status[2] = 0;
+    return;
+  }
+  // per realization r = tid
+  int r = tid;
+  bool ract = false;
+  double rd2 = 0.0, rstp = 0.0;
+  long long rs = 0, rsd = 0;
+  int rcyc = 0;
+  if (r < p.nr) {
+    if (p.coupled) {
+      ract = b_act != 0;
+      rd2 = b_d2; rstp = b_stp; rs = b_s; rsd = b_sd; rcyc = b_cyc;
+    } else {
+      ract = act;
+      rd2 = d2; rstp = stp; rs = sg; rsd = sd; rcyc = cyc;
+    }
+    s_of[r] = ract ? rs : -1;
+  }
+  __syncthreads();
+  if (ract) {
+    // stable order by stage count, descending (ties in realization order)
+    int rank = 0;
+    for (int q = 0; q < p.nr; q++) {
+      long long o = s_of[q];
+      rank += (o > rs || (o == rs && q < r)) ? 1 : 0;
+    }
+    p.order[rank] = r;
+    s_sorted[rank] = rs;
+    p.s[r] = rs;
+    p.cdt[r] = rd2;
+    p.ub1[r] = plan_sts_w(rs);
+    p.last_s[r] = rs;
+    p.last_cycles[r] = rcyc;
+  }
+  int n;
+  block_prefix(ract, wsum, &n);  // (also a barrier: s_sorted complete)
+  // controller state after the round
+  if (act) {
+    p.dt2[g] = d2;
+    p.sdiag[g] = sd;
+    p.stepped[g] = stp + d2;
+    int dnn = dn;
+    if (!auto_sc && cyc == p.subcycles) dnn = 1;
+    if (dnn) p.done[g] = 1;
+    else p.cyc[g] = cyc + 1;
+  }
+  // fused launch q advances the realizations with s >= 2 + qT
+  for (int q = tid; q < p.gcap; q += blockDim.x) {
+    long long k0 = 2 + (long long)q * p.T;
+    int lo = 0, hi = n;
+    while (lo < hi) {
+      int mid = (lo + hi) >> 1;
+      if (s_sorted[mid] >= k0) lo = mid + 1;
+      else hi = mid;
+    }
+    int a = lo;
+    if (a == 0) {
+      p.slots[3 * q] = 0;
+      p.slots[3 * q + 1] = 1;
+      p.slots[3 * q + 2] = 1;
+    } else {
+      int H = p.htab[a];
+      int nb = (p.ntm + H - 1) / H;
+      p.slots[3 * q] = a * nb;
+      p.slots[3 * q + 1] = H;
+      p.slots[3 * q + 2] = nb;
+    }
+  }
+  if (tid == 0) p.status[2] = n;
+}
+
+// Coefficient table of the round's super-step: thread (z, k) writes stage
+// k = 2..s of realization order[z].
+__global__ __launch_bounds__(kBlock) void k_plan_coefs(PlanArgs p) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int z = i / p.kspan;
+  int64_t k = 2 + (i - z * p.kspan);
+  if (z >= p.status[2]) return;
+  int r = p.order[z];
+  int64_t s = p.s[r];
+  if (k > s) return;
+  p.coef[(size_t)k * p.nr + r] = plan_sts_coef(s, k, p.cdt[r]);
+}
+
+// Final stages of a planned super-step into f: fused launch q writes its
+// last stage to src[q & 1], and realization r finishes in launch (s_r-2)/T.
+struct PlanCopyArgs {
+  double *dst;
+  const double *src[2];
+  const int *order;
+  const int *n;
+  const int64_t *s;
+  size_t rstride;
+  int per, T;
+};
+
+__global__ __launch_bounds__(kBlock) void k_copy_finals_plan(PlanCopyArgs a) {
+  int z = blockIdx.x / a.per;
+  if (z >= *a.n) return;
+  size_t i = (size_t)(blockIdx.x - z * a.per) * kBlock + threadIdx.x;
+  if (i >= a.rstride / 2) return;
+  int r = a.order[z];
+  int q = (int)((a.s[r] - 2) / a.T);
+  size_t o = (size_t)r * a.rstride + 2 * i;
+  *reinterpret_cast<double2 *>(a.dst + o) = *reinterpret_cast<const double2 *>(a.src[q & 1] + o);
+}
+
+// After the last round of the sequence (p.round = the number of rounds it
+// holds): a controller that has not finished its step needs more rounds.
+__global__ void k_plan_end(PlanArgs p) {
+  if (threadIdx.x != 0 || p.status[0]) return;
+  for (int g = 0; g < p.ng; g++)
+    if (!p.done[g]) {
+      p.status[0] = 1;
+      p.status[1] = p.round;
+      return;
+    }
 }
 
 }  // namespace hft

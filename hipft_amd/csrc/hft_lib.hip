@@ -152,6 +152,22 @@ struct hft_ctx {
   int graphs_on = 0;
   std::vector<std::pair<std::vector<int64_t>, hipGraphExec_t>> graphs;
   int64_t graph_hits = 0, graph_builds = 0;
+  // whole flux-transport steps planned on the device and replayed as one
+  // hipGraph (tuning "step_graph"): the captured sequence, its key, the key
+  // of the previous step, and the largest stage count seen in each round of
+  // sub-cycles (sizes the sequence)
+  int step_graph = 1, step_graph_rounds = 1;
+  hipGraphExec_t sg_exec = nullptr;
+  std::vector<int64_t> sg_key, sg_prev, sg_smax;
+  int64_t sg_hits = 0, sg_builds = 0, sg_overflows = 0;
+  int64_t epoch = 0;  // bumped by every setter that changes what a captured step reads
+  char *d_plan = nullptr, *h_plan = nullptr;  // controller state (device) / its pinned copy
+  size_t plan_bytes = 0;
+  int *d_slots = nullptr, *d_htab = nullptr;
+  size_t slots_cap = 0;
+  double *d_dte = nullptr;
+  std::vector<int64_t> *obs_smax = nullptr;  // host controller: record each round's largest s
+  const int *adv_skip = nullptr;             // advection launches skip when *adv_skip != 0
   bool have_fval = false;
   // flows
   bool flow_set = false, flow2d = false;
@@ -769,6 +785,9 @@ void sts_coefs(int64_t s, std::vector<double> &uj, std::vector<double> &vj,
 void clear_graphs(hft_ctx *c) {
   for (auto &g : c->graphs) hipGraphExecDestroy(g.second);
   c->graphs.clear();
+  if (c->sg_exec) hipGraphExecDestroy(c->sg_exec);
+  c->sg_exec = nullptr;
+  c->sg_key.clear();
 }
 
 struct CoefCache {
@@ -945,6 +964,56 @@ int fused_band_count(int ntm, int T, int active, int cap, double kShare) {
   return best;
 }
 
+// Band rows of a fused launch over `active` realizations.
+int fused_rows(hft_ctx *c, int active, int *H) {
+  int T = c->fused_T;
+  int h = c->band_rows;
+  if (c->fused_target_blocks > 0 &&
+      (int64_t)active * ((c->ntm + h - 1) / h) < c->fused_target_blocks) {
+    // few realizations left: thinner bands so the launch still fills the GPU
+    int64_t t = (int64_t)c->ntm * active / c->fused_target_blocks;
+    h = (int)std::max<int64_t>(4, std::min<int64_t>(h, t));
+  }
+  if (c->fused_bands && c->fused_kernel == 3) {
+    int ncol = ((c->nu + 63) / 64) * 64;
+    int threads = ((ncol / 2 + 63) / 64) * 64, cap = 0, rc;
+    size_t sh2 = (size_t)(T + 1) * 3 * 2 * threads * sizeof(double);
+    switch (T) {
+      case 2: rc = fused2_capacity<2>(c, threads, sh2, &cap); break;
+      case 3: rc = fused2_capacity<3>(c, threads, sh2, &cap); break;
+      case 5: rc = fused2_capacity<5>(c, threads, sh2, &cap); break;
+      default: rc = fused2_capacity<4>(c, threads, sh2, &cap); break;
+    }
+    if (rc) return rc;
+    int nb = fused_band_count(c->ntm, T, active, cap, 0.01 * c->fused_band_share);
+    h = (c->ntm + nb - 1) / nb;
+  }
+  *H = h;
+  return 0;
+}
+
+// Launch of k_rkg2_fused2<T> (the LDS row length a compile-time constant
+// when it is 1024 and interior steps are on).
+int launch_fused2(hft_ctx *c, const FusedArgs &fa, int blocks, double units, double sunits) {
+  int T = c->fused_T;
+  int threads = ((fa.ncol / 2 + 63) / 64) * 64;
+  size_t sh2 = (size_t)(T + 1) * 3 * 2 * threads * sizeof(double);
+  if (fa.body && 2 * threads == 1024) {
+    switch (T) {
+      case 2: return launch_ex(c, kGroupFused, k_rkg2_fused2<2, 1024>, blocks, threads, sh2, fa, units, sunits);
+      case 3: return launch_ex(c, kGroupFused, k_rkg2_fused2<3, 1024>, blocks, threads, sh2, fa, units, sunits);
+      case 5: return launch_ex(c, kGroupFused, k_rkg2_fused2<5, 1024>, blocks, threads, sh2, fa, units, sunits);
+      default: return launch_ex(c, kGroupFused, k_rkg2_fused2<4, 1024>, blocks, threads, sh2, fa, units, sunits);
+    }
+  }
+  switch (T) {
+    case 2: return launch_ex(c, kGroupFused, k_rkg2_fused2<2>, blocks, threads, sh2, fa, units, sunits);
+    case 3: return launch_ex(c, kGroupFused, k_rkg2_fused2<3>, blocks, threads, sh2, fa, units, sunits);
+    case 5: return launch_ex(c, kGroupFused, k_rkg2_fused2<5>, blocks, threads, sh2, fa, units, sunits);
+    default: return launch_ex(c, kGroupFused, k_rkg2_fused2<4>, blocks, threads, sh2, fa, units, sunits);
+  }
+}
+
 // One fused launch: stages k..k+T-1 of the first `active` realizations of
 // ord, reading u_{k-1} (pA) and u_{k-2} (pB); the last stage goes to oA (or
 // each realization's final stage, if earlier), the one before to oB, and
@@ -955,26 +1024,10 @@ int launch_fused(hft_ctx *c, int64_t k, int active, const std::vector<int> &ord,
   int nr = c->nr, T = c->fused_T;
   FusedArgs fa{};
   fa.m = c->m;
-  fa.H = c->band_rows;
-  if (c->fused_target_blocks > 0 &&
-      (int64_t)active * ((c->ntm + fa.H - 1) / fa.H) < c->fused_target_blocks) {
-    // few realizations left: thinner bands so the launch still fills the GPU
-    int64_t h = (int64_t)c->ntm * active / c->fused_target_blocks;
-    fa.H = (int)std::max<int64_t>(4, std::min<int64_t>(fa.H, h));
-  }
   fa.ncol = ((c->nu + 63) / 64) * 64;
-  if (c->fused_bands && c->fused_kernel == 3) {
-    int threads = ((fa.ncol / 2 + 63) / 64) * 64, cap = 0, rc;
-    size_t sh2 = (size_t)(T + 1) * 3 * 2 * threads * sizeof(double);
-    switch (T) {
-      case 2: rc = fused2_capacity<2>(c, threads, sh2, &cap); break;
-      case 3: rc = fused2_capacity<3>(c, threads, sh2, &cap); break;
-      case 5: rc = fused2_capacity<5>(c, threads, sh2, &cap); break;
-      default: rc = fused2_capacity<4>(c, threads, sh2, &cap); break;
-    }
+  {
+    int rc = fused_rows(c, active, &fa.H);
     if (rc) return rc;
-    int nb = fused_band_count(c->ntm, T, active, cap, 0.01 * c->fused_band_share);
-    fa.H = (c->ntm + nb - 1) / nb;
   }
   fa.nbands = (c->ntm + fa.H - 1) / fa.H;
   fa.k0 = (int)k;
@@ -999,26 +1052,11 @@ int launch_fused(hft_ctx *c, int64_t k, int active, const std::vector<int> &ord,
   bool store = W[0] || W[1] || W[2] || W[3] || W[4];
   if (c->fused_kernel == 3) {
     // two phi columns per thread: a multiple of 64 threads, LDS rows of 2 *
-    // threads (3-row rings of the T stage levels and of u0)
-    int threads = ((fa.ncol / 2 + 63) / 64) * 64;
-    size_t sh2 = (size_t)(T + 1) * 3 * 2 * threads * sizeof(double);
-    // interior steps without row tests when the buffers' byte offsets fit
-    // the 32-bit buffer-resource range
+    // threads (3-row rings of the T stage levels and of u0); interior steps
+    // without row tests when the buffers' byte offsets fit the 32-bit
+    // buffer-resource range
     fa.body = (c->fused_body && c->rstride * sizeof(double) < (size_t)0x7fff0000) ? 1 : 0;
-    if (fa.body && 2 * threads == 1024) {
-      switch (T) {
-        case 2: return launch_ex(c, kGroupFused, k_rkg2_fused2<2, 1024>, blocks, threads, sh2, fa, units, sunits);
-        case 3: return launch_ex(c, kGroupFused, k_rkg2_fused2<3, 1024>, blocks, threads, sh2, fa, units, sunits);
-        case 5: return launch_ex(c, kGroupFused, k_rkg2_fused2<5, 1024>, blocks, threads, sh2, fa, units, sunits);
-        default: return launch_ex(c, kGroupFused, k_rkg2_fused2<4, 1024>, blocks, threads, sh2, fa, units, sunits);
-      }
-    }
-    switch (T) {
-      case 2: return launch_ex(c, kGroupFused, k_rkg2_fused2<2>, blocks, threads, sh2, fa, units, sunits);
-      case 3: return launch_ex(c, kGroupFused, k_rkg2_fused2<3>, blocks, threads, sh2, fa, units, sunits);
-      case 5: return launch_ex(c, kGroupFused, k_rkg2_fused2<5>, blocks, threads, sh2, fa, units, sunits);
-      default: return launch_ex(c, kGroupFused, k_rkg2_fused2<4>, blocks, threads, sh2, fa, units, sunits);
-    }
+    return launch_fused2(c, fa, blocks, units, sunits);
   }
   if (c->fused_kernel == 2 && !store) {
     switch (T) {
@@ -1499,6 +1537,10 @@ int hft_destroy(hft_ctx *c) {
   for (auto e : c->free_ev) hipEventDestroy(e);
   if (c->coef_ev) hipEventDestroy(c->coef_ev);
   for (auto &g : c->graphs) hipGraphExecDestroy(g.second);
+  if (c->sg_exec) hipGraphExecDestroy(c->sg_exec);
+  for (void *q : {(void *)c->d_plan, (void *)c->d_slots, (void *)c->d_htab, (void *)c->d_dte})
+    if (q) hipFree(q);
+  if (c->h_plan) hipHostFree(c->h_plan);
   for (auto *vec : {&c->adj_fix, &c->adj_ck, &c->adj_w, &c->adj_st})
     for (double *q : *vec)
       if (q) hipFree(q);
@@ -1589,6 +1631,7 @@ int hft_set_flow_analytic(hft_ctx *c, const double *p0, const double *p2, const 
                           const double *p1, const double *p3, const double *p5,
                           double vp_rigid_omega, double vt_const) {
   if (!c) return fail(HFT_E_ARG, "null context");
+  c->epoch++;
   int nr = c->nr;
   auto cp = [nr](const double *s, std::vector<double> &d) {
     if (s) d.assign(s, s + nr); else d.clear();
@@ -1609,6 +1652,7 @@ int hft_set_flow_analytic(hft_ctx *c, const double *p0, const double *p2, const 
 
 int hft_set_flow_profiles(hft_ctx *c, const double *vt, const double *vp) {
   if (!c) return fail(HFT_E_ARG, "null context");
+  c->epoch++;
   if (vt) c->prof_vt.assign(vt, vt + (size_t)c->nr * c->nt); else c->prof_vt.clear();
   if (vp) c->prof_vp.assign(vp, vp + (size_t)c->nr * c->ntm); else c->prof_vp.clear();
   c->ff = false;  // profiles replace 2-D base flows
@@ -1628,6 +1672,7 @@ int hft_set_flow_profiles(hft_ctx *c, const double *vt, const double *vp) {
 // update_flow's order.  vt = vp = NULL removes the base flows.
 int hft_set_flow_arrays(hft_ctx *c, const double *vt, const double *vp, int per_realization) {
   if (!c) return fail(HFT_E_ARG, "null context");
+  c->epoch++;
   if ((vt == nullptr) != (vp == nullptr))
     return fail(HFT_E_ARG, "hft_set_flow_arrays: vt and vp must be both set or both NULL");
   int rc;
@@ -1666,6 +1711,7 @@ int hft_set_flow_arrays(hft_ctx *c, const double *vt, const double *vp, int per_
 
 int hft_set_flow_attenuation(hft_ctx *c, const double *value) {
   if (!c) return fail(HFT_E_ARG, "null context");
+  c->epoch++;
   if (!value) {
     c->atten = false;
     c->flow2d = false;
@@ -1689,6 +1735,7 @@ int hft_set_flow_attenuation(hft_ctx *c, const double *value) {
 
 int hft_update_flow(hft_ctx *c) {
   if (!c) return fail(HFT_E_ARG, "null context");
+  c->epoch++;
   if (!c->atten) return 0;
   return update_flow_2d(c);
 }
@@ -1747,6 +1794,7 @@ int hft_flow_dtmax(hft_ctx *c, int strang, double *dt, double *dt_r) {
 
 int hft_set_diffusion(hft_ctx *c, const double *eta, double factor) {
   if (!c || !eta) return fail(HFT_E_ARG, "hft_set_diffusion: null argument");
+  c->epoch++;
   int nr = c->nr, ntm = c->ntm, npm = c->npm;
   c->D.resize(nr);
   c->dt_euler_r.resize(nr);
@@ -1821,6 +1869,7 @@ static int advection_stage(hft_ctx *c, const double *x, const double *y, double 
     a.nbands = (c->ntm + a.H - 1) / a.H;
     a.ncol = ((c->nu + 63) / 64) * 64;
     a.order = order; a.rstride = c->rstride;
+    a.skip = c->adv_skip;
     int blocks = n * a.nbands;
     size_t shmem = ((size_t)17 * a.ncol + (size_t)kAdvRowVals * (a.H + 6)) * sizeof(double);
     if (c->flow2d) {
@@ -1839,6 +1888,7 @@ static int advection_stage(hft_ctx *c, const double *x, const double *y, double 
   a.x = x; a.y = y; a.out = out; a.ca = ca; a.cb = cb; a.cdt = cdt;
   a.dtr = dtr; a.cmul = cmul;
   a.order = order; a.rstride = c->rstride; a.write_aop = aop_only ? 1 : 0;
+  a.skip = c->adv_skip;
   int nb = nblocks(a.g);
   if (c->flow2d) {
     if (y) return launch(c, 1, k_advection_stage<true, true>, nb, a, u);
@@ -1936,8 +1986,20 @@ static int advection_step_impl(hft_ctx *c, double dtl) {
 typedef std::function<int(const std::vector<int> &, const std::vector<double> &,
                           const std::vector<int64_t> &, bool)>
     SuperstepFn;
+// Controller state of a step resumed part-way (after the rounds a planned
+// step ran on the device): round index to start at (0-based) and, per
+// controller, cycle number, done flag, time stepped, last cycle dt and
+// diagnostic sub-cycle count.
+struct CtlResume {
+  int round0 = 0;
+  std::vector<int> cyc;
+  std::vector<char> done;
+  std::vector<double> stepped, dt2;
+  std::vector<int64_t> sdiag;
+};
 static int diffusion_impl(hft_ctx *c, double dtl, int subcycles, hft_cycle_fn cb, void *user,
-                          const std::function<int(int)> *pre, const SuperstepFn *superstep);
+                          const std::function<int(int)> *pre, const SuperstepFn *superstep,
+                          const CtlResume *resume = nullptr);
 
 int hft_diffusion_step(hft_ctx *c, double dtl, int subcycles, hft_cycle_fn cb, void *user) {
   if (!c) return fail(HFT_E_ARG, "null context");
@@ -1949,7 +2011,8 @@ int hft_diffusion_step(hft_ctx *c, double dtl, int subcycles, hft_cycle_fn cb, v
 }
 
 static int diffusion_impl(hft_ctx *c, double dtl, int subcycles, hft_cycle_fn cb, void *user,
-                          const std::function<int(int)> *pre, const SuperstepFn *superstep) {
+                          const std::function<int(int)> *pre, const SuperstepFn *superstep,
+                          const CtlResume *resume) {
   if (!c) return fail(HFT_E_ARG, "null context");
   if (!c->diff_set) return fail(HFT_E_STATE, "hft_diffusion_step: diffusion not set");
   if (subcycles < 0) return fail(HFT_E_ARG, "hft_diffusion_step: subcycles < 0");
@@ -1968,7 +2031,17 @@ static int diffusion_impl(hft_ctx *c, double dtl, int subcycles, hft_cycle_fn cb
   std::vector<hft_cycle> recs(nr);
   std::vector<double> dt2r(nr), dtin(nr), ptl;
   std::vector<int64_t> sr(nr, 0);
-  for (int round = 1;; round++) {
+  int round1 = 1;
+  if (resume) {
+    if ((int)resume->cyc.size() != ng) return fail(HFT_E_STATE, "diffusion resume: %d controllers", ng);
+    cyc = resume->cyc;
+    done = resume->done;
+    stepped = resume->stepped;
+    dt2 = resume->dt2;
+    sdiag = resume->sdiag;
+    round1 = resume->round0 + 1;
+  }
+  for (int round = round1;; round++) {
     std::vector<int> gs, reals;
     for (int g = 0; g < ng; g++)
       if (!done[g]) gs.push_back(g);
@@ -2015,6 +2088,11 @@ static int diffusion_impl(hft_ctx *c, double dtl, int subcycles, hft_cycle_fn cb
         sr[g] = s;
         dt2r[g] = dt2[g];
       }
+    }
+    if (c->obs_smax) {
+      int64_t sm = 0;
+      for (int r : reals) sm = std::max(sm, sr[r]);
+      c->obs_smax->push_back(sm);
     }
     int rc = superstep ? (*superstep)(reals, dt2r, sr, auto_sc)
                        : rkg2_superstep(c, reals, dt2r, sr, auto_sc);
@@ -2065,7 +2143,374 @@ int hft_flux_transport_step(hft_ctx *c, double dtl, int strang, int subcycles, h
   return phase_end(c, ev, kPhaseStep, flux_transport_impl(c, dtl, strang, subcycles, cb, user));
 }
 
+// ---- device-planned steps replayed as one hipGraph --------------------------
+// A flux-transport step whose diffusion part is planned on the device
+// (k_plan_* kernels) is a fixed launch sequence: the advection stages, then
+// per round of sub-cycles the PTL pass, the controller, the coefficient
+// table, stage 1, gcap[round] fused RKG2 launches and the final copies, then
+// the second advection half-step.  The sequence is captured once per key
+// (step length, splitting, sub-cycles, setters' epoch, launch capacities)
+// and replayed with one hipGraphLaunch; the host reads back the controller
+// state once per step.  gcap[0] holds every stage count a step can need (a
+// cycle is never longer than the step, so s <= sts_stages(dt, dt_euler)).
+// By default the sequence holds one round (tuning "step_graph_rounds" = 1):
+// a step that needs more rounds is finished by the host controller from
+// the device state (bit-identical: same kernels, same arithmetic).  Extra
+// rounds usually cycle a few realizations with many stages, where empty
+// launches held in reserve cost more than the host controller's launches.
+// With step_graph_rounds = 0 the number of rounds and their launch counts
+// are learned from the stage counts of earlier steps (fixed sub-cycles:
+// every round alike, known in advance).
+constexpr int kPlanRoundsMax = 64;
+constexpr int kPlanSlotsMax = 4096;
+
+struct PlanLayout {
+  size_t stepped, dt2, sdiag, last_s, last_cycles, smax, cyc, done, status, bytes;
+};
+
+static PlanLayout plan_layout(int ng, int nr) {
+  PlanLayout L;
+  size_t o = 0;
+  L.stepped = o; o += 8 * (size_t)ng;
+  L.dt2 = o; o += 8 * (size_t)ng;
+  L.sdiag = o; o += 8 * (size_t)ng;
+  L.last_s = o; o += 8 * (size_t)nr;
+  L.last_cycles = o; o += 8 * (size_t)nr;
+  L.smax = o; o += 8 * (size_t)kPlanRoundsMax;
+  L.cyc = o; o += 4 * (size_t)ng;
+  L.done = o; o += 4 * (size_t)ng;
+  L.status = o; o += 4 * 4;
+  L.bytes = (o + 255) & ~(size_t)255;
+  return L;
+}
+
+static bool sg_eligible(const hft_ctx *c, hft_cycle_fn cb, int subcycles) {
+  return c->step_graph && !cb && c->stream && !c->timing && !c->phase_timing && c->diff_set &&
+         !sources_active(c) && !c->persist && fused_usable(c) && c->fused_kernel == 3 &&
+         c->nr <= 1024 && subcycles >= 0 && subcycles <= kPlanRoundsMax;
+}
+
+// Largest RKG2 stage count a cycle of this step can have.
+static int64_t sg_stage_bound(const hft_ctx *c, double dtl, int subcycles) {
+  double d = subcycles > 0 ? dtl / (double)subcycles : dtl;
+  if (c->mode == HFT_MODE_COUPLED) return sts_stages(d, c->dt_euler);
+  int64_t b = 0;
+  for (int r = 0; r < c->nr; r++) b = std::max(b, sts_stages(d, c->dt_euler_r[r]));
+  return b;
+}
+
+// Fused launches per round of the captured sequence (false: too many).
+static bool sg_capacity(const hft_ctx *c, int subcycles, int64_t sbound, std::vector<int> &gcap) {
+  int T = c->fused_T;
+  auto G = [&](int64_t s) { return s >= 2 ? (int)((s - 1 + T - 1) / T) : 0; };
+  int gb = G(sbound);
+  gcap.clear();
+  if (subcycles > 0) {
+    gcap.assign(subcycles, gb);
+  } else {
+    int R = 1;
+    for (size_t i = 0; i < c->sg_smax.size(); i++)
+      if (c->sg_smax[i] > 0) R = std::max(R, (int)i + 1);
+    if (c->step_graph_rounds > 0) R = std::min(R, c->step_graph_rounds);
+    R = std::min(R, kPlanRoundsMax);
+    gcap.push_back(gb);
+    for (int i = 1; i < R; i++) {
+      int g = i < (int)c->sg_smax.size() ? G(c->sg_smax[i]) : 0;
+      gcap.push_back(std::min(gb, g + std::max(1, g / 8)));
+    }
+  }
+  int tot = 0;
+  for (int g : gcap) tot += g;
+  return gb >= 1 && tot <= kPlanSlotsMax;
+}
+
+// Record the largest stage count of each round, starting at round `from`.
+static void sg_merge(hft_ctx *c, const std::vector<int64_t> &sm, int from) {
+  for (size_t i = 0; i < sm.size(); i++) {
+    size_t q = from + i;
+    if (q >= (size_t)kPlanRoundsMax) break;
+    if (c->sg_smax.size() <= q) c->sg_smax.resize(q + 1, 0);
+    c->sg_smax[q] = std::max(c->sg_smax[q], sm[i]);
+  }
+}
+
+// Device buffers of a planned step, the band-row table and the Euler limits.
+static int sg_prepare(hft_ctx *c, int64_t sbound, int gmax, int *maxtiles) {
+  int nr = c->nr, ng = c->mode == HFT_MODE_COUPLED ? 1 : nr, rc;
+  if (!c->buf2) {
+    if ((rc = dalloc(&c->buf2, c->nfield)) || (rc = dalloc(&c->buf3, c->nfield))) return rc;
+  }
+  size_t need = (size_t)(sbound + 1) * nr;
+  if (need > c->coef_cap) {
+    clear_graphs(c);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (c->d_coef) HIPCHK(hipFree(c->d_coef));
+    c->d_coef = nullptr;
+    size_t cap = std::max(need, (size_t)1024);
+    if ((rc = dalloc(&c->d_coef, cap))) return rc;
+    c->coef_cap = cap;
+  }
+  PlanLayout L = plan_layout(ng, nr);
+  if (!c->d_plan) {
+    if ((rc = dalloc(&c->d_plan, L.bytes))) return rc;
+    HIPCHK(hipMemset(c->d_plan, 0, L.bytes));
+    HIPCHK(hipHostMalloc((void **)&c->h_plan, L.bytes));
+    c->plan_bytes = L.bytes;
+    if ((rc = dalloc(&c->d_htab, (size_t)nr + 1)) || (rc = dalloc(&c->d_dte, (size_t)ng))) return rc;
+  }
+  if ((size_t)gmax * 3 > c->slots_cap) {
+    clear_graphs(c);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (c->d_slots) HIPCHK(hipFree(c->d_slots));
+    c->d_slots = nullptr;
+    size_t cap = std::max((size_t)gmax * 3, (size_t)384);
+    if ((rc = dalloc(&c->d_slots, cap))) return rc;
+    c->slots_cap = cap;
+  }
+  std::vector<int> htab(nr + 1, 1);
+  *maxtiles = 1;
+  for (int a = 1; a <= nr; a++) {
+    if ((rc = fused_rows(c, a, &htab[a]))) return rc;
+    int nb = (c->ntm + htab[a] - 1) / htab[a];
+    *maxtiles = std::max(*maxtiles, a * nb);
+  }
+  std::vector<double> dte(ng);
+  for (int g = 0; g < ng; g++) dte[g] = ng == 1 && c->mode == HFT_MODE_COUPLED ? c->dt_euler : c->dt_euler_r[g];
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipMemcpy(c->d_htab, htab.data(), (nr + 1) * sizeof(int), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_dte, dte.data(), ng * sizeof(double), hipMemcpyHostToDevice));
+  return 0;
+}
+
+static PlanArgs sg_args(hft_ctx *c, double dtl, int subcycles, int64_t sbound) {
+  int nr = c->nr;
+  bool coupled = c->mode == HFT_MODE_COUPLED;
+  int ng = coupled ? 1 : nr;
+  PlanLayout L = plan_layout(ng, nr);
+  char *b = c->d_plan;
+  PlanArgs p{};
+  p.stepped = (double *)(b + L.stepped);
+  p.dt2 = (double *)(b + L.dt2);
+  p.sdiag = (int64_t *)(b + L.sdiag);
+  p.cyc = (int *)(b + L.cyc);
+  p.done = (int *)(b + L.done);
+  p.status = (int *)(b + L.status);
+  p.smax = (int64_t *)(b + L.smax);
+  p.last_s = (int64_t *)(b + L.last_s);
+  p.last_cycles = (int64_t *)(b + L.last_cycles);
+  p.order = c->d_order;
+  p.s = c->d_s;
+  p.cdt = c->d_cdt;
+  p.ub1 = c->d_ub1;
+  p.coef = c->d_coef;
+  p.slots = c->d_slots;
+  p.htab = c->d_htab;
+  p.dte = c->d_dte;
+  p.absmax = c->d_absmax;
+  p.dtmin = c->d_dtmin;
+  p.dtl = dtl;
+  p.dt2_fixed = subcycles > 0 ? dtl / (double)subcycles : 0.0;
+  p.subcycles = subcycles;
+  p.coupled = coupled ? 1 : 0;
+  p.nr = nr;
+  p.ng = ng;
+  p.T = c->fused_T;
+  p.ntm = c->ntm;
+  p.kspan = (int)(sbound - 1);
+  return p;
+}
+
+// The launch sequence of a planned step (enqueued while the stream captures).
+static int sg_enqueue(hft_ctx *c, double dtl, int strang, int subcycles, int64_t sbound,
+                      const std::vector<int> &gcap, int maxtiles) {
+  int nr = c->nr, T = c->fused_T, rc;
+  bool adv = c->flow_set, auto_sc = subcycles == 0;
+  double h = strang ? dtl * half : dtl;
+  if (adv && (rc = advection_step_impl(c, h))) return rc;
+  PlanArgs p = sg_args(c, dtl, subcycles, sbound);
+  int *status = p.status;
+  if ((rc = launch_ex(c, 2, k_plan_init, 1, 1024, 0, p, 0.0))) return rc;
+  Tiles gall = tiles_for(c, nr);
+  int ncol = ((c->nu + 63) / 64) * 64;
+  int ncoef = nr * p.kspan;
+  int per = (int)((c->rstride / 2 + kBlock - 1) / kBlock);
+  int R = (int)gcap.size();
+  for (int round = 0; round < R; round++) {
+    p.round = round;
+    p.gcap = gcap[round];
+    if (auto_sc) {
+      // ptl_pass over the realizations still cycling
+      if ((rc = launch_ex(c, 2, k_plan_prep, 1, 1024, 0, p, 0.0))) return rc;
+      LapArgs la;
+      la.m = c->m; la.g = gall; la.x = c->f; la.y = c->dty0; la.D = c->d_D;
+      la.order = c->d_order; la.rstride = c->rstride; la.absmax = c->d_absmax; la.dn = status + 2;
+      if ((rc = launch(c, 2, k_laplacian, nblocks(gall), la, 0.0))) return rc;
+      PtlArgs pa;
+      pa.m = c->m; pa.g = gall; pa.f = c->f; pa.Af = c->dty0; pa.absmax = c->d_absmax;
+      pa.dtmin = c->d_dtmin; pa.order = c->d_order; pa.coupled = p.coupled; pa.nlaunch = nr;
+      pa.rstride = c->rstride; pa.dn = status + 2;
+      if ((rc = launch(c, 2, k_ptl_min, nblocks(gall), pa, 0.0))) return rc;
+    }
+    if ((rc = launch_ex(c, 2, k_plan_round, 1, 1024, 0, p, 0.0))) return rc;
+    if ((rc = launch_ex(c, 0, k_plan_coefs, (ncoef + kBlock - 1) / kBlock, kBlock, 0, p, 0.0))) return rc;
+    Stage1Args a1;
+    a1.m = c->m; a1.g = gall; a1.f = c->f; a1.dty0 = c->dty0; a1.u1 = c->buf1;
+    a1.D = c->d_D; a1.dt = c->d_cdt; a1.ub1 = c->d_ub1; a1.order = c->d_order;
+    a1.Af = auto_sc ? c->dty0 : nullptr; a1.rstride = c->rstride; a1.dn = status + 2;
+    if ((rc = launch(c, 0, k_rkg2_stage1, nblocks(gall), a1, 0.0))) return rc;
+    // launch q reads u_{k0-1}, u_{k0-2} from the previous launch's outputs
+    // (launch 0: u_1 from stage 1 and u_0 = f) and writes buf0/buf2 (q
+    // even) or buf1/buf3 (q odd)
+    for (int q = 0; q < gcap[round]; q++) {
+      FusedArgs fa{};
+      fa.m = c->m;
+      fa.ncol = ncol;
+      fa.H = 1;
+      fa.nbands = 1;
+      fa.k0 = 2 + q * T;
+      fa.A = q == 0 ? c->buf1 : ((q & 1) ? c->buf0 : c->buf1);
+      fa.B = q == 0 ? c->f : ((q & 1) ? c->buf2 : c->buf3);
+      fa.outA = (q & 1) ? c->buf1 : c->buf0;
+      fa.outB = (q & 1) ? c->buf3 : c->buf2;
+      fa.f = c->f; fa.dty0 = c->dty0; fa.D = c->d_D; fa.coef = c->d_coef; fa.nr_table = nr;
+      fa.s = c->d_s; fa.order = c->d_order; fa.rstride = c->rstride; fa.cdt = c->d_cdt;
+      fa.body = (c->fused_body && c->rstride * sizeof(double) < (size_t)0x7fff0000) ? 1 : 0;
+      fa.slot = c->d_slots + 3 * q;
+      if ((rc = launch_fused2(c, fa, maxtiles, 0.0, 0.0))) return rc;
+    }
+    PlanCopyArgs ca;
+    ca.dst = c->f; ca.src[0] = c->buf0; ca.src[1] = c->buf1; ca.order = c->d_order;
+    ca.n = status + 2; ca.s = c->d_s; ca.rstride = c->rstride; ca.per = per; ca.T = T;
+    if ((rc = launch(c, 0, k_copy_finals_plan, nr * per, ca, 0.0))) return rc;
+  }
+  p.round = R;
+  if ((rc = launch_ex(c, 2, k_plan_end, 1, 64, 0, p, 0.0))) return rc;
+  if (adv && strang) {
+    // the second half-step does nothing when the diffusion step overflowed
+    c->adv_skip = status;
+    rc = advection_step_impl(c, h);
+    c->adv_skip = nullptr;
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+// Run the step as a replayed planned sequence when possible (*ran = true).
+static int sg_step(hft_ctx *c, double dtl, int strang, int subcycles, bool *ran) {
+  *ran = false;
+  int nr = c->nr, rc;
+  bool coupled = c->mode == HFT_MODE_COUPLED, adv = c->flow_set;
+  int ng = coupled ? 1 : nr;
+  double h = strang ? dtl * half : dtl;
+  if (adv) {
+    // per-realization CFL sub-steps vary the advection launches: not planned
+    std::vector<int> nsub;
+    int nmax = 1;
+    if ((rc = advection_substeps(c, h, nsub, nmax))) return rc;
+    if (nmax != 1) return 0;
+  }
+  int64_t dbits;
+  memcpy(&dbits, &dtl, sizeof(dbits));
+  std::vector<int64_t> base = {dbits, strang, subcycles, c->epoch, c->flow_gen, adv ? 1 : 0};
+  // with more than one round captured, the rounds' stage counts of a new
+  // kind of step are learned from a host-controlled step first
+  bool learn = subcycles == 0 && c->step_graph_rounds != 1;
+  if (base != c->sg_prev) {
+    c->sg_prev = base;
+    c->sg_smax.clear();
+    if (learn) return 0;
+  }
+  if (learn && c->sg_smax.empty()) return 0;
+  int64_t sbound = sg_stage_bound(c, dtl, subcycles);
+  std::vector<int> gcap;
+  if (!sg_capacity(c, subcycles, sbound, gcap)) return 0;
+  std::vector<int64_t> key = base;
+  key.push_back(sbound);
+  key.push_back(c->fused_T);
+  for (int g : gcap) key.push_back(g);
+  if (!c->sg_exec || key != c->sg_key) {
+    int gmax = *std::max_element(gcap.begin(), gcap.end()), maxtiles = 1;
+    if ((rc = sg_prepare(c, sbound, gmax, &maxtiles))) return rc;
+    clear_graphs(c);
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int rcl = sg_enqueue(c, dtl, strang, subcycles, sbound, gcap, maxtiles);
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(c->stream, &graph);
+    c->adv_skip = nullptr;
+    if (rcl) {
+      if (graph) hipGraphDestroy(graph);
+      return rcl;
+    }
+    if (e != hipSuccess) return fail(HFT_E_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    hipGraphExec_t exec = nullptr;
+    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    if (e != hipSuccess) return fail(HFT_E_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    c->sg_exec = exec;
+    c->sg_key = key;
+    c->sg_builds++;
+  } else {
+    c->sg_hits++;
+  }
+  HIPCHK(hipGraphLaunch(c->sg_exec, c->stream));
+  *ran = true;
+  PlanLayout L = plan_layout(ng, nr);
+  HIPCHK(hipMemcpyAsync(c->h_plan, c->d_plan, L.bytes, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  const char *hb = c->h_plan;
+  const int *st = (const int *)(hb + L.status);
+  memcpy(c->last_s.data(), hb + L.last_s, nr * sizeof(int64_t));
+  memcpy(c->last_cycles.data(), hb + L.last_cycles, nr * sizeof(int64_t));
+  int R = (int)gcap.size(), ovf = st[0], c0 = st[1];
+  int nread = ovf ? std::min(c0 + 1, R) : R;
+  const int64_t *sm = (const int64_t *)(hb + L.smax);
+  sg_merge(c, std::vector<int64_t>(sm, sm + nread), 0);
+  if (!ovf) return 0;
+  // finish the step on the host from the controller state the device kept
+  c->sg_overflows++;
+  if (c0 < 0 || c0 > R) return fail(HFT_E_STATE, "planned step: bad resume round %d", c0);
+  CtlResume res;
+  res.round0 = c0;
+  const double *stp = (const double *)(hb + L.stepped), *d2 = (const double *)(hb + L.dt2);
+  const int64_t *sd = (const int64_t *)(hb + L.sdiag);
+  const int *cy = (const int *)(hb + L.cyc), *dn = (const int *)(hb + L.done);
+  res.stepped.assign(stp, stp + ng);
+  res.dt2.assign(d2, d2 + ng);
+  res.sdiag.assign(sd, sd + ng);
+  res.cyc.assign(cy, cy + ng);
+  res.done.resize(ng);
+  for (int g = 0; g < ng; g++) res.done[g] = dn[g] ? 1 : 0;
+  std::vector<int64_t> obs;
+  c->obs_smax = &obs;
+  rc = diffusion_impl(c, dtl, subcycles, nullptr, nullptr, nullptr, nullptr, &res);
+  c->obs_smax = nullptr;
+  if (rc) return rc;
+  sg_merge(c, obs, c0);
+  if (adv && strang && (rc = advection_step_impl(c, h))) return rc;
+  return 0;
+}
+
+static int flux_transport_host(hft_ctx *c, double dtl, int strang, int subcycles,
+                               hft_cycle_fn cb, void *user);
+
 static int flux_transport_impl(hft_ctx *c, double dtl, int strang, int subcycles,
+                               hft_cycle_fn cb, void *user) {
+  bool learn = false;
+  if (c->diff_set && sg_eligible(c, cb, subcycles)) {
+    bool ran = false;
+    int rc = sg_step(c, dtl, strang, subcycles, &ran);
+    if (rc || ran) return rc;
+    learn = subcycles == 0 && c->step_graph_rounds != 1;
+  }
+  std::vector<int64_t> obs;
+  if (learn) c->obs_smax = &obs;
+  int rc = flux_transport_host(c, dtl, strang, subcycles, cb, user);
+  c->obs_smax = nullptr;
+  if (!rc && learn) sg_merge(c, obs, 0);
+  return rc;
+}
+
+static int flux_transport_host(hft_ctx *c, double dtl, int strang, int subcycles,
                                hft_cycle_fn cb, void *user) {
   int rc;
   bool adv = c->flow_set, dif = c->diff_set, src = sources_active(c);
@@ -2437,6 +2882,7 @@ int hft_analysis_noalloc(hft_ctx *c, const double *f_dev, int64_t ld, double *ou
 
 int hft_set_weno_eps(hft_ctx *c, double eps) {
   if (!c) return fail(HFT_E_ARG, "null context");
+  c->epoch++;
   if (!(eps == eps)) return fail(HFT_E_ARG, "hft_set_weno_eps: eps is NaN");
   // load_weno, src/hipft.f90:6895-6901
   for (int j = 1; j <= c->ntm; j++) c->wt[6][j] = eps <= zero ? c->dt[j] : eps;
@@ -2497,6 +2943,7 @@ int hft_stream(hft_ctx *c, void **stream) {
 int hft_set_tuning(hft_ctx *c, const char *key, int value) {
   if (!c || !key) return fail(HFT_E_ARG, "hft_set_tuning: null argument");
   clear_graphs(c);  // captured launch sequences depend on the tuning
+  c->epoch++;
   if (!strcmp(key, "fused")) c->fused = value != 0;
   else if (!strcmp(key, "fused_stages")) {
     if (value < 2 || value > 5) return fail(HFT_E_ARG, "fused_stages must be 2..5 (5: fused_kernel 3 only)");
@@ -2543,6 +2990,13 @@ int hft_set_tuning(hft_ctx *c, const char *key, int value) {
     c->persist_long = value;
   } else if (!strcmp(key, "graphs")) {
     c->graphs_on = value != 0;
+  } else if (!strcmp(key, "step_graph")) {
+    if (value < 0 || value > 1) return fail(HFT_E_ARG, "step_graph must be 0 or 1");
+    c->step_graph = value;
+  } else if (!strcmp(key, "step_graph_rounds")) {
+    if (value < 0 || value > kPlanRoundsMax)
+      return fail(HFT_E_ARG, "step_graph_rounds must be 0..%d", kPlanRoundsMax);
+    c->step_graph_rounds = value;
   } else if (!strcmp(key, "adjoint_segment")) {
     if (value < 0) return fail(HFT_E_ARG, "adjoint_segment must be >= 0");
     c->adj_seg = value;
@@ -2590,6 +3044,11 @@ int hft_get_tuning(hft_ctx *c, const char *key, int *value) {
   else if (!strcmp(key, "graphs")) *value = c->graphs_on;
   else if (!strcmp(key, "graph_hits")) *value = (int)c->graph_hits;
   else if (!strcmp(key, "graph_builds")) *value = (int)c->graph_builds;
+  else if (!strcmp(key, "step_graph")) *value = c->step_graph;
+  else if (!strcmp(key, "step_graph_rounds")) *value = c->step_graph_rounds;
+  else if (!strcmp(key, "step_graph_hits")) *value = (int)c->sg_hits;
+  else if (!strcmp(key, "step_graph_builds")) *value = (int)c->sg_builds;
+  else if (!strcmp(key, "step_graph_overflows")) *value = (int)c->sg_overflows;
   else if (!strcmp(key, "adjoint_segment")) *value = c->adj_seg;
   else if (!strcmp(key, "adjoint_mem_mb")) *value = (int)c->adj_mem_mb;
   else if (!strcmp(key, "adjoint_band_rows")) *value = c->adj_band_rows;

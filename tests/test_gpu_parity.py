@@ -690,7 +690,8 @@ def test_graph_replay_matches_direct_launches(gpu_available, fused):
         g.set_diffusion(eta)
         g.set_flow_profiles(vt, None)
         g.set_flow_analytic(*W.dr_coefficients())
-        g.set_tuning(fused=fused, fused_min_realizations=1, graphs=graphs)
+        # RKG2 super-step graphs (the host-controlled step; no planned step)
+        g.set_tuning(fused=fused, fused_min_realizations=1, graphs=graphs, step_graph=0)
         states = []
         for i in range(6):
             g.flux_transport_step(1.0, strang=True, subcycles=2 if i < 4 else 0)
@@ -720,7 +721,7 @@ def test_graph_replay_after_coef_table_growth(gpu_available):
         g = FluxTransport(t, p, nr, mode="independent")
         g.set_state(maps, layout="pt")
         g.set_diffusion(eta)
-        g.set_tuning(fused_min_realizations=1, graphs=graphs)
+        g.set_tuning(fused_min_realizations=1, graphs=graphs, step_graph=0)
         states = []
         for dt in dts:
             g.diffusion_step(dt, subcycles=1)
