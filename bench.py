@@ -307,7 +307,9 @@ def stream_ceiling(nbytes=2 << 30, reps=10):
 
 
 def pmc_probe(m, nbytes=256 << 20):
-    """Counter calibration copy of a known size, then one workload step."""
+    """Counter calibration copy of a known size, then one workload step.  The
+    step is host-controlled, the launch sequence whose per-launch times the
+    roofline's event pass measures."""
     import ctypes
     hip = ctypes.CDLL("libamdhip64.so")
     a, b = ctypes.c_void_p(), ctypes.c_void_p()
@@ -319,6 +321,7 @@ def pmc_probe(m, nbytes=256 << 20):
     assert hip.hipDeviceSynchronize() == 0
     hip.hipFree(a)
     hip.hipFree(b)
+    m.set_tuning(step_graph=0)
     step(m)
     m.synchronize()
     print(json.dumps({"pmc_probe": "done", "copy_bytes": nbytes}), flush=True)

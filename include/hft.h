@@ -278,7 +278,21 @@ int hft_stream(hft_ctx *ctx, void **stream);
  *                            realizations have the same stage counts (0);
  *                            "graph_hits", "graph_builds" (read only) count
  *                            replays and captures
- *   "adjoint_segment"        RKG2 stages per recomputed segment of the
+ *   "step_graph"             1 = plan hft_flux_transport_step on the device
+ *                            (PTL step rule, stage counts and coefficient
+ *                            tables of diffusion_step / load_sts_rkg2,
+ *                            src/hipft.f90:4589-4690, 5399-5461) and replay the
+ *                            whole step as one captured hipGraph (default);
+ *                            not used with a cycle callback, timing, sources,
+ *                            per-realization CFL sub-steps, the null stream
+ *                            or more than 1024 realizations
+ *   "step_graph_rounds"      sub-cycle rounds the captured step holds (1); a
+ *                            step needing more is finished by the host
+ *                            controller; 0 = learned from earlier steps.
+ *                            "step_graph_builds", "step_graph_hits",
+ *                            "step_graph_overflows" (read only) count
+ *                            captures, replays and steps finished on the host
+ *   "adjoint_segment"       RKG2 stages per recomputed segment of the
  *                            adjoint (0 = all stages of a cycle when they fit
  *                            in "adjoint_mem_mb", else sqrt(s))
  *   "adjoint_mem_mb"         device memory for stored RKG2 stages (32768)
