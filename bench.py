@@ -177,6 +177,15 @@ def adjoint_bench(nr=16, ntm=256, npm1=512, nsteps=200, every=20, seed=77):
         run(f0, eta, v0, om, model, nsteps, checkpoint_every=nsteps)
         torch.cuda.synchronize()
         fwd = time.perf_counter() - t0
+        # the production forward: PTL sub-cycling (the differentiated step
+        # takes one RKG2 cycle per step, whose length does not depend on the
+        # state)
+        run(f0, eta, v0, om, model, 2, checkpoint_every=nsteps, subcycles=0)  # warm
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        run(f0, eta, v0, om, model, nsteps, checkpoint_every=nsteps, subcycles=0)
+        torch.cuda.synchronize()
+        fwd_ptl = time.perf_counter() - t0
     for rep in range(2):  # the first pass allocates checkpoints and adjoint buffers
         x = f0.clone().requires_grad_(True)
         e, v, o = (a.clone().requires_grad_(True) for a in (eta, v0, om))
@@ -197,6 +206,8 @@ def adjoint_bench(nr=16, ntm=256, npm1=512, nsteps=200, every=20, seed=77):
                 backward_s=round(t2 - t1, 4),
                 adjoint_over_forward=round((t2 - t1) / fwd, 3),
                 gradient_over_forward=round((t2 - t0) / fwd, 3),
+                forward_ptl_s=round(fwd_ptl, 4),
+                gradient_over_forward_ptl=round((t2 - t0) / fwd_ptl, 3),
                 grad_norms=dict(f0=float(x.grad.norm()), eta=float(e.grad.norm()),
                                 v0=float(v.grad.norm()), omega=float(o.grad.norm())))
 

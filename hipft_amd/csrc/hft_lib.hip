@@ -1631,7 +1631,6 @@ int hft_set_flow_analytic(hft_ctx *c, const double *p0, const double *p2, const 
                           const double *p1, const double *p3, const double *p5,
                           double vp_rigid_omega, double vt_const) {
   if (!c) return fail(HFT_E_ARG, "null context");
-  c->epoch++;
   int nr = c->nr;
   auto cp = [nr](const double *s, std::vector<double> &d) {
     if (s) d.assign(s, s + nr); else d.clear();
@@ -1652,7 +1651,6 @@ int hft_set_flow_analytic(hft_ctx *c, const double *p0, const double *p2, const 
 
 int hft_set_flow_profiles(hft_ctx *c, const double *vt, const double *vp) {
   if (!c) return fail(HFT_E_ARG, "null context");
-  c->epoch++;
   if (vt) c->prof_vt.assign(vt, vt + (size_t)c->nr * c->nt); else c->prof_vt.clear();
   if (vp) c->prof_vp.assign(vp, vp + (size_t)c->nr * c->ntm); else c->prof_vp.clear();
   c->ff = false;  // profiles replace 2-D base flows
@@ -1672,7 +1670,6 @@ int hft_set_flow_profiles(hft_ctx *c, const double *vt, const double *vp) {
 // update_flow's order.  vt = vp = NULL removes the base flows.
 int hft_set_flow_arrays(hft_ctx *c, const double *vt, const double *vp, int per_realization) {
   if (!c) return fail(HFT_E_ARG, "null context");
-  c->epoch++;
   if ((vt == nullptr) != (vp == nullptr))
     return fail(HFT_E_ARG, "hft_set_flow_arrays: vt and vp must be both set or both NULL");
   int rc;
@@ -1711,7 +1708,6 @@ int hft_set_flow_arrays(hft_ctx *c, const double *vt, const double *vp, int per_
 
 int hft_set_flow_attenuation(hft_ctx *c, const double *value) {
   if (!c) return fail(HFT_E_ARG, "null context");
-  c->epoch++;
   if (!value) {
     c->atten = false;
     c->flow2d = false;
@@ -1735,7 +1731,6 @@ int hft_set_flow_attenuation(hft_ctx *c, const double *value) {
 
 int hft_update_flow(hft_ctx *c) {
   if (!c) return fail(HFT_E_ARG, "null context");
-  c->epoch++;
   if (!c->atten) return 0;
   return update_flow_2d(c);
 }
@@ -2411,7 +2406,21 @@ static int sg_step(hft_ctx *c, double dtl, int strang, int subcycles, bool *ran)
   }
   int64_t dbits;
   memcpy(&dbits, &dtl, sizeof(dbits));
-  std::vector<int64_t> base = {dbits, strang, subcycles, c->epoch, c->flow_gen, adv ? 1 : 0};
+  std::vector<int64_t> base = {dbits, strang, subcycles, c->epoch, adv ? 1 : 0, c->flow2d ? 1 : 0};
+  // device arrays and mesh the captured launches read (flow and diffusivity
+  // values are rewritten in place between steps, their arrays stay)
+  for (const void *q : {(const void *)c->f, (const void *)c->buf0, (const void *)c->buf1,
+                        (const void *)c->dty0, (const void *)c->d_D, (const void *)c->d_vt1,
+                        (const void *)c->d_vp1, (const void *)c->d_at1, (const void *)c->d_ap1,
+                        (const void *)c->d_vt2, (const void *)c->d_vp2, (const void *)c->d_at2,
+                        (const void *)c->d_ap2, (const void *)c->d_order_id})
+    base.push_back((int64_t)(intptr_t)q);
+  {
+    uint64_t hsh = 1469598103934665603ull;  // FNV-1a over the mesh descriptor
+    const unsigned char *mb = reinterpret_cast<const unsigned char *>(&c->m);
+    for (size_t q = 0; q < sizeof(Mesh); q++) hsh = (hsh ^ mb[q]) * 1099511628211ull;
+    base.push_back((int64_t)hsh);
+  }
   // with more than one round captured, the rounds' stage counts of a new
   // kind of step are learned from a host-controlled step first
   bool learn = subcycles == 0 && c->step_graph_rounds != 1;

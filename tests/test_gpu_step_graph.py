@@ -185,3 +185,45 @@ def test_planned_step_benchmark_batch(gpu_available):
     assert g.get_tuning("step_graph_hits") == 3
     h.close()
     g.close()
+
+
+def test_planned_step_attenuated_flows_updated_each_step(gpu_available):
+    """Flows attenuated by |Br| (2-D flow arrays) recomputed from the state
+    before every step, as the driver does: the captured step reads the flow
+    arrays in place, so it is captured once and replayed, bit for bit with
+    host-controlled steps (tanh of the attenuation is the same device code on
+    both paths)."""
+    t, p = grid(91, 181)
+    eta = np.array([200.0, 600.0])
+    maps = maps_for(t, p, 2, 340, noise=0.0)
+    h = model(t, p, maps, eta, "independent", 0)
+    g = model(t, p, maps, eta, "independent", 1)
+    for m in (h, g):
+        m.set_flow_attenuation([300.0, 150.0])
+    for i in range(4):
+        for m in (h, g):
+            m.update_flow()
+        h.flux_transport_step(1.0, strang=True, subcycles=0)
+        g.flux_transport_step(1.0, strang=True, subcycles=0)
+        same(g.get_state("tp"), h.get_state("tp"), f"state after step {i + 1}")
+    assert g.get_tuning("step_graph_builds") == 1
+    assert g.get_tuning("step_graph_hits") == 3
+    h.close()
+    g.close()
+
+
+def test_planned_step_declines_cfl_substeps(gpu_available):
+    """A step whose advection takes per-realization CFL sub-steps (here a
+    24 h step) is not planned: it runs host-controlled, with the same
+    results."""
+    t, p = grid(91, 181)
+    eta = np.array([200.0, 600.0, 900.0])
+    maps = maps_for(t, p, 3, 350, noise=0.0)
+    h = model(t, p, maps, eta, "independent", 0)
+    g = model(t, p, maps, eta, "independent", 1)
+    nsub = np.ceil(12.0 / g.flow_dtmax(False)[1])  # the Strang half-step of 12 h
+    assert nsub.max() > 1, nsub
+    lockstep(h, g, 2, 24.0)
+    assert g.get_tuning("step_graph_builds") == 0
+    h.close()
+    g.close()
