@@ -123,6 +123,7 @@ struct hft_ctx {
   // co-resident blocks (fused_band_count); 0: bands of band_rows rows
   int fused_bands = 0;
   int fused_band_share = 32;  // percent: slowdown of a round from an idle to a full device
+  int fused_band_min_rows = 8;  // thinnest band fused_band_count may choose
   int fused_cap[6] = {0, 0, 0, 0, 0, 0};  // co-resident k_rkg2_fused2<T> blocks on the device
   int fused_kernel = 3;  // 1: k_rkg2_fused, 2: k_rkg2_march, 3: k_rkg2_fused2 (2 columns/thread)
   int fused_body = 1;    // k_rkg2_fused2: interior march steps without row tests
@@ -940,15 +941,15 @@ int fused2_capacity(hft_ctx *c, int threads, size_t shmem, int *cap) {
 // by kShare (about a third; tuning key "fused_band_share" in percent) from
 // an idle to a full device, with the blocks sharing the memory system
 // (measured on 512 x 1024 bands of 32-128 rows).  Of the band counts
-// whose bands have 16..(kFusedMaxRows - 2T) rows, the one with the least
+// whose bands have hmin..(kFusedMaxRows - 2T) rows, the one with the least
 // modelled time; a launch whose last round is nearly empty (active * nbands
 // just above a multiple of cap) moves to fewer, taller bands or to more,
 // shorter ones that fill the rounds.
-int fused_band_count(int ntm, int T, int active, int cap, double kShare) {
+int fused_band_count(int ntm, int T, int active, int cap, double kShare, int hmin) {
   int hmax = kFusedMaxRows - 2 * T;
   int best = (ntm + hmax - 1) / hmax;
   double best_t = 1e300;
-  for (int nb = std::max(1, (ntm + hmax - 1) / hmax); nb <= std::max(1, ntm / 16); nb++) {
+  for (int nb = std::max(1, (ntm + hmax - 1) / hmax); nb <= std::max(1, ntm / hmin); nb++) {
     int H = (ntm + nb - 1) / nb;
     if (H > hmax) continue;
     int64_t blocks = (int64_t)active * nb;
@@ -985,7 +986,8 @@ int fused_rows(hft_ctx *c, int active, int *H) {
       default: rc = fused2_capacity<4>(c, threads, sh2, &cap); break;
     }
     if (rc) return rc;
-    int nb = fused_band_count(c->ntm, T, active, cap, 0.01 * c->fused_band_share);
+    int nb = fused_band_count(c->ntm, T, active, cap, 0.01 * c->fused_band_share,
+                              c->fused_band_min_rows);
     h = (c->ntm + nb - 1) / nb;
   }
   *H = h;
@@ -2979,6 +2981,9 @@ int hft_set_tuning(hft_ctx *c, const char *key, int value) {
   } else if (!strcmp(key, "fused_bands")) {
     if (value < 0 || value > 1) return fail(HFT_E_ARG, "fused_bands must be 0 or 1");
     c->fused_bands = value;
+  } else if (!strcmp(key, "fused_band_min_rows")) {
+    if (value < 4 || value > 64) return fail(HFT_E_ARG, "fused_band_min_rows must be 4..64");
+    c->fused_band_min_rows = value;
   } else if (!strcmp(key, "fused_band_share")) {
     if (value < 0 || value > 200) return fail(HFT_E_ARG, "fused_band_share must be 0..200");
     c->fused_band_share = value;
@@ -3045,6 +3050,7 @@ int hft_get_tuning(hft_ctx *c, const char *key, int *value) {
   else if (!strcmp(key, "fused_body")) *value = c->fused_body;
   else if (!strcmp(key, "weno_fastdiv")) *value = c->weno_fastdiv;
   else if (!strcmp(key, "fused_band_share")) *value = c->fused_band_share;
+  else if (!strcmp(key, "fused_band_min_rows")) *value = c->fused_band_min_rows;
   else if (!strcmp(key, "fused_kernel")) *value = c->fused_kernel;
   else if (!strcmp(key, "persist")) *value = c->persist;
   else if (!strcmp(key, "persist_rows")) *value = c->persist_rows;

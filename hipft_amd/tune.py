@@ -24,12 +24,12 @@ CACHE = os.path.join(HERE, "tuning_cache.json")
 
 DIFF_KEYS = ("fused", "fused_stages", "band_rows", "fused_min_realizations",
              "fused_target_blocks", "fused_kernel", "fused_bands", "fused_band_share",
-             "fused_body")
+             "fused_body", "fused_band_min_rows")
 ADV_KEYS = ("advection_band", "advection_band_rows")
 KEYS = DIFF_KEYS + ADV_KEYS
 DEFAULTS = dict(fused=1, fused_stages=4, band_rows=32, fused_min_realizations=8,
                 fused_target_blocks=0, fused_kernel=3, fused_bands=0, fused_band_share=32,
-                fused_body=1, advection_band=1, advection_band_rows=16)
+                fused_body=1, fused_band_min_rows=8, advection_band=1, advection_band_rows=16)
 
 
 def adv_candidates():

@@ -260,13 +260,18 @@ int hft_stream(hft_ctx *ctx, void **stream);
  *                            thread, 2 = the same with lean bookkeeping,
  *                            3 = two phi columns per thread (default)
  *   "fused_bands"            1 = each fused launch of kernel 3 picks its band
- *                            count (bands of 16..(160 - 2 x stages) rows) so
+ *                            count (bands of fused_band_min_rows..(160 - 2 x
+ *                            stages) rows) so
  *                            that its blocks fill whole rounds of the blocks
  *                            the device holds at once; band_rows and
  *                            fused_target_blocks are then unused (0)
  *   "fused_band_share"       that choice's model of how much longer a round
  *                            takes on a full device than on an idle one, in
  *                            percent (32)
+ *   "fused_band_min_rows"    the thinnest band that choice may pick (8; 4..64):
+ *                            a launch over few realizations is bound by the
+ *                            latency of its march, so thinner bands finish
+ *                            sooner
  *   "advection_band"         1 = WENO3 stages by theta bands computing each
  *                            face once (default); 0 = one thread per point
  *   "advection_band_rows"    theta rows per block of the banded WENO3 (16)
