@@ -63,7 +63,8 @@ def candidates(nr):
                                 fused_target_blocks=0, fused_kernel=3, fused_bands=1,
                                 fused_band_share=share))
     # interior march steps without row tests: same results, never slower
-    return [dict({"fused_band_share": DEFAULTS["fused_band_share"], "fused_body": 1}, **c)
+    return [dict({"fused_band_share": DEFAULTS["fused_band_share"], "fused_body": 1,
+                  "fused_band_min_rows": DEFAULTS["fused_band_min_rows"]}, **c)
             for c in out]
 
 
