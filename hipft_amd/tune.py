@@ -68,6 +68,14 @@ def candidates(nr):
             for c in out]
 
 
+def tail_candidates():
+    """Switch to one-stage launches below m realizations, bands down to h
+    rows: the launches late in a super-step and in extra PTL rounds, which
+    only whole steps of an evolving state exercise."""
+    return [dict(fused_min_realizations=m, fused_band_min_rows=h)
+            for m in (1, 2, 4, 8) for h in (4, 8, 16)]
+
+
 def cache_key(model):
     # libhft is built for gfx950 only; the marketing name that HIP reports
     # varies with how the runtime was initialised, so it is not part of the key
@@ -99,11 +107,13 @@ def nearest_cached(model, cache):
 
 
 def autotune(model, dt=1.0, reps=3, use_cache=True, write_cache=True, cands=None, log=None,
-             borrow=False):
+             borrow=False, step_runs=8):
     """Pick and apply the fastest launch configuration for this batch.
     borrow: with no cached choice for this batch size, apply the one cached
     for the nearest batch size on the same grid instead of measuring
-    (results do not depend on the choice, only the speed does)."""
+    (results do not depend on the choice, only the speed does).
+    step_runs: Strang steps per measurement of the last stage (the tail
+    knobs, timed on whole steps from the batch's state); 0 skips it."""
     key = cache_key(model)
     cache = load_cache()
     if use_cache and key in cache:
@@ -142,8 +152,17 @@ def autotune(model, dt=1.0, reps=3, use_cache=True, write_cache=True, cands=None
                              lambda: model.advection_step(0.5 * dt), "advection")
         best = adv
         best["ms_per_advection_step"] = round(1e3 * adv_t, 3)
-    model.set_state(state, layout="tp")
     best["ms_per_diffusion_step"] = round(1e3 * best_t, 3)
+    if step_runs > 0 and best.get("fused") and best.get("fused_kernel") == 3 and \
+            best.get("fused_bands"):
+        def steps():
+            for _ in range(step_runs):
+                model.flux_transport_step(dt, strang=True, subcycles=0)
+        keep = {k: v for k, v in best.items() if k.startswith("ms_per_")}
+        tail, tail_t = measure([dict(best, **c) for c in tail_candidates()], steps, "steps")
+        best = dict(tail, **keep)
+        best["ms_per_flux_transport_step"] = round(1e3 * tail_t / step_runs, 3)
+    model.set_state(state, layout="tp")
     apply(model, best)
     if write_cache:
         cache = load_cache()

@@ -45,6 +45,13 @@ class FakeModel:
         import time
         time.sleep(1e-4 * (2.0 if self.cfg["advection_band"] == 0 else 1.0))
 
+    def flux_transport_step(self, dt, strang, subcycles):
+        import time
+        self.steps = getattr(self, "steps", 0) + 1
+        time.sleep(1e-4 * self.step_cost(self.cfg))
+
+    step_cost = staticmethod(lambda c: 1.0)
+
 
 def test_candidates_cover_knobs():
     c1 = tune.candidates(1)
@@ -80,6 +87,25 @@ def test_autotune_picks_fastest_and_caches(tmp_path, monkeypatch):
     m2 = FakeModel(nr=4, cost=cost)
     again = tune.autotune(m2, use_cache=True)
     assert again == cache[tune.cache_key(m)] and m2.calls == 0
+
+
+def test_autotune_tail_stage_times_whole_steps(tmp_path, monkeypatch):
+    """With the band-count kernel chosen, a last stage times whole steps for
+    the tail knobs (one-stage threshold, thinnest band) and keeps the
+    diffusion and advection choices."""
+    monkeypatch.setattr(tune, "CACHE", str(tmp_path / "cache.json"))
+    best_cfg = dict(fused=1, fused_stages=4, fused_kernel=3, fused_bands=1,
+                    fused_min_realizations=4, fused_band_share=16)
+    cost = lambda c: 0.2 if all(c.get(k) == v for k, v in best_cfg.items()) else 3.0
+    m = FakeModel(nr=64, cost=cost)
+    m.step_cost = lambda c: 0.3 if (c["fused_min_realizations"], c["fused_band_min_rows"]) == \
+        (2, 4) else 2.0
+    got = tune.autotune(m, reps=1, use_cache=False, step_runs=2)
+    assert (got["fused_min_realizations"], got["fused_band_min_rows"]) == (2, 4)
+    assert all(got[k] == v for k, v in best_cfg.items() if k != "fused_min_realizations")
+    assert got["advection_band"] == 1
+    assert m.steps == 2 * 2 * len(tune.tail_candidates())  # warm-up run + one timed run each
+    assert "ms_per_flux_transport_step" in got and "ms_per_diffusion_step" in got
 
 
 def test_bench_refuses_more_than_one_gpu():
