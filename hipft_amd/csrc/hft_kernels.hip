@@ -2262,7 +2262,6 @@ __device__ inline void block_reduce(double *sh, double v, int op, double *out) {
 }
 
 __global__ __launch_bounds__(kBlock) void k_analysis_part(AnaArgs a) {
-  __shared__ double sh[kBlock];
   int r = blockIdx.y, b = blockIdx.x;
   const double huge = 1.7976931348623157e308, tiny = 2.2250738585072014e-308;
   double mn = huge, mx = tiny, mnabs = huge, verr = 0.0, sfv = 0.0;
@@ -2308,8 +2307,30 @@ __global__ __launch_bounds__(kBlock) void k_analysis_part(AnaArgs a) {
   }
   double *o = a.part + ((size_t)r * a.nblk + b) * kNPart;
   double vals[kNPart] = {mn, mx, mnabs, verr, sfv, fp, fm, fppn, fmpn, fpps, fmps, apn, aps, e1, e2, ax};
-  int ops[kNPart] = {1, 2, 1, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-  for (int q = 0; q < kNPart; q++) block_reduce(sh, vals[q], ops[q], o + q);
+  // every partial reduced across the wavefront by a fixed butterfly, then
+  // the block's wavefronts combined in order by one thread per partial (one
+  // barrier for all sixteen instead of a shared-memory tree each)
+  __shared__ double wpart[kBlock / 64][kNPart];
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < kNPart; q++) {
+    double v = vals[q];
+    for (int h = 32; h >= 1; h >>= 1) {
+      double u = __shfl_xor(v, h, 64);
+      v = q == 0 || q == 2 ? fmin(v, u) : (q == 1 ? fmax(v, u) : v + u);
+    }
+    if (lane == 0) wpart[w][q] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < kNPart) {
+    int q = threadIdx.x;
+    double v = wpart[0][q];
+    for (int k = 1; k < kBlock / 64; k++) {
+      double u = wpart[k][q];
+      v = q == 0 || q == 2 ? fmin(v, u) : (q == 1 ? fmax(v, u) : v + u);
+    }
+    o[q] = v;
+  }
 }
 
 __global__ void k_analysis_final(const double *part, int nblk, int nr, int ntm, int npm,
