@@ -292,12 +292,14 @@ def cfg5_bench(days=30, nr=128, ntm=512, npm1=1024, ckpt=24, adjoint_mem_gb=0):
 
 def stream_ceiling(nbytes=2 << 30, reps=10):
     """Achievable HBM bandwidth of plain streaming kernels on this device
-    (PyTorch's copy and add over 2 GiB FP64 arrays): the practical ceiling
-    beside the published 8 TB/s peak."""
+    (PyTorch's copy, add and addcmul over 2 GiB FP64 arrays: 1, 2 and 3
+    reads per write), the practical ceiling beside the published 8 TB/s
+    peak."""
     import torch
     n = nbytes // 8
     a = torch.rand(n, dtype=torch.float64, device="cuda")
     b = torch.rand(n, dtype=torch.float64, device="cuda")
+    d = torch.rand(n, dtype=torch.float64, device="cuda")
     c = torch.empty_like(a)
 
     def rate(f, moved):
@@ -311,8 +313,9 @@ def stream_ceiling(nbytes=2 << 30, reps=10):
 
     out = dict(copy_1r1w_GBs=rate(lambda: c.copy_(a), 2 * nbytes),
                add_2r1w_GBs=rate(lambda: torch.add(a, b, out=c), 3 * nbytes),
+               addcmul_3r1w_GBs=rate(lambda: torch.addcmul(a, b, d, out=c), 4 * nbytes),
                fill_1w_GBs=rate(lambda: c.fill_(1.0), nbytes))
-    del a, b, c
+    del a, b, c, d
     torch.cuda.empty_cache()
     return out
 
@@ -521,7 +524,7 @@ def main():
         if traffic:
             # the dominant kernel's measured HBM rate against the best of
             # the plain streaming kernels with reads and writes
-            best = max(sc["copy_1r1w_GBs"], sc["add_2r1w_GBs"])
+            best = max(sc["copy_1r1w_GBs"], sc["add_2r1w_GBs"], sc["addcmul_3r1w_GBs"])
             roof["hbm_frac_of_stream_ceiling"] = round(roof["hbm_achieved"] / best, 4)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(wl)
