@@ -2146,8 +2146,11 @@ int hft_flux_transport_step(hft_ctx *c, double dtl, int strang, int subcycles, h
 // per round of sub-cycles the PTL pass, the controller, the coefficient
 // table, stage 1, gcap[round] fused RKG2 launches and the final copies, then
 // the second advection half-step.  The sequence is captured once per key
-// (step length, splitting, sub-cycles, setters' epoch, launch capacities)
-// and replayed with one hipGraphLaunch; the host reads back the controller
+// (step length, splitting, sub-cycles, the epoch bumped by setters that
+// change what the launches take by value, the device arrays and mesh they
+// read, launch capacities) and replayed with one hipGraphLaunch; flow and
+// diffusivity values rewritten in place between steps keep the capture
+// valid.  The host reads back the controller
 // state once per step.  gcap[0] holds every stage count a step can need (a
 // cycle is never longer than the step, so s <= sts_stages(dt, dt_euler)).
 // By default the sequence holds one round (tuning "step_graph_rounds" = 1):
