@@ -227,3 +227,34 @@ def test_planned_step_declines_cfl_substeps(gpu_available):
     assert g.get_tuning("step_graph_builds") == 0
     h.close()
     g.close()
+
+
+def test_planned_step_single_realization(gpu_available):
+    """One realization per context (thin bands fill the device), planned and
+    host-controlled, bit for bit."""
+    t, p = grid(181, 361)
+    eta = np.array([500.0])
+    maps = maps_for(t, p, 1, 360, noise=0.0)
+    h = model(t, p, maps, eta, "independent", 0, fused_target_blocks=256, band_rows=64)
+    g = model(t, p, maps, eta, "independent", 1, fused_target_blocks=256, band_rows=64)
+    lockstep(h, g, 3, 1.0)
+    assert g.get_tuning("step_graph_builds") == 1
+    assert g.get_tuning("step_graph_hits") == 2
+    h.close()
+    g.close()
+
+
+def test_planned_step_coupled_fixed_subcycles(gpu_available):
+    """Coupled controller with fixed sub-cycles (the testsuite's drop-in
+    semantics with diffusion_subcycles > 0), planned and host-controlled."""
+    t, p = grid(91, 181)
+    eta = np.array([150.0, 420.0, 880.0, 300.0])
+    maps = maps_for(t, p, 4, 370, noise=0.0)
+    h = model(t, p, maps, eta, "coupled", 0)
+    g = model(t, p, maps, eta, "coupled", 1)
+    lockstep(h, g, 3, 3.0, subcycles=2)
+    assert g.get_tuning("step_graph_builds") == 1
+    assert g.get_tuning("step_graph_hits") == 2
+    assert g.get_tuning("step_graph_overflows") == 0
+    h.close()
+    g.close()
