@@ -855,13 +855,25 @@ struct Fused2State {
 // 16-byte access.  When nu is odd the last thread's second column is the
 // row's first padding column, which no kernel reads as data; it may receive
 // that thread's (unused) value.
-// The rows stream through once per launch (the fields of a batch exceed the
-// caches), so loads and stores carry the non-temporal hint: measured 5-6 %
-// faster fused launches at 64 realizations of 512x1024.
+// The rows of u_{k0-1}, u_{k0-2} and the outputs stream through once per
+// launch (the fields of a batch exceed the caches), so their loads and
+// stores carry the non-temporal hint: measured 5-6 % faster fused launches
+// at 64 realizations of 512x1024.  u0 is read again by every launch of the
+// super-step and by the kernels around it, and one copy of it for the
+// batch (268 MB at 64 realizations of 512x1024) is about the size of the
+// 256 MB Infinity Cache: its rows are loaded with the default policy, so
+// they may stay there (fused time per diffusion step 15.9 -> 15.4 ms, whole
+// step 20.6 -> 19.5 ms on that batch).
 typedef double fused_d2 __attribute__((ext_vector_type(2)));
 
 __device__ __forceinline__ void load_pair(const double *row, int c0, double &x, double &y) {
   fused_d2 v = __builtin_nontemporal_load(reinterpret_cast<const fused_d2 *>(row + c0));
+  x = v.x;
+  y = v.y;
+}
+
+__device__ __forceinline__ void load_pair_u0(const double *row, int c0, double &x, double &y) {
+  fused_d2 v = *reinterpret_cast<const fused_d2 *>(row + c0);
   x = v.x;
   y = v.y;
 }
@@ -905,7 +917,7 @@ __device__ __forceinline__ void fused2_step(const FusedArgs &a, Fused2State<T> &
     size_t oa = (size_t)qn * pitch + cl, o = (size_t)rn * pitch + cl;
     load_pair(A + oa, 0, S.pf[pls][0][0], S.pf[pls][0][1]);
     load_pair(B + o, 0, S.pf[pls][1][0], S.pf[pls][1][1]);
-    load_pair(U0 + oa, 0, S.pf[pls][2][0], S.pf[pls][2][1]);
+    load_pair_u0(U0 + oa, 0, S.pf[pls][2][0], S.pf[pls][2][1]);
   }
   // u0 and dty0 of row q-5 (age 4: the slot that rows q-1 take now), for a
   // fifth stage
@@ -1080,16 +1092,18 @@ __device__ __forceinline__ void fused2_step(const FusedArgs &a, Fused2State<T> &
 // registers.
 // ---------------------------------------------------------------------------
 typedef int fused_v4i __attribute__((ext_vector_type(4)));
-constexpr int kBufNT = 2;  // cache-policy operand: non-temporal
+constexpr int kBufNT = 2;    // cache-policy operand: non-temporal
+constexpr int kBufKeep = 0;  // default policy (u0 rows, which are read again)
 
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t field_rsrc(const double *p, int bytes) {
   // raw buffer (stride 0), 32-bit data format word of the gfx9 resource
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<double *>(p), (short)0, bytes, 0x00020000);
 }
 
+template <int POL = kBufNT>
 __device__ __forceinline__ void bload_pair(__amdgpu_buffer_rsrc_t r, int voff, int soff, double &x,
                                            double &y) {
-  fused_v4i v = __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, kBufNT);
+  fused_v4i v = __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, POL);
   double2 d = __builtin_bit_cast(double2, v);
   x = d.x;
   y = d.y;
@@ -1144,7 +1158,7 @@ __device__ __forceinline__ void fused2_body_step(Fused2State<T> &S, double *lds,
     int oa = qn * ba.pitch8, ob = rn * ba.pitch8;
     bload_pair(rA, ba.vld, oa, S.pf[pls][0][0], S.pf[pls][0][1]);
     bload_pair(rB, ba.vld, ob, S.pf[pls][1][0], S.pf[pls][1][1]);
-    bload_pair(rU, ba.vld, oa, S.pf[pls][2][0], S.pf[pls][2][1]);
+    bload_pair<kBufKeep>(rU, ba.vld, oa, S.pf[pls][2][0], S.pf[pls][2][1]);
   }
   // u0 and dty0 of row q-5 (age 4, the slot rows q-1 take now), for a fifth stage
   double hu4[2] = {S.hu[s0][0], S.hu[s0][1]}, hd4[2] = {S.hd[s0][0], S.hd[s0][1]};
@@ -1346,7 +1360,7 @@ __device__ __forceinline__ void fused2_tile(const FusedArgs &a, int z, int band,
     double fn = wave_pole_sum(nu, [&](int kk) { return (D2 * (U0[m.pitch + kk] - U0[kk])) * m.dp[kk + 1]; });
     double Lp = fn * m.bc_diff_n;
     dt_pole[0] = dt_pole[1] = dtr * Lp;
-    if (c0 < nu) load_pair(U0, c0, u0_pole[0], u0_pole[1]);
+    if (c0 < nu) load_pair_u0(U0, c0, u0_pole[0], u0_pole[1]);
   }
   int nsteps = (hi0 - lo0) + T;
 #pragma unroll
@@ -1355,7 +1369,7 @@ __device__ __forceinline__ void fused2_tile(const FusedArgs &a, int z, int band,
     size_t oa = (size_t)q * m.pitch + cl, o = (size_t)rn * m.pitch + cl;
     load_pair(A + oa, 0, S.pf[i][0][0], S.pf[i][0][1]);
     load_pair(B + o, 0, S.pf[i][1][0], S.pf[i][1][1]);
-    load_pair(U0 + oa, 0, S.pf[i][2][0], S.pf[i][2][1]);
+    load_pair_u0(U0 + oa, 0, S.pf[i][2][0], S.pf[i][2][1]);
   }
   lds_barrier();
   int i = 0;
