@@ -496,7 +496,7 @@ int adjoint_begin(hft_ctx *c, const double *lam_dev, int64_t ld, double **LAM, i
     if ((rc = dalloc(&c->d_apD, nD)) || (rc = dalloc(&c->d_gout, (size_t)nr * 3))) return rc;
   }
   if (nA > c->apA_cap) {
-    if (c->d_apA) hipFree(c->d_apA);
+    if (c->d_apA) HIPCHK(hipFree(c->d_apA));
     c->d_apA = nullptr;
     if ((rc = dalloc(&c->d_apA, nA))) return rc;
     c->apA_cap = nA;

@@ -784,9 +784,9 @@ void sts_coefs(int64_t s, std::vector<double> &uj, std::vector<double> &vj,
 
 // Captured RKG2 super-steps (hipGraph execs) of the context.
 void clear_graphs(hft_ctx *c) {
-  for (auto &g : c->graphs) hipGraphExecDestroy(g.second);
+  for (auto &g : c->graphs) (void)hipGraphExecDestroy(g.second);
   c->graphs.clear();
-  if (c->sg_exec) hipGraphExecDestroy(c->sg_exec);
+  if (c->sg_exec) (void)hipGraphExecDestroy(c->sg_exec);
   c->sg_exec = nullptr;
   c->sg_key.clear();
 }
@@ -821,7 +821,7 @@ int upload_rkg2(hft_ctx *c, const std::vector<int> &reals, const std::vector<dou
   // the previous upload from the pinned buffers must be complete
   HIPCHK(hipEventSynchronize(c->coef_ev));
   if (need > c->h_coef_cap) {
-    if (c->h_coef) hipHostFree(c->h_coef);
+    if (c->h_coef) HIPCHK(hipHostFree(c->h_coef));
     size_t cap = std::max(need, (size_t)1024);
     HIPCHK(hipHostMalloc((void **)&c->h_coef, cap * sizeof(StageCoef)));
     c->h_coef_cap = cap;
@@ -906,13 +906,13 @@ int rkg2_superstep(hft_ctx *c, const std::vector<int> &reals, const std::vector<
   hipGraph_t graph = nullptr;
   hipError_t e = hipStreamEndCapture(c->stream, &graph);
   if (rcl) {
-    if (graph) hipGraphDestroy(graph);
+    if (graph) (void)hipGraphDestroy(graph);
     return rcl;
   }
   if (e != hipSuccess) return fail(HFT_E_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
   hipGraphExec_t exec = nullptr;
   e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-  hipGraphDestroy(graph);
+  (void)hipGraphDestroy(graph);
   if (e != hipSuccess) return fail(HFT_E_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
   c->graphs.push_back({key, exec});
   c->graph_builds++;
@@ -1517,8 +1517,8 @@ int hft_create(int ntm, int npm1, const double *t, const double *p, int nr, int 
 
 int hft_destroy(hft_ctx *c) {
   if (!c) return 0;
-  if (c->stream) hipStreamSynchronize(c->stream);
-  for (double *d : c->dmesh) hipFree(d);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (double *d : c->dmesh) (void)hipFree(d);
   void *ptrs[] = {c->f, c->buf0, c->buf1, c->buf2, c->buf3, c->dty0, c->fval, c->d_vt1, c->d_vp1, c->d_at1,
                   c->d_ap1, c->d_atten_inv, c->d_vt2, c->d_vp2, c->d_at2, c->d_ap2, c->d_D,
                   c->d_coef, c->d_s, c->d_order, c->d_order_id, c->d_cdt, c->d_ub1,
@@ -1528,40 +1528,40 @@ int hft_destroy(hft_ctx *c) {
                   c->d_omp, c->d_src, c->d_rfe[0], c->d_rfe[1], c->d_rnd, c->d_rfe_dati,
                   c->d_rfe_scale};
   for (void *q : ptrs)
-    if (q) hipFree(q);
-  if (c->h_coef) hipHostFree(c->h_coef);
-  if (c->h_small) hipHostFree(c->h_small);
-  if (c->h_smalld) hipHostFree(c->h_smalld);
+    if (q) (void)hipFree(q);
+  if (c->h_coef) (void)hipHostFree(c->h_coef);
+  if (c->h_small) (void)hipHostFree(c->h_small);
+  if (c->h_smalld) (void)hipHostFree(c->h_smalld);
   for (auto &tp : c->pending) {
-    hipEventDestroy(tp.a);
-    hipEventDestroy(tp.b);
+    (void)hipEventDestroy(tp.a);
+    (void)hipEventDestroy(tp.b);
   }
-  for (auto e : c->free_ev) hipEventDestroy(e);
-  if (c->coef_ev) hipEventDestroy(c->coef_ev);
-  for (auto &g : c->graphs) hipGraphExecDestroy(g.second);
-  if (c->sg_exec) hipGraphExecDestroy(c->sg_exec);
+  for (auto e : c->free_ev) (void)hipEventDestroy(e);
+  if (c->coef_ev) (void)hipEventDestroy(c->coef_ev);
+  for (auto &g : c->graphs) (void)hipGraphExecDestroy(g.second);
+  if (c->sg_exec) (void)hipGraphExecDestroy(c->sg_exec);
   for (void *q : {(void *)c->d_plan, (void *)c->d_slots, (void *)c->d_htab, (void *)c->d_dte})
-    if (q) hipFree(q);
-  if (c->h_plan) hipHostFree(c->h_plan);
+    if (q) (void)hipFree(q);
+  if (c->h_plan) (void)hipHostFree(c->h_plan);
   for (auto *vec : {&c->adj_fix, &c->adj_ck, &c->adj_w, &c->adj_st})
     for (double *q : *vec)
-      if (q) hipFree(q);
+      if (q) (void)hipFree(q);
   for (double *q : {c->d_apD, c->d_apA, c->d_gout, c->d_adv_dt})
-    if (q) hipFree(q);
-  if (c->d_adv_order) hipFree(c->d_adv_order);
-  if (c->h_adv_order) hipHostFree(c->h_adv_order);
-  if (c->h_adv_dt) hipHostFree(c->h_adv_dt);
-  if (c->adv_ev) hipEventDestroy(c->adv_ev);
-  if (c->ptiles_ev) hipEventDestroy(c->ptiles_ev);
+    if (q) (void)hipFree(q);
+  if (c->d_adv_order) (void)hipFree(c->d_adv_order);
+  if (c->h_adv_order) (void)hipHostFree(c->h_adv_order);
+  if (c->h_adv_dt) (void)hipHostFree(c->h_adv_dt);
+  if (c->adv_ev) (void)hipEventDestroy(c->adv_ev);
+  if (c->ptiles_ev) (void)hipEventDestroy(c->ptiles_ev);
   for (void *q : {(void *)c->d_ptiles, (void *)c->d_pint, (void *)c->d_pctl})
-    if (q) hipFree(q);
-  if (c->h_ptiles) hipHostFree(c->h_ptiles);
-  if (c->h_pint) hipHostFree(c->h_pint);
+    if (q) (void)hipFree(q);
+  if (c->h_ptiles) (void)hipHostFree(c->h_ptiles);
+  if (c->h_pint) (void)hipHostFree(c->h_pint);
   if (c->own_stream) {
-    hipStreamSynchronize(c->own_stream);
-    hipStreamDestroy(c->own_stream);  // c->stream belongs to the caller
+    (void)hipStreamSynchronize(c->own_stream);
+    (void)hipStreamDestroy(c->own_stream);  // c->stream belongs to the caller
   } else if (c->stream) {
-    hipStreamDestroy(c->stream);
+    (void)hipStreamDestroy(c->stream);
   }
   delete c;
   return 0;
@@ -2452,13 +2452,13 @@ static int sg_step(hft_ctx *c, double dtl, int strang, int subcycles, bool *ran)
     hipError_t e = hipStreamEndCapture(c->stream, &graph);
     c->adv_skip = nullptr;
     if (rcl) {
-      if (graph) hipGraphDestroy(graph);
+      if (graph) (void)hipGraphDestroy(graph);
       return rcl;
     }
     if (e != hipSuccess) return fail(HFT_E_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
     hipGraphExec_t exec = nullptr;
     e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-    hipGraphDestroy(graph);
+    (void)hipGraphDestroy(graph);
     if (e != hipSuccess) return fail(HFT_E_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
     c->sg_exec = exec;
     c->sg_key = key;
@@ -2639,7 +2639,7 @@ int hft_assimilate(hft_ctx *c, const double *map, int per_real, int custom, cons
   if (!c || !map) return fail(HFT_E_ARG, "hft_assimilate: null argument");
   size_t n = (size_t)3 * c->ntm * c->npm1 * (per_real ? c->nr : 1);
   if (n > c->map_cap) {
-    if (c->d_map) hipFree(c->d_map);
+    if (c->d_map) HIPCHK(hipFree(c->d_map));
     c->d_map = nullptr;
     int rc = dalloc(&c->d_map, n);
     if (rc) return rc;
