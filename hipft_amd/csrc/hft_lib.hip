@@ -861,6 +861,16 @@ int upload_rkg2(hft_ctx *c, const std::vector<int> &reals, const std::vector<dou
   return 0;
 }
 
+// Interior march steps of k_rkg2_fused2 (tuning fused_body): they address
+// rows through 32-bit buffer offsets, so the realization's field must stay
+// below 2 GiB.  Five stages per launch run the general steps only: their
+// interior steps gave values that differ from the general steps on some
+// band shapes (512x1024, 32- and 64-row bands), a cause not found.
+int fused_body_level(const hft_ctx *c) {
+  if (c->fused_T > 4 || c->rstride * sizeof(double) >= (size_t)0x7fff0000) return 0;
+  return c->fused_body;
+}
+
 bool fused_usable(const hft_ctx *c) {
   return c->fused && c->nu <= 1024 && c->fused_T >= 2 &&
          (c->fused_T <= 4 || (c->fused_T == 5 && c->fused_kernel == 3)) &&
@@ -1057,7 +1067,7 @@ int launch_fused(hft_ctx *c, int64_t k, int active, const std::vector<int> &ord,
     // threads (3-row rings of the T stage levels and of u0); interior steps
     // without row tests when the buffers' byte offsets fit the 32-bit
     // buffer-resource range
-    fa.body = (c->fused_body && c->rstride * sizeof(double) < (size_t)0x7fff0000) ? 1 : 0;
+    fa.body = fused_body_level(c);
     return launch_fused2(c, fa, blocks, units, sunits);
   }
   if (c->fused_kernel == 2 && !store) {
@@ -2374,7 +2384,7 @@ static int sg_enqueue(hft_ctx *c, double dtl, int strang, int subcycles, int64_t
       fa.outB = (q & 1) ? c->buf3 : c->buf2;
       fa.f = c->f; fa.dty0 = c->dty0; fa.D = c->d_D; fa.coef = c->d_coef; fa.nr_table = nr;
       fa.s = c->d_s; fa.order = c->d_order; fa.rstride = c->rstride; fa.cdt = c->d_cdt;
-      fa.body = (c->fused_body && c->rstride * sizeof(double) < (size_t)0x7fff0000) ? 1 : 0;
+      fa.body = fused_body_level(c);
       fa.slot = c->d_slots + 3 * q;
       if ((rc = launch_fused2(c, fa, maxtiles, 0.0, 0.0))) return rc;
     }

@@ -492,6 +492,30 @@ def test_fused_rkg2_five_stages_bitexact(gpu_available, H, B):
     assert_same(g.get_state("tp"), o.get_f(), f"fused kernel 3 T=5 H={H} B={B}")
 
 
+def test_fused_five_stages_full_size_bitexact(gpu_available):
+    """Five stages per launch on the 512x1024 batch with fixed 32- and
+    64-row bands (general march steps: interior steps are off for five
+    stages) against one stage per launch, bit for bit."""
+    t, p = grid(512, 1024)
+    nr = 6
+    maps = np.stack([W.bipole_map(t, p, 40, 300 + r) for r in range(nr)])
+    eta = np.array([100.0, 180.0, 320.0, 560.0, 800.0, 1000.0])
+    out = {}
+    for key, tun in (("one-stage", dict(fused=0)),
+                     ("H=32", dict(fused=1, fused_stages=5, band_rows=32)),
+                     ("H=64", dict(fused=1, fused_stages=5, band_rows=64))):
+        m = FluxTransport(t, p, nr, mode="independent")
+        m.set_state(maps, layout="pt")
+        m.set_diffusion(eta)
+        m.set_tuning(fused_min_realizations=1, fused_kernel=3, fused_bands=0,
+                     fused_target_blocks=0, fused_body=1, **tun)
+        m.diffusion_step(1.0, subcycles=0)
+        out[key] = m.get_state("tp")
+        m.close()
+    for key in ("H=32", "H=64"):
+        assert_same(out[key], out["one-stage"], f"512x1024 five stages {key}")
+
+
 @pytest.mark.parametrize("T", [3, 4, 5])
 def test_fused_round_aware_bands_bitexact(gpu_available, T):
     """Band counts chosen per launch against whole rounds of co-resident
@@ -529,13 +553,14 @@ def test_fused_round_aware_bands_bitexact(gpu_available, T):
     g.close()
 
 
-@pytest.mark.parametrize("T", [2, 3, 4, 5])
+@pytest.mark.parametrize("T", [2, 3, 4])
 def test_fused_interior_steps_bitexact(gpu_available, T):
     """The two-column kernel's interior march steps (no row tests, buffer
     loads and stores, compile-time ring slots) against the oracle on a
     91x181 grid with bands tall enough for them (runtime LDS row length),
     and against the same kernel without them on the 512x1024 batch (LDS row
-    length 1024), bit for bit."""
+    length 1024), bit for bit.  Five stages per launch run the general
+    steps only."""
     o, g = pair(91, 181, nr=3)
     eta = [150.0, 450.0, 950.0]
     o.load_diffusion(eta)
@@ -549,17 +574,18 @@ def test_fused_interior_steps_bitexact(gpu_available, T):
     t, p = grid(512, 1024)
     nr = 6
     maps = np.stack([W.bipole_map(t, p, 40, 300 + r) for r in range(nr)])
-    out = []
-    for body in (0, 1):
-        m = FluxTransport(t, p, nr, mode="independent")
-        m.set_state(maps, layout="pt")
-        m.set_diffusion(np.array([100.0, 180.0, 320.0, 560.0, 800.0, 1000.0]))
-        m.set_tuning(fused=1, fused_stages=T, fused_min_realizations=1, fused_kernel=3,
-                     fused_bands=1, fused_body=body)
-        m.diffusion_step(1.0, subcycles=0)
-        out.append(m.get_state("tp"))
-        m.close()
-    assert_same(out[1], out[0], f"512x1024 interior steps T={T}")
+    for shape in (dict(fused_bands=1), dict(fused_bands=0, fused_target_blocks=0, band_rows=64)):
+        out = []
+        for body in (0, 1):
+            m = FluxTransport(t, p, nr, mode="independent")
+            m.set_state(maps, layout="pt")
+            m.set_diffusion(np.array([100.0, 180.0, 320.0, 560.0, 800.0, 1000.0]))
+            m.set_tuning(fused=1, fused_stages=T, fused_min_realizations=1, fused_kernel=3,
+                         fused_body=body, **shape)
+            m.diffusion_step(1.0, subcycles=0)
+            out.append(m.get_state("tp"))
+            m.close()
+        assert_same(out[1], out[0], f"512x1024 interior steps T={T} {shape}")
 
 
 @pytest.mark.parametrize("kernel", [1, 2, 3])
