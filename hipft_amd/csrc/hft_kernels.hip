@@ -983,9 +983,10 @@ __device__ __forceinline__ void fused2_step(const FusedArgs &a, Fused2State<T> &
       // same coefficient and summation order), kept for the later stages
       double Lu[2];
       if (R == ntm - 1) {
-        const double *r0 = lu0 + ((ntm - 1) % 3) * nrow, *r1p = lu0 + ((ntm - 2) % 3) * nrow;
-        double fs = wave_pole_sum(nu, [&](int kk) { return (D2 * (r0[kk] - r1p[kk])) * m.dp[kk + 1]; });
-        Lu[0] = Lu[1] = -fs * m.bc_diff_s;
+        // the south-pole row's dty0 as stage 1 wrote it (the same
+        // expression, dt times the polar closure's sum), uniform along the row
+        double d = a.dty0[(U0 - a.f) + (size_t)(ntm - 1) * pitch];
+        S.hd[s0][0] = S.hd[s0][1] = d;
       } else {
         const double *rc = lu0 + (R % 3) * nrow;
         const double *rr = rowc + 4 * (R - lo0);
@@ -1009,9 +1010,9 @@ __device__ __forceinline__ void fused2_step(const FusedArgs &a, Fused2State<T> &
           L = L + c5 * xe[e];
           Lu[e] = L;
         }
+        S.hd[s0][0] = dtr * Lu[0];
+        S.hd[s0][1] = dtr * Lu[1];
       }
-      S.hd[s0][0] = dtr * Lu[0];
-      S.hd[s0][1] = dtr * Lu[1];
     }
     if (act && R > 0) {
       double Lv[2];
@@ -1355,11 +1356,11 @@ __device__ __forceinline__ void fused2_tile(const FusedArgs &a, int z, int band,
   S.u0n[0] = S.u0n[1] = 0.0;
   double u0_pole[2] = {0.0, 0.0}, dt_pole[2] = {0.0, 0.0};
   if (lo0 == 0) {
-    // dty0 of the north-pole row, dt L(u0) with the polar closure's sum
-    // (every lane takes part in the wave sum)
-    double fn = wave_pole_sum(nu, [&](int kk) { return (D2 * (U0[m.pitch + kk] - U0[kk])) * m.dp[kk + 1]; });
-    double Lp = fn * m.bc_diff_n;
-    dt_pole[0] = dt_pole[1] = dtr * Lp;
+    // dty0 of the north-pole row, dt L(u0) with the polar closure's sum:
+    // stage 1 wrote it (the same expression and summation order), uniform
+    // along the row.  Reading it spares the pole band a wave sum over two
+    // rows of u0 before its march, which made it the launch's last block.
+    dt_pole[0] = dt_pole[1] = a.dty0[base];
     if (c0 < nu) load_pair_u0(U0, c0, u0_pole[0], u0_pole[1]);
   }
   int nsteps = (hi0 - lo0) + T;
