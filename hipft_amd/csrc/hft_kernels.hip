@@ -893,7 +893,7 @@ __device__ __forceinline__ void fused2_step(const FusedArgs &a, Fused2State<T> &
                                             const double *B, const double *U0, double dtr,
                                             double *OA, double *OB, const double (&u0_pole)[2],
                                             const double (&dt_pole)[2], const double (*cfs)[5],
-                                            double *const *Wst) {
+                                            double *const *Wst, const double *sdp) {
   const Mesh &m = a.m;
   const int nu = m.nu, ntm = m.ntm, pitch = m.pitch;
   constexpr int s0 = U & 3, s1 = (U + 3) & 3, s2 = (U + 2) & 3;
@@ -951,7 +951,7 @@ __device__ __forceinline__ void fused2_step(const FusedArgs &a, Fused2State<T> &
     if (act && R == 1 && rlo == 0) {
       // late north-pole row 0 of stage t, from level t-1 rows 0 and 1 in LDS
       const double *r0 = lp + 0 * nrow, *r1p = lp + 1 * nrow;
-      double fn = wave_pole_sum(nu, [&](int kk) { return (D2 * (r1p[kk] - r0[kk])) * m.dp[kk + 1]; });
+      double fn = wave_pole_sum(nu, [&](int kk) { return (D2 * (r1p[kk] - r0[kk])) * sdp[kk]; });
       double Lv = fn * m.bc_diff_n;
       double bp0 = 0.0, bp1 = 0.0;
       if (t == 1 && c0 < nu) load_pair(B, c0, bp0, bp1);
@@ -1018,7 +1018,7 @@ __device__ __forceinline__ void fused2_step(const FusedArgs &a, Fused2State<T> &
       double Lv[2];
       if (R == ntm - 1) {
         const double *r0 = lp + ((ntm - 1) % 3) * nrow, *r1p = lp + ((ntm - 2) % 3) * nrow;
-        double fs = wave_pole_sum(nu, [&](int kk) { return (D2 * (r0[kk] - r1p[kk])) * m.dp[kk + 1]; });
+        double fs = wave_pole_sum(nu, [&](int kk) { return (D2 * (r0[kk] - r1p[kk])) * sdp[kk]; });
         Lv[0] = Lv[1] = -fs * m.bc_diff_s;
       } else {
         const double *rc = lp + (R % 3) * nrow;
@@ -1282,7 +1282,7 @@ template <int T, int NR = 0>
 __device__ __forceinline__ void fused2_tile(const FusedArgs &a, int z, int band, int Hb, int k0,
                                             const double *Ain, const double *Bin, double *OAin,
                                             double *OBin, double *lds, double (*cfs)[5],
-                                            double *rowc, bool body = false) {
+                                            double *rowc, double *sdp, bool body = false) {
   const Mesh &m = a.m;
   const int nu = m.nu, ntm = m.ntm;
   const int nrow = 2 * (int)blockDim.x;
@@ -1355,6 +1355,10 @@ __device__ __forceinline__ void fused2_tile(const FusedArgs &a, int z, int band,
     for (int e = 0; e < 2; e++) S.hu[d][e] = S.hd[d][e] = 0.0;
   S.u0n[0] = S.u0n[1] = 0.0;
   double u0_pole[2] = {0.0, 0.0}, dt_pole[2] = {0.0, 0.0};
+  if (lo0 == 0 || hi0 == ntm) {
+    // the phi cell widths of the polar sums, read from LDS by the march
+    for (int c = threadIdx.x; c < nu; c += blockDim.x) sdp[c] = m.dp[c + 1];
+  }
   if (lo0 == 0) {
     // dty0 of the north-pole row, dt L(u0) with the polar closure's sum:
     // stage 1 wrote it (the same expression and summation order), uniform
@@ -1386,16 +1390,16 @@ __device__ __forceinline__ void fused2_tile(const FusedArgs &a, int z, int band,
       for (; i < ib; i += 4) {
         fused2_step<T, 0>(a, S, lds, rowc, i, nsteps, lo0, hi0, j0, j1, nst, final_here, c0, cl,
                           cw0, ce0, ce1, col1, nrow, D2, P1, P5, A, B, U0, dtr, OA, OB, u0_pole,
-                          dt_pole, cfs, Wst);
+                          dt_pole, cfs, Wst, sdp);
         fused2_step<T, 1>(a, S, lds, rowc, i + 1, nsteps, lo0, hi0, j0, j1, nst, final_here, c0,
                           cl, cw0, ce0, ce1, col1, nrow, D2, P1, P5, A, B, U0, dtr, OA, OB,
-                          u0_pole, dt_pole, cfs, Wst);
+                          u0_pole, dt_pole, cfs, Wst, sdp);
         fused2_step<T, 2>(a, S, lds, rowc, i + 2, nsteps, lo0, hi0, j0, j1, nst, final_here, c0,
                           cl, cw0, ce0, ce1, col1, nrow, D2, P1, P5, A, B, U0, dtr, OA, OB,
-                          u0_pole, dt_pole, cfs, Wst);
+                          u0_pole, dt_pole, cfs, Wst, sdp);
         fused2_step<T, 3>(a, S, lds, rowc, i + 3, nsteps, lo0, hi0, j0, j1, nst, final_here, c0,
                           cl, cw0, ce0, ce1, col1, nrow, D2, P1, P5, A, B, U0, dtr, OA, OB,
-                          u0_pole, dt_pole, cfs, Wst);
+                          u0_pole, dt_pole, cfs, Wst, sdp);
       }
       BodyArgs<T> ba;
       ba.hi0 = hi0;
@@ -1426,19 +1430,19 @@ __device__ __forceinline__ void fused2_tile(const FusedArgs &a, int z, int band,
   }
   for (; i < nsteps; i += 4) {
     fused2_step<T, 0>(a, S, lds, rowc, i, nsteps, lo0, hi0, j0, j1, nst, final_here, c0, cl, cw0,
-                      ce0, ce1, col1, nrow, D2, P1, P5, A, B, U0, dtr, OA, OB, u0_pole, dt_pole, cfs, Wst);
+                      ce0, ce1, col1, nrow, D2, P1, P5, A, B, U0, dtr, OA, OB, u0_pole, dt_pole, cfs, Wst, sdp);
     if (i + 1 < nsteps)
       fused2_step<T, 1>(a, S, lds, rowc, i + 1, nsteps, lo0, hi0, j0, j1, nst, final_here, c0,
                         cl, cw0, ce0, ce1, col1, nrow, D2, P1, P5, A, B, U0, dtr, OA, OB, u0_pole, dt_pole,
-                        cfs, Wst);
+                        cfs, Wst, sdp);
     if (i + 2 < nsteps)
       fused2_step<T, 2>(a, S, lds, rowc, i + 2, nsteps, lo0, hi0, j0, j1, nst, final_here, c0,
                         cl, cw0, ce0, ce1, col1, nrow, D2, P1, P5, A, B, U0, dtr, OA, OB, u0_pole, dt_pole,
-                        cfs, Wst);
+                        cfs, Wst, sdp);
     if (i + 3 < nsteps)
       fused2_step<T, 3>(a, S, lds, rowc, i + 3, nsteps, lo0, hi0, j0, j1, nst, final_here, c0,
                         cl, cw0, ce0, ce1, col1, nrow, D2, P1, P5, A, B, U0, dtr, OA, OB, u0_pole, dt_pole,
-                        cfs, Wst);
+                        cfs, Wst, sdp);
   }
 }
 
@@ -1449,6 +1453,7 @@ __global__ __launch_bounds__(512) void k_rkg2_fused2(FusedArgs a) {
   extern __shared__ double lds[];  // [T + 1][3][2 * blockDim]: stage levels 0..T-1, then u0
   __shared__ double cfs[T][5];
   __shared__ __align__(16) double rowc[4 * kFusedMaxRows];
+  __shared__ double sdp[1024];  // phi cell widths for the polar sums (pole bands)
   int nt = (int)gridDim.x, nb = a.nbands, H = a.H;
   if (a.slot) {
     nt = a.slot[0];
@@ -1461,7 +1466,7 @@ __global__ __launch_bounds__(512) void k_rkg2_fused2(FusedArgs a) {
   bool stored = false;
 #pragma unroll
   for (int t = 0; t < T; t++) stored = stored || a.W[t] != nullptr;
-  fused2_tile<T, NR>(a, z, band, H, a.k0, a.A, a.B, a.outA, a.outB, lds, cfs, rowc,
+  fused2_tile<T, NR>(a, z, band, H, a.k0, a.A, a.B, a.outA, a.outB, lds, cfs, rowc, sdp,
                      a.body != 0 && !stored);
 }
 
@@ -1512,6 +1517,7 @@ __global__ __launch_bounds__(512) void k_rkg2_persist(PersistArgs p) {
   extern __shared__ double lds[];
   __shared__ double cfs[T][5];
   __shared__ double rowc[4 * kFusedMaxRows];
+  __shared__ double sdp[1024];
   __shared__ int sh_tile, sh_stop;
   const FusedArgs &a = p.fa;
   for (;;) {
@@ -1563,7 +1569,7 @@ __global__ __launch_bounds__(512) void k_rkg2_persist(PersistArgs p) {
     const double *Bin = g == 0 ? a.f : ((g & 1) ? p.P[3] : p.P[1]);
     double *OAo = (g & 1) ? p.P[0] : p.P[2];
     double *OBo = (g & 1) ? p.P[1] : p.P[3];
-    fused2_tile<T>(a, z, b, p.zH[z], 2 + g * T, Ain, Bin, OAo, OBo, lds, cfs, rowc);
+    fused2_tile<T>(a, z, b, p.zH[z], 2 + g * T, Ain, Bin, OAo, OBo, lds, cfs, rowc, sdp);
     // release: this thread's stores of the tile, then the block's count
     __threadfence();
     __syncthreads();
