@@ -1381,12 +1381,14 @@ __device__ __forceinline__ void fused2_tile(const FusedArgs &a, int z, int band,
   {
     // body rows: q in [qlo, qhi] puts every level's row R = q - t in the
     // band's own range and away from the poles; the run starts at a step
-    // index divisible by 4 whose row is divisible by 3
+    // index divisible by 4 whose row is divisible by 3 and covers whole
+    // groups of 4 steps (register and prefetch slots are the general
+    // steps' again after each group; ring slots follow the row)
     int qlo = max(j0, 2) + T, qhi = min(j1, ntm - 1);
     int ib = (qlo - lo0 + 3) & ~3;
     while ((lo0 + ib) % 3) ib += 4;
-    int nchunks = (qhi - (lo0 + ib) + 1) / 12;
-    if (body && nst == T && nchunks > 0) {
+    int nquads = (qhi - (lo0 + ib) + 1) / 4;
+    if (body && nst == T && nquads > 0) {
       for (; i < ib; i += 4) {
         fused2_step<T, 0>(a, S, lds, rowc, i, nsteps, lo0, hi0, j0, j1, nst, final_here, c0, cl,
                           cw0, ce0, ce1, col1, nrow, D2, P1, P5, A, B, U0, dtr, OA, OB, u0_pole,
@@ -1416,15 +1418,28 @@ __device__ __forceinline__ void fused2_tile(const FusedArgs &a, int z, int band,
       __amdgpu_buffer_rsrc_t rA = field_rsrc(A, bytes), rB = field_rsrc(B, bytes),
                              rU = field_rsrc(U0, bytes), rOA = field_rsrc(OA, bytes),
                              rOB = field_rsrc(OB, bytes);
-      for (int ch = 0; ch < nchunks; ch++, i += 12) {
+      // runs of 12 steps (the ring slots' period) that may end after 4 or 8
+      // (exits elsewhere in a run cost register spills)
+      const int iend = ib + 4 * nquads;
+      while (i < iend) {
         int q = lo0 + i;
         const double *rb = rowc + 4 * (i - T);
 #define HFT_BODY(u)                                                                           \
   fused2_body_step<T, u, NR>(S, lds, rb + 4 * (u), nrow, q + (u), ba, rA, rB, rU, rOA, rOB, c0, \
                              cw0, ce0, ce1, P1, P5, dtr)
-        HFT_BODY(0); HFT_BODY(1); HFT_BODY(2); HFT_BODY(3); HFT_BODY(4); HFT_BODY(5);
-        HFT_BODY(6); HFT_BODY(7); HFT_BODY(8); HFT_BODY(9); HFT_BODY(10); HFT_BODY(11);
+        HFT_BODY(0); HFT_BODY(1); HFT_BODY(2); HFT_BODY(3);
+        if (i + 4 >= iend) {
+          i += 4;
+          break;
+        }
+        HFT_BODY(4); HFT_BODY(5); HFT_BODY(6); HFT_BODY(7);
+        if (i + 8 >= iend) {
+          i += 8;
+          break;
+        }
+        HFT_BODY(8); HFT_BODY(9); HFT_BODY(10); HFT_BODY(11);
 #undef HFT_BODY
+        i += 12;
       }
     }
   }
