@@ -4,12 +4,17 @@ Usage (after the two `rocprofv3 --pmc FETCH_SIZE ...` / `--pmc WRITE_SIZE ...`
 runs of `bench.py --pmc-probe`, each with `--output-format csv`):
 
     python -m hipft_amd.pmc_summary FETCH_DIR WRITE_DIR OUT_JSON OUT_MD [--copy-bytes N]
+                                    [--probe PROBE_LOG]
 
 FETCH_SIZE and WRITE_SIZE are reported in KiB per dispatch.  The probe's
 first large device-to-device copy (N bytes, default 256 MiB; the largest
 `copyBuffer` dispatch) calibrates them: scale = N / (counter x 1024), so a
 copy of N bytes reads and writes N bytes after scaling.  The JSON feeds
 bench.py's `roofline.traffic`; the markdown table is the readable summary.
+PROBE_LOG is the output of one of the probe runs: its `pmc_probe` line
+(the algorithmic bytes per fused launch of the probed step) is kept in the
+JSON as `probe`, so measured and algorithmic bytes refer to the same
+launches.
 """
 import csv
 import json
@@ -63,14 +68,33 @@ def markdown(s):
     return "\n".join(lines)
 
 
+def probe_line(path):
+    """The last JSON line of a probe log that carries "pmc_probe"."""
+    found = None
+    with open(path) as fh:
+        for line in fh:
+            line = line.strip()
+            if line.startswith("{") and '"pmc_probe"' in line:
+                found = json.loads(line)
+    if found is None:
+        raise ValueError(f"no pmc_probe line in {path}")
+    return found
+
+
 def main(argv):
-    args = [a for a in argv if not a.startswith("--")]
-    copy_bytes = 256 << 20
-    if "--copy-bytes" in argv:
-        copy_bytes = int(argv[argv.index("--copy-bytes") + 1])
-        args = [a for a in args if a != str(copy_bytes)]
+    opts = {}
+    args = []
+    it = iter(argv)
+    for a in it:
+        if a in ("--copy-bytes", "--probe"):
+            opts[a] = next(it)
+        else:
+            args.append(a)
+    copy_bytes = int(opts.get("--copy-bytes", 256 << 20))
     fetch_dir, write_dir, out_json, out_md = args[:4]
     s = summarize(fetch_dir, write_dir, copy_bytes)
+    if "--probe" in opts:
+        s["probe"] = probe_line(opts["--probe"])
     with open(out_json, "w") as fh:
         json.dump(s, fh, indent=1)
     with open(out_md, "w") as fh:

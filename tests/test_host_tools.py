@@ -113,3 +113,39 @@ def test_bench_refuses_more_than_one_gpu():
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 2
     assert "one GPU" in json.loads(r.stdout.strip().splitlines()[-1])["error"]
+
+
+def _counter_csv(d, counter, rows):
+    os.makedirs(d, exist_ok=True)
+    with open(os.path.join(d, "run_counter_collection.csv"), "w") as fh:
+        fh.write("Kernel_Name,Counter_Name,Counter_Value\n")
+        for name, value in rows:
+            fh.write(f'"{name}",{counter},{value}\n')
+
+
+def test_pmc_summary_scales_counters_and_keeps_probe(tmp_path):
+    """The counter summary scales FETCH_SIZE/WRITE_SIZE so the calibration
+    copy reads and writes its own size, averages per dispatch, and keeps the
+    probe's algorithmic bytes per fused launch."""
+    from hipft_amd import pmc_summary
+    copy = 256 << 20
+    kib = copy // 1024
+    fdir, wdir = str(tmp_path / "f"), str(tmp_path / "w")
+    # the copy's FETCH_SIZE reads half its size (a 2x scale), WRITE_SIZE its size
+    _counter_csv(fdir, "FETCH_SIZE", [("__amd_rocclr_copyBuffer", kib // 2),
+                                      ("k_fused", 1000), ("k_fused", 3000)])
+    _counter_csv(wdir, "WRITE_SIZE", [("__amd_rocclr_copyBuffer", kib),
+                                      ("k_fused", 500), ("k_fused", 500)])
+    log = tmp_path / "probe.log"
+    log.write_text('noise\n{"pmc_probe": "done", "copy_bytes": 268435456, '
+                   '"fused_launches": 2, "fused_alg_bytes_per_launch": 3000000}\n')
+    out_json, out_md = str(tmp_path / "s.json"), str(tmp_path / "s.md")
+    pmc_summary.main([fdir, wdir, out_json, out_md, "--probe", str(log)])
+    s = json.load(open(out_json))
+    k = s["kernels"]["k_fused"]
+    assert s["fetch_scale"] == 2.0 and s["write_scale"] == 1.0
+    assert k["dispatches"] == 2
+    assert k["read_bytes_avg"] == 2 * 1024 * 2000
+    assert k["write_bytes_avg"] == 1024 * 500
+    assert s["probe"]["fused_alg_bytes_per_launch"] == 3000000
+    assert "k_fused" in open(out_md).read()
