@@ -1605,6 +1605,10 @@ struct LapArgs {
   // PTL: per-realization max |Lx| over points with |x| > 1e-14
   unsigned long long *absmax;  // may be null
   const int *dn = nullptr;     // device realization count (as Stage1Args)
+  // PTL: per tile (logical tile index) the tile's max |Lx| over those points
+  // (bits, x = low and y = high word), how many of its points reach it (z)
+  // and the first such thread (w); may be null
+  uint4 *rec = nullptr;
 };
 
 __global__ __launch_bounds__(kBlock) void k_laplacian(LapArgs a) {
@@ -1630,13 +1634,32 @@ __global__ __launch_bounds__(kBlock) void k_laplacian(LapArgs a) {
     // block max, then one atomic per block (|Af| >= 0: the integer order of
     // the bit patterns is the numeric order)
     __shared__ double wmax[kBlock / 64];
-    for (int h = 32; h >= 1; h >>= 1) cand = fmax(cand, __shfl_xor(cand, h, 64));
-    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = cand;
+    __shared__ int wcnt[kBlock / 64], wfirst[kBlock / 64];
+    double wm = cand;
+    for (int h = 32; h >= 1; h >>= 1) wm = fmax(wm, __shfl_xor(wm, h, 64));
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = wm;
     __syncthreads();
-    if (threadIdx.x == 0) {
-      double mx = wmax[0];
-      for (int q = 1; q < kBlock / 64; q++) mx = fmax(mx, wmax[q]);
-      if (mx > 0.0) atomicMax(&a.absmax[r], (unsigned long long)__double_as_longlong(mx));
+    double mx = wmax[0];
+    for (int q = 1; q < kBlock / 64; q++) mx = fmax(mx, wmax[q]);
+    if (threadIdx.x == 0 && mx > 0.0)
+      atomicMax(&a.absmax[r], (unsigned long long)__double_as_longlong(mx));
+    if (a.rec) {
+      // the tile's record: its max, how many points reach it, the first one
+      unsigned long long hit = __ballot(mx > 0.0 && cand == mx);
+      if ((threadIdx.x & 63) == 0) {
+        wcnt[threadIdx.x >> 6] = __popcll(hit);
+        wfirst[threadIdx.x >> 6] = hit ? (threadIdx.x & ~63) + __ffsll((long long)hit) - 1 : -1;
+      }
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        int cnt = 0, first = -1;
+        for (int q = 0; q < kBlock / 64; q++) {
+          cnt += wcnt[q];
+          if (first < 0) first = wfirst[q];
+        }
+        unsigned long long b = (unsigned long long)__double_as_longlong(mx);
+        a.rec[t] = make_uint4((unsigned)b, (unsigned)(b >> 32), (unsigned)cnt, (unsigned)first);
+      }
     }
   }
 }
@@ -1654,20 +1677,27 @@ struct PtlArgs {
   int nlaunch;
   size_t rstride;
   const int *dn = nullptr;  // device realization count (replaces nlaunch)
+  const uint4 *rec = nullptr;  // k_ptl_pick: k_laplacian's tile records
 };
 
-__global__ __launch_bounds__(kBlock) void k_ptl_min(PtlArgs a) {
-  int t = tile_index(a.g, a.dn);
-  if (t < 0) return;
-  int z, j, ct;
-  tile_of(t, a.g, z, j, ct);
-  int r = a.order[z];
+// The PTL step limit from the tile records of k_laplacian: one block per
+// realization of the launch visits the records of its tiles and evaluates
+// the limit only at the points of tiles whose max equals the target (the
+// recorded point when it is the tile's only one, else every point of the
+// tile): at each point off the poles where |Af| equals the target and
+// |f| > 1e-14, min over the four neighbours with du dF < -1e-14 of
+// -du/dF.  The minimum does not depend on the order the points are visited
+// in, and no pass over f and Af is needed to find them.
+__global__ __launch_bounds__(256) void k_ptl_pick(PtlArgs a) {
+  const uint4 *rec = a.rec;
   const Mesh &m = a.m;
-  int c = ct * kBlock + threadIdx.x;
+  int z = blockIdx.x;
+  int nl = a.dn ? *a.dn : a.nlaunch;
+  if (z >= nl) return;
+  int r = a.order[z];
   double target;
   if (a.coupled) {
     unsigned long long mx = 0;
-    int nl = a.dn ? *a.dn : a.nlaunch;
     for (int q = 0; q < nl; q++) {
       unsigned long long v = a.absmax[a.order[q]];
       mx = v > mx ? v : mx;
@@ -1677,24 +1707,47 @@ __global__ __launch_bounds__(kBlock) void k_ptl_min(PtlArgs a) {
     target = __longlong_as_double((long long)a.absmax[r]);
   }
   double best = 1.7976931348623157e308;
-  if (target > 0.0 && c < m.nu && j > 0 && j < m.ntm - 1) {
+  if (target > 0.0) {
     size_t base = (size_t)r * a.rstride;
     const double *f = a.f + base, *A = a.Af + base;
-    size_t off = (size_t)j * m.pitch + c;
-    double fc = f[off], ac = A[off];
-    if (target == fabs(ac) && fabs(fc) > 1e-14) {
-      size_t nb[4] = {off - m.pitch, off + m.pitch, (size_t)j * m.pitch + wrapc(c - 1, m.nu),
-                      (size_t)j * m.pitch + wrapc(c + 1, m.nu)};
-      for (int q = 0; q < 4; q++) {
-        double du = fc - f[nb[q]];
-        double dF = ac - A[nb[q]];
-        if (du * dF < -1e-14) best = fmin(-du / dF, best);
+    auto eval = [&](int j, int c) {
+      size_t off = (size_t)j * m.pitch + c;
+      double fc = f[off], ac = A[off];
+      if (target == fabs(ac) && fabs(fc) > 1e-14) {
+        size_t nb[4] = {off - m.pitch, off + m.pitch, (size_t)j * m.pitch + wrapc(c - 1, m.nu),
+                        (size_t)j * m.pitch + wrapc(c + 1, m.nu)};
+        for (int q = 0; q < 4; q++) {
+          double du = fc - f[nb[q]];
+          double dF = ac - A[nb[q]];
+          if (du * dF < -1e-14) best = fmin(-du / dF, best);
+        }
+      }
+    };
+    const int per = a.g.ntm * a.g.ctiles;
+    for (int k = threadIdx.x; k < per; k += blockDim.x) {
+      uint4 q = rec[(size_t)z * per + k];
+      unsigned long long b = (unsigned long long)q.x | ((unsigned long long)q.y << 32);
+      if (__longlong_as_double((long long)b) != target || q.z == 0) continue;
+      int j = k / a.g.ctiles, ct = k - j * a.g.ctiles;
+      if (j == 0 || j == m.ntm - 1) continue;
+      if (q.z == 1) {
+        int c = ct * kBlock + (int)q.w;
+        if (c < m.nu) eval(j, c);
+      } else {
+        for (int c = ct * kBlock; c < min(m.nu, (ct + 1) * kBlock); c++) eval(j, c);
       }
     }
   }
+  __shared__ double wb[4];
   for (int h = 32; h >= 1; h >>= 1) best = fmin(best, __shfl_xor(best, h, 64));
-  if ((threadIdx.x & 63) == 0 && best < 1.7976931348623157e308)
-    atomicMin(&a.dtmin[r], (unsigned long long)__double_as_longlong(best));
+  if ((threadIdx.x & 63) == 0) wb[threadIdx.x >> 6] = best;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double b = wb[0];
+    for (int q = 1; q < (int)(blockDim.x >> 6); q++) b = fmin(b, wb[q]);
+    if (b < 1.7976931348623157e308)
+      atomicMin(&a.dtmin[r], (unsigned long long)__double_as_longlong(b));
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -2707,7 +2760,7 @@ __global__ __launch_bounds__(kBlock) void k_flow_dtmax(const double *vt, const d
 // the device, so that a whole time step is one fixed launch sequence that
 // can be replayed as a hipGraph without a host round trip:
 //   k_plan_init;
-//   per round:  [k_plan_prep, k_laplacian, k_ptl_min]   (PTL sub-cycling)
+//   per round:  [k_plan_prep, k_laplacian, k_ptl_pick]  (PTL sub-cycling)
 //               k_plan_round, k_plan_coefs, k_rkg2_stage1,
 //               gcap x k_rkg2_fused2 (launch q: stages 2+qT .. 2+qT+T-1),
 //               k_copy_finals_plan;

@@ -211,6 +211,7 @@ struct hft_ctx {
   int *d_order = nullptr, *d_order_id = nullptr;
   double *d_cdt = nullptr, *d_ub1 = nullptr;
   unsigned long long *d_absmax = nullptr, *d_dtmin = nullptr, *d_target = nullptr;
+  uint4 *d_prec = nullptr;  // PTL tile records of k_laplacian (nr x ntm x column tiles)
   int *h_small = nullptr;  // pinned scratch for small uploads
   double *h_smalld = nullptr;
   std::vector<int64_t> last_s, last_cycles;
@@ -502,7 +503,8 @@ int setup_device(hft_ctx *c) {
   if ((rc = dalloc(&c->d_order, nr)) || (rc = dalloc(&c->d_s, nr)) ||
       (rc = dalloc(&c->d_cdt, nr)) || (rc = dalloc(&c->d_ub1, nr)) ||
       (rc = dalloc(&c->d_absmax, nr)) || (rc = dalloc(&c->d_dtmin, nr)) ||
-      (rc = dalloc(&c->d_target, 1)) || (rc = dalloc(&c->d_D, nr)))
+      (rc = dalloc(&c->d_target, 1)) || (rc = dalloc(&c->d_D, nr)) ||
+      (rc = dalloc(&c->d_prec, (size_t)nblocks(tiles_for(c, nr)))))
     return rc;
   HIPCHK(hipHostMalloc((void **)&c->h_small, sizeof(int) * (nr * 8 + 64)));
   HIPCHK(hipHostMalloc((void **)&c->h_smalld, sizeof(double) * (nr * 8 + 64)));
@@ -1367,14 +1369,14 @@ int ptl_pass(hft_ctx *c, const std::vector<int> &reals, bool coupled,
   HIPCHK(hipEventRecord(c->coef_ev, c->stream));
   LapArgs la;
   la.m = c->m; la.g = tiles_for(c, n); la.x = c->f; la.y = c->dty0; la.D = c->d_D;
-  la.order = c->d_order; la.rstride = c->rstride; la.absmax = c->d_absmax;
+  la.order = c->d_order; la.rstride = c->rstride; la.absmax = c->d_absmax; la.rec = c->d_prec;
   int rc = launch(c, 2, k_laplacian, nblocks(la.g), la, 2.0 * n);
   if (rc) return rc;
   PtlArgs pa;
   pa.m = c->m; pa.g = la.g; pa.f = c->f; pa.Af = c->dty0; pa.absmax = c->d_absmax;
   pa.dtmin = c->d_dtmin; pa.order = c->d_order; pa.coupled = coupled ? 1 : 0; pa.nlaunch = n;
-  pa.rstride = c->rstride;
-  rc = launch(c, 2, k_ptl_min, nblocks(pa.g), pa, 2.0 * n);
+  pa.rstride = c->rstride; pa.rec = c->d_prec;
+  rc = launch_ex(c, 2, k_ptl_pick, n, 256, 0, pa, 0.0);
   if (rc) return rc;
   std::vector<unsigned long long> mx(nr), mn(nr);
   HIPCHK(hipMemcpyAsync(mx.data(), c->d_absmax, nr * 8, hipMemcpyDeviceToHost, c->stream));
@@ -1550,7 +1552,8 @@ int hft_destroy(hft_ctx *c) {
   if (c->coef_ev) (void)hipEventDestroy(c->coef_ev);
   for (auto &g : c->graphs) (void)hipGraphExecDestroy(g.second);
   if (c->sg_exec) (void)hipGraphExecDestroy(c->sg_exec);
-  for (void *q : {(void *)c->d_plan, (void *)c->d_slots, (void *)c->d_htab, (void *)c->d_dte})
+  for (void *q : {(void *)c->d_plan, (void *)c->d_slots, (void *)c->d_htab, (void *)c->d_dte,
+                  (void *)c->d_prec})
     if (q) (void)hipFree(q);
   if (c->h_plan) (void)hipHostFree(c->h_plan);
   for (auto *vec : {&c->adj_fix, &c->adj_ck, &c->adj_w, &c->adj_st})
@@ -2354,12 +2357,13 @@ static int sg_enqueue(hft_ctx *c, double dtl, int strang, int subcycles, int64_t
       LapArgs la;
       la.m = c->m; la.g = gall; la.x = c->f; la.y = c->dty0; la.D = c->d_D;
       la.order = c->d_order; la.rstride = c->rstride; la.absmax = c->d_absmax; la.dn = status + 2;
+      la.rec = c->d_prec;
       if ((rc = launch(c, 2, k_laplacian, nblocks(gall), la, 0.0))) return rc;
       PtlArgs pa;
       pa.m = c->m; pa.g = gall; pa.f = c->f; pa.Af = c->dty0; pa.absmax = c->d_absmax;
       pa.dtmin = c->d_dtmin; pa.order = c->d_order; pa.coupled = p.coupled; pa.nlaunch = nr;
-      pa.rstride = c->rstride; pa.dn = status + 2;
-      if ((rc = launch(c, 2, k_ptl_min, nblocks(gall), pa, 0.0))) return rc;
+      pa.rstride = c->rstride; pa.dn = status + 2; pa.rec = c->d_prec;
+      if ((rc = launch_ex(c, 2, k_ptl_pick, nr, 256, 0, pa, 0.0))) return rc;
     }
     if ((rc = launch_ex(c, 2, k_plan_round, 1, 1024, 0, p, 0.0))) return rc;
     if ((rc = launch_ex(c, 0, k_plan_coefs, (ncoef + kBlock - 1) / kBlock, kBlock, 0, p, 0.0))) return rc;

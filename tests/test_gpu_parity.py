@@ -317,6 +317,53 @@ def test_ptl_diffusion_step_bitexact(gpu_available):
     assert_same(g.get_state("tp"), o.get_f(), "RKG2 + PTL")
 
 
+@pytest.mark.parametrize("mode", ["coupled", "independent"])
+def test_ptl_ties_bitexact(gpu_available, mode):
+    """The PTL step limit where max |L f| is reached at several points: a
+    map made of one 4-column pattern repeated exactly along phi (512
+    columns), so |L f| ties at 11 points of the equator row, several in one
+    column tile (the Laplacian pass records a tie count above one) and in
+    both tiles of the row; the limit binds (two cycles in 24 h).  Cycle
+    lengths, stage counts and the state equal the oracle's bit for bit."""
+    t, p = W.uniform_theta(91), W.uniform_phi(513)
+    th = np.asarray(t)[:, None]
+    per = 4
+    block = np.sin(th) ** 2 * np.cos(2 * np.pi * np.arange(per)[None, :] / per) * 10.0 \
+        + np.sin(th) ** 4 * 3.0
+    m = np.tile(block, (1, 512 // per))
+    m = np.concatenate([m, m[:, :1]], axis=1)
+    maps = np.stack([m, 0.5 * m])
+    eta = [800.0, 800.0]
+    g = FluxTransport(t, p, 2, mode=mode)
+    g.set_state(maps, layout="pt")
+    g.set_diffusion(eta)
+    if mode == "coupled":
+        o = Oracle(t, p, 2)
+        o.ingest_pt(maps)
+        o.load_diffusion(eta)
+        recs = []
+        res = o.diffusion_step(24.0, subcycles=0)
+        g.diffusion_step(24.0, subcycles=0, callback=lambda cyc, rows: recs.append(rows))
+        assert res["ncycles"] >= 2, res["ncycles"]
+        assert len(recs) == res["ncycles"]
+        for k, rows in enumerate(recs):
+            assert rows[0][1] == res["dt"][k]
+            assert rows[0][2] == res["s"][k]
+        assert_same(g.get_state("tp"), o.get_f(), "PTL with ties")
+    else:
+        g.diffusion_step(24.0, subcycles=0)
+        f = g.get_state("tp")
+        _, cyc = g.diffusion_stats()
+        assert cyc.max() >= 2, cyc
+        for r in range(2):
+            o1 = Oracle(t, p, 1)
+            o1.ingest_pt(maps[r:r + 1])
+            o1.load_diffusion([eta[r]])
+            o1.diffusion_step(24.0, subcycles=0)
+            assert_same(f[r:r + 1], o1.get_f(), f"PTL with ties, realization {r}")
+    g.close()
+
+
 def test_independent_mode_matches_single_runs(gpu_available):
     t, p = grid(61, 121)
     eta = [100.0, 350.0, 1000.0]
