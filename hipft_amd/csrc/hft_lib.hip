@@ -985,7 +985,7 @@ int fused_rows(hft_ctx *c, int active, int *H) {
       (int64_t)active * ((c->ntm + h - 1) / h) < c->fused_target_blocks) {
     // few realizations left: thinner bands so the launch still fills the GPU
     int64_t t = (int64_t)c->ntm * active / c->fused_target_blocks;
-    h = (int)std::max<int64_t>(4, std::min<int64_t>(h, t));
+    h = (int)std::max<int64_t>(2, std::min<int64_t>(h, t));
   }
   if (c->fused_bands && c->fused_kernel == 3) {
     int ncol = ((c->nu + 63) / 64) * 64;
@@ -2977,7 +2977,7 @@ int hft_set_tuning(hft_ctx *c, const char *key, int value) {
     if (value < 2 || value > 5) return fail(HFT_E_ARG, "fused_stages must be 2..5 (5: fused_kernel 3 only)");
     c->fused_T = value;
   } else if (!strcmp(key, "band_rows")) {
-    if (value < 4 || value > 4096) return fail(HFT_E_ARG, "band_rows must be 4..4096");
+    if (value < 2 || value > 4096) return fail(HFT_E_ARG, "band_rows must be 2..4096");
     c->band_rows = value;
   } else if (!strcmp(key, "fused_min_realizations")) {
     if (value < 1) return fail(HFT_E_ARG, "fused_min_realizations must be >= 1");
@@ -2999,7 +2999,7 @@ int hft_set_tuning(hft_ctx *c, const char *key, int value) {
     if (value < 0 || value > 1) return fail(HFT_E_ARG, "fused_bands must be 0 or 1");
     c->fused_bands = value;
   } else if (!strcmp(key, "fused_band_min_rows")) {
-    if (value < 4 || value > 64) return fail(HFT_E_ARG, "fused_band_min_rows must be 4..64");
+    if (value < 2 || value > 64) return fail(HFT_E_ARG, "fused_band_min_rows must be 2..64");
     c->fused_band_min_rows = value;
   } else if (!strcmp(key, "fused_band_share")) {
     if (value < 0 || value > 200) return fail(HFT_E_ARG, "fused_band_share must be 0..200");

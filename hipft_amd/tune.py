@@ -73,7 +73,7 @@ def tail_candidates():
     rows: the launches late in a super-step and in extra PTL rounds, which
     only whole steps of an evolving state exercise."""
     return [dict(fused_min_realizations=m, fused_band_min_rows=h)
-            for m in (1, 2, 4, 8) for h in (4, 8, 16)]
+            for m in (1, 2, 4, 8) for h in (2, 4, 8, 16)]
 
 
 def cache_key(model):

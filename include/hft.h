@@ -250,12 +250,12 @@ int hft_stream(hft_ctx *ctx, void **stream);
  *   "fused"                  1 = run RKG2 stages several per launch (default)
  *   "fused_stages"           stages per fused launch, 2..4 (default 4); 5 with
  *                            fused_kernel 3
- *   "band_rows"              theta rows per block of a fused launch (32)
+ *   "band_rows"              theta rows per block of a fused launch (32; 2..4096)
  *   "fused_min_realizations" fused launches only while at least this many
  *                            realizations are still stepping (8)
  *   "fused_target_blocks"    when a fused launch would have fewer blocks
  *                            (realizations x bands) than this, its bands are
- *                            made thinner (>= 4 rows) to fill the GPU (0: off)
+ *                            made thinner (>= 2 rows) to fill the GPU (0: off)
  *   "fused_kernel"           fused launch kernel: 1 = one phi column per
  *                            thread, 2 = the same with lean bookkeeping,
  *                            3 = two phi columns per thread (default)
@@ -268,7 +268,7 @@ int hft_stream(hft_ctx *ctx, void **stream);
  *   "fused_band_share"       that choice's model of how much longer a round
  *                            takes on a full device than on an idle one, in
  *                            percent (32)
- *   "fused_band_min_rows"    the thinnest band that choice may pick (8; 4..64):
+ *   "fused_band_min_rows"    the thinnest band that choice may pick (8; 2..64):
  *                            a launch over few realizations is bound by the
  *                            latency of its march, so thinner bands finish
  *                            sooner

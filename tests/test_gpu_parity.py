@@ -507,7 +507,7 @@ def test_atten_map_vs_reference(gpu_available):
 
 @pytest.mark.parametrize("kernel", [1, 2, 3])
 @pytest.mark.parametrize("T,H,B", [(2, 4, 0), (3, 7, 0), (4, 32, 0), (4, 5, 0), (4, 32, 40),
-                                   (3, 64, 1000)])
+                                   (3, 64, 1000), (4, 2, 0), (3, 3, 0)])
 def test_fused_rkg2_bitexact(gpu_available, T, H, B, kernel):
     """Several RKG2 stages per launch (LDS row rings, band halos) give the
     oracle's result bit for bit, across band boundaries and the poles, also
