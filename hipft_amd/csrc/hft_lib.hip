@@ -2263,6 +2263,18 @@ static int sg_prepare(hft_ctx *c, int64_t sbound, int gmax, int *maxtiles) {
     if ((rc = dalloc(&c->d_coef, cap))) return rc;
     c->coef_cap = cap;
   }
+  // the host controller that finishes an overflowing step fills the same
+  // table from pinned memory: size that buffer now, so no later round of a
+  // step allocates pinned memory (a stage count never exceeds sbound)
+  if (need > c->h_coef_cap) {
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipEventSynchronize(c->coef_ev));
+    if (c->h_coef) HIPCHK(hipHostFree(c->h_coef));
+    c->h_coef = nullptr;
+    size_t cap = std::max(need, (size_t)1024);
+    HIPCHK(hipHostMalloc((void **)&c->h_coef, cap * sizeof(StageCoef)));
+    c->h_coef_cap = cap;
+  }
   PlanLayout L = plan_layout(ng, nr);
   if (!c->d_plan) {
     if ((rc = dalloc(&c->d_plan, L.bytes))) return rc;
