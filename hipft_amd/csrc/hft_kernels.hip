@@ -3099,6 +3099,28 @@ __global__ __launch_bounds__(kBlock) void k_plan_coefs(PlanArgs p) {
   p.coef[(size_t)k * p.nr + r] = plan_sts_coef(s, k, p.cdt[r]);
 }
 
+// The stage table of a host-controlled super-step on the device (the
+// expressions of k_plan_coefs): realizations order[0..n) with stage counts
+// s and cycle steps cdt, one thread per (realization, stage k >= 2).
+struct CoefArgs {
+  StageCoef *coef;  // [k][nr]
+  const int *order;
+  const int64_t *s;
+  const double *cdt;
+  int nr, n, kspan;  // kspan: stages 2..kspan+1 per realization
+};
+
+__global__ __launch_bounds__(kBlock) void k_rkg2_coefs(CoefArgs a) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int z = i / a.kspan;
+  int64_t k = 2 + (i - z * a.kspan);
+  if (z >= a.n) return;
+  int r = a.order[z];
+  int64_t s = a.s[r];
+  if (k > s) return;
+  a.coef[(size_t)k * a.nr + r] = plan_sts_coef(s, k, a.cdt[r]);
+}
+
 // Final stages of a planned super-step into f: fused launch q writes its
 // last stage to src[q & 1], and realization r finishes in launch (s_r-2)/T.
 struct PlanCopyArgs {

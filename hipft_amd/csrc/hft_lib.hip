@@ -822,6 +822,34 @@ int upload_rkg2(hft_ctx *c, const std::vector<int> &reals, const std::vector<dou
   }
   // the previous upload from the pinned buffers must be complete
   HIPCHK(hipEventSynchronize(c->coef_ev));
+  if (!W) {
+    // the table is built on the device from the stage counts and cycle
+    // steps (k_rkg2_coefs: the expressions of sts_coefs), so only those go
+    // up, not the (smax + 1) x nr table of which the realizations still
+    // cycling use a few columns
+    int64_t *hs = (int64_t *)(c->h_small + 2 * nr + 16);
+    for (int z = 0; z < n; z++) {
+      int r = ord[z];
+      c->h_small[z] = r;
+      hs[r] = s[r];
+      c->h_smalld[r] = dt2[r];                                        // cycle dt
+      c->h_smalld[nr + r] = six / (double)((s[r] + 4) * (s[r] - 1));  // sts_ubj(1)
+    }
+    HIPCHK(hipMemcpyAsync(c->d_order, c->h_small, n * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->d_s, hs, nr * sizeof(int64_t), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->d_cdt, c->h_smalld, nr * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->d_ub1, c->h_smalld + nr, nr * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipEventRecord(c->coef_ev, c->stream));
+    if (smax >= 2) {
+      CoefArgs ca;
+      ca.coef = c->d_coef; ca.order = c->d_order; ca.s = c->d_s; ca.cdt = c->d_cdt;
+      ca.nr = nr; ca.n = n; ca.kspan = (int)(smax - 1);
+      int64_t threads = (int64_t)n * ca.kspan;
+      return launch_ex(c, 0, k_rkg2_coefs, (int)((threads + kBlock - 1) / kBlock), kBlock, 0, ca,
+                       0.0);
+    }
+    return 0;
+  }
   if (need > c->h_coef_cap) {
     if (c->h_coef) HIPCHK(hipHostFree(c->h_coef));
     size_t cap = std::max(need, (size_t)1024);
