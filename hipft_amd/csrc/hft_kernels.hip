@@ -2362,19 +2362,23 @@ __global__ __launch_bounds__(kBlock) void k_analysis_part(AnaArgs a) {
   for (int j = j0; j < j1; j++) {
     const double *row = a.f + base + (size_t)j * a.pitch;
     const double *vrow = a.fval ? a.fval + base + (size_t)j * a.pitch : nullptr;
-    for (int c = threadIdx.x; c < a.nu; c += kBlock) {
-      double f = row[c];
-      double fv = vrow ? (a.fval_scale ? vrow[c] * scale : vrow[c]) : f;
-      mn = fmin(f, mn);
-      mx = fmax(f, mx);
-      mnabs = fmin(fabs(f), mnabs);
-      double d = f - fv;
-      verr = verr + d * d;
-      sfv = sfv + fabs(f) * fabs(fv);
-    }
     double dat = a.da_t[j], snt = a.sn_t[j], cst = a.cs_t[j], tj = a.tt[j];
+    // one pass over the row: the reference's columns i = 1..npm-1 are the
+    // unique columns nu-1, 0, 1, ..., nu-1 (pcol, computed here rather than
+    // loaded, so a point's load does not wait on another); the
+    // min/max/error terms take each unique column once (i >= 1, column i-1)
     for (int i = threadIdx.x; i < a.nu + 1; i += kBlock) {
-      double f = row[a.pcol[i]];
+      double f = row[i == 0 ? a.nu - 1 : i - 1];
+      if (i >= 1) {
+        int c = i - 1;
+        double fv = vrow ? (a.fval_scale ? vrow[c] * scale : vrow[c]) : f;
+        mn = fmin(f, mn);
+        mx = fmax(f, mx);
+        mnabs = fmin(fabs(f), mnabs);
+        double d = f - fv;
+        verr = verr + d * d;
+        sfv = sfv + fabs(f) * fabs(fv);
+      }
       double dap = a.da_p[i];
       double fv = f * dat;
       fv = fv * dap;
