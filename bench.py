@@ -396,7 +396,7 @@ def main():
                          "calibration) followed by one step of the workload")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+        print(json.dumps({"error": "bench.py runs on one GPU; --gpus must be 1 "
                                    "(realizations would shard across GPUs with no per-step "
                                    "communication in independent mode)"}))
         sys.exit(2)

@@ -73,7 +73,7 @@ DEFAULTS = dict(
 
 MAX_REALIZATIONS = 2000  # src/hipft.f90:236
 
-# Reference options this build does not implement, with the only value it
+# Reference options the library does not implement, with the only value it
 # accepts: a hipft.in asking for anything else is refused rather than run
 # with a different method.
 UNSUPPORTED = (
@@ -93,7 +93,7 @@ UNSUPPORTED = (
 
 def check_config(c):
     """process_input_parameters' checks (src/hipft.f90:8482-8540) and the
-    options this build refuses; raises ValueError."""
+    options the library refuses; raises ValueError."""
     errs = []
     for key, ok, why in UNSUPPORTED:
         v = c.get(key, ok)

@@ -108,7 +108,7 @@ def test_run_parameters_used_matches_reference(tmp_path):
 
 def test_namelist_refuses_unknown_and_unsupported_items():
     """An item outside &hipft_input_parameters is an error (as for a Fortran
-    namelist read); reference options this build does not implement are
+    namelist read); reference options the library does not implement are
     refused instead of silently running another method."""
     import pytest
     base = "&hipft_input_parameters\n res_nt=31\n res_np=61\n time_end=2.0\n"
