@@ -1611,54 +1611,108 @@ struct LapArgs {
   uint4 *rec = nullptr;
 };
 
+// Rows of one k_laplacian block: a column tile of kLapRows consecutive rows,
+// whose interior points load their stencils before any arithmetic (four
+// rows of loads in flight per thread instead of one).
+static constexpr int kLapRows = 4;
+
+__device__ inline int lap_groups(int ntm) { return (ntm + kLapRows - 1) / kLapRows; }
+
 __global__ __launch_bounds__(kBlock) void k_laplacian(LapArgs a) {
-  int t = tile_index(a.g, a.dn);
-  if (t < 0) return;
-  int z, j, ct;
-  tile_of(t, a.g, z, j, ct);
-  int r = a.order[z];
   const Mesh &m = a.m;
-  int c = ct * kBlock + threadIdx.x;
-  size_t base = (size_t)r * a.rstride;
+  const int ng = lap_groups(m.ntm);
+  int nt = (int)gridDim.x;
+  if (a.dn) {
+    nt = *a.dn * ng * a.g.ctiles;
+    if ((int)blockIdx.x >= nt) return;
+  }
+  const int t = xcd_remap(blockIdx.x, nt);
+  const int ct = t % a.g.ctiles, rg = t / a.g.ctiles;
+  const int jg = rg % ng, z = rg / ng;
+  const int r = a.order[z];
+  const int c = ct * kBlock + threadIdx.x, j0 = jg * kLapRows;
+  const bool inc = c < m.nu;
+  const size_t base = (size_t)r * a.rstride;
   const double *x = a.x + base;
-  double L = (j == 0 || j == m.ntm - 1 || c < m.nu)
-                 ? laplacian(m, a.D[r], x, j, c < m.nu ? c : 0, m.ntm, m.nu, m.pitch)
-                 : 0.0;
-  double cand = 0.0;
-  if (c < m.nu) {
-    size_t off = (size_t)j * m.pitch + c;
-    a.y[base + off] = L;
-    if (fabs(x[off]) > 1e-14) cand = fabs(L);
+  const double D = a.D[r];
+  double L[kLapRows];
+  // interior rows: every load unconditional (row and column clamped into
+  // the interior, which has rows 1 .. ntm-2 as ntm >= 5), the value kept
+  // where the point is an interior one
+  const int cc = inc ? c : 0;
+#pragma unroll
+  for (int q = 0; q < kLapRows; q++) {
+    int j = min(max(j0 + q, 1), m.ntm - 2);
+    L[q] = lap_interior(m, D, x, j, cc, m.nu, m.pitch);
+  }
+  // polar rows: every thread of the row takes part in the wavefront sum
+  if (j0 == 0) L[0] = lap_pole(m, D, x, true, m.pitch, m.ntm, m.nu);
+  const int qs = m.ntm - 1 - j0;
+  if (qs < kLapRows) {
+    double ls = lap_pole(m, D, x, false, m.pitch, m.ntm, m.nu);
+#pragma unroll
+    for (int q = 0; q < kLapRows; q++)
+      if (q == qs) L[q] = ls;
+  }
+  double cand[kLapRows];
+#pragma unroll
+  for (int q = 0; q < kLapRows; q++) {
+    cand[q] = 0.0;
+    int j = j0 + q;
+    if (inc && j < m.ntm) {
+      size_t off = (size_t)j * m.pitch + c;
+      a.y[base + off] = L[q];
+      if (fabs(x[off]) > 1e-14) cand[q] = fabs(L[q]);
+    }
   }
   if (a.absmax) {
-    // block max, then one atomic per block (|Af| >= 0: the integer order of
-    // the bit patterns is the numeric order)
-    __shared__ double wmax[kBlock / 64];
-    __shared__ int wcnt[kBlock / 64], wfirst[kBlock / 64];
-    double wm = cand;
-    for (int h = 32; h >= 1; h >>= 1) wm = fmax(wm, __shfl_xor(wm, h, 64));
-    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = wm;
+    // per row: the tile's max, then one atomic per block for the rows'
+    // max (|Af| >= 0: the integer order of the bit patterns is the numeric
+    // order)
+    constexpr int kW = kBlock / 64;
+    __shared__ double wmax[kLapRows][kW];
+    __shared__ int wcnt[kLapRows][kW], wfirst[kLapRows][kW];
+    const int w = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < kLapRows; q++) {
+      double wm = cand[q];
+      for (int h = 32; h >= 1; h >>= 1) wm = fmax(wm, __shfl_xor(wm, h, 64));
+      if ((threadIdx.x & 63) == 0) wmax[q][w] = wm;
+    }
     __syncthreads();
-    double mx = wmax[0];
-    for (int q = 1; q < kBlock / 64; q++) mx = fmax(mx, wmax[q]);
-    if (threadIdx.x == 0 && mx > 0.0)
-      atomicMax(&a.absmax[r], (unsigned long long)__double_as_longlong(mx));
+    double mx[kLapRows], bmax = 0.0;
+#pragma unroll
+    for (int q = 0; q < kLapRows; q++) {
+      mx[q] = wmax[q][0];
+      for (int v = 1; v < kW; v++) mx[q] = fmax(mx[q], wmax[q][v]);
+      bmax = fmax(bmax, mx[q]);
+    }
+    if (threadIdx.x == 0 && bmax > 0.0)
+      atomicMax(&a.absmax[r], (unsigned long long)__double_as_longlong(bmax));
     if (a.rec) {
-      // the tile's record: its max, how many points reach it, the first one
-      unsigned long long hit = __ballot(mx > 0.0 && cand == mx);
-      if ((threadIdx.x & 63) == 0) {
-        wcnt[threadIdx.x >> 6] = __popcll(hit);
-        wfirst[threadIdx.x >> 6] = hit ? (threadIdx.x & ~63) + __ffsll((long long)hit) - 1 : -1;
+      // each row tile's record: its max, how many points reach it, the
+      // first one
+#pragma unroll
+      for (int q = 0; q < kLapRows; q++) {
+        unsigned long long hit = __ballot(mx[q] > 0.0 && cand[q] == mx[q]);
+        if ((threadIdx.x & 63) == 0) {
+          wcnt[q][w] = __popcll(hit);
+          wfirst[q][w] = hit ? (threadIdx.x & ~63) + __ffsll((long long)hit) - 1 : -1;
+        }
       }
       __syncthreads();
-      if (threadIdx.x == 0) {
+      const int q = threadIdx.x;
+      if (q < kLapRows && j0 + q < m.ntm) {
         int cnt = 0, first = -1;
-        for (int q = 0; q < kBlock / 64; q++) {
-          cnt += wcnt[q];
-          if (first < 0) first = wfirst[q];
+        for (int v = 0; v < kW; v++) {
+          cnt += wcnt[q][v];
+          if (first < 0) first = wfirst[q][v];
         }
-        unsigned long long b = (unsigned long long)__double_as_longlong(mx);
-        a.rec[t] = make_uint4((unsigned)b, (unsigned)(b >> 32), (unsigned)cnt, (unsigned)first);
+        double mq = wmax[q][0];
+        for (int v = 1; v < kW; v++) mq = fmax(mq, wmax[q][v]);
+        unsigned long long b = (unsigned long long)__double_as_longlong(mq);
+        size_t rt = ((size_t)z * m.ntm + (j0 + q)) * a.g.ctiles + ct;
+        a.rec[rt] = make_uint4((unsigned)b, (unsigned)(b >> 32), (unsigned)cnt, (unsigned)first);
       }
     }
   }
@@ -2350,6 +2404,10 @@ __device__ inline void block_reduce(double *sh, double v, int op, double *out) {
   __syncthreads();
 }
 
+// Columns per chunk of a k_analysis_part thread (1280 = 5 x 256 columns
+// cover the 1024-column map in one chunk).
+static constexpr int kAnaU = 5;
+
 __global__ __launch_bounds__(kBlock) void k_analysis_part(AnaArgs a) {
   int r = blockIdx.y, b = blockIdx.x;
   const double huge = 1.7976931348623157e308, tiny = 2.2250738585072014e-308;
@@ -2366,36 +2424,49 @@ __global__ __launch_bounds__(kBlock) void k_analysis_part(AnaArgs a) {
     // one pass over the row: the reference's columns i = 1..npm-1 are the
     // unique columns nu-1, 0, 1, ..., nu-1 (pcol, computed here rather than
     // loaded, so a point's load does not wait on another); the
-    // min/max/error terms take each unique column once (i >= 1, column i-1)
-    for (int i = threadIdx.x; i < a.nu + 1; i += kBlock) {
-      double f = row[i == 0 ? a.nu - 1 : i - 1];
-      if (i >= 1) {
-        int c = i - 1;
-        double fv = vrow ? (a.fval_scale ? vrow[c] * scale : vrow[c]) : f;
-        mn = fmin(f, mn);
-        mx = fmax(f, mx);
-        mnabs = fmin(fabs(f), mnabs);
-        double d = f - fv;
-        verr = verr + d * d;
-        sfv = sfv + fabs(f) * fabs(fv);
+    // min/max/error terms take each unique column once (i >= 1, column i-1).
+    // A thread takes its columns in chunks of kAnaU, every load of a chunk
+    // issued before its arithmetic, which runs in the column order.
+    for (int i0 = threadIdx.x; i0 < a.nu + 1; i0 += kAnaU * kBlock) {
+      double fr[kAnaU];
+#pragma unroll
+      for (int u = 0; u < kAnaU; u++) {
+        int i = i0 + u * kBlock;
+        fr[u] = i < a.nu + 1 ? row[i == 0 ? a.nu - 1 : i - 1] : 0.0;
       }
-      double dap = a.da_p[i];
-      double fv = f * dat;
-      fv = fv * dap;
-      if (fv > 0.0) fp = fp + fv; else fm = fm + fv;
-      double area = dat * dap;
-      if (tj <= a.lat_n) {
-        if (fv > 0.0) fppn = fppn + fv; else fmpn = fmpn + fv;
-        apn = apn + area;
+#pragma unroll
+      for (int u = 0; u < kAnaU; u++) {
+        int i = i0 + u * kBlock;
+        if (i >= a.nu + 1) break;
+        double f = fr[u];
+        if (i >= 1) {
+          int c = i - 1;
+          double fv = vrow ? (a.fval_scale ? vrow[c] * scale : vrow[c]) : f;
+          mn = fmin(f, mn);
+          mx = fmax(f, mx);
+          mnabs = fmin(fabs(f), mnabs);
+          double d = f - fv;
+          verr = verr + d * d;
+          sfv = sfv + fabs(f) * fabs(fv);
+        }
+        double dap = a.da_p[i];
+        double fv = f * dat;
+        fv = fv * dap;
+        if (fv > 0.0) fp = fp + fv; else fm = fm + fv;
+        double area = dat * dap;
+        if (tj <= a.lat_n) {
+          if (fv > 0.0) fppn = fppn + fv; else fmpn = fmpn + fv;
+          apn = apn + area;
+        }
+        if (tj >= a.lat_s) {
+          if (fv > 0.0) fpps = fpps + fv; else fmps = fmps + fv;
+          aps = aps + area;
+        }
+        double q = f * snt;
+        e1 = e1 + ((q * a.cs_p[i]) * dat) * dap;
+        e2 = e2 + ((q * a.sn_p[i]) * dat) * dap;
+        ax = ax + ((f * cst) * dat) * dap;
       }
-      if (tj >= a.lat_s) {
-        if (fv > 0.0) fpps = fpps + fv; else fmps = fmps + fv;
-        aps = aps + area;
-      }
-      double q = f * snt;
-      e1 = e1 + ((q * a.cs_p[i]) * dat) * dap;
-      e2 = e2 + ((q * a.sn_p[i]) * dat) * dap;
-      ax = ax + ((f * cst) * dat) * dap;
     }
   }
   double *o = a.part + ((size_t)r * a.nblk + b) * kNPart;

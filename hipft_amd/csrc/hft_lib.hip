@@ -354,6 +354,8 @@ Tiles tiles_for(const hft_ctx *c, int nreal) {
   return g;
 }
 int nblocks(const Tiles &g) { return g.ctiles * g.ntm * g.nreal; }
+// k_laplacian: one block per column tile of kLapRows rows
+int lap_blocks(const Tiles &g) { return g.ctiles * ((g.ntm + kLapRows - 1) / kLapRows) * g.nreal; }
 
 // ---- set_mesh: src/hipft.f90:6495-6653 ------------------------------------
 void build_mesh(hft_ctx *c, const double *tin, const double *pin) {
@@ -1398,7 +1400,7 @@ int ptl_pass(hft_ctx *c, const std::vector<int> &reals, bool coupled,
   LapArgs la;
   la.m = c->m; la.g = tiles_for(c, n); la.x = c->f; la.y = c->dty0; la.D = c->d_D;
   la.order = c->d_order; la.rstride = c->rstride; la.absmax = c->d_absmax; la.rec = c->d_prec;
-  int rc = launch(c, 2, k_laplacian, nblocks(la.g), la, 2.0 * n);
+  int rc = launch(c, 2, k_laplacian, lap_blocks(la.g), la, 2.0 * n);
   if (rc) return rc;
   PtlArgs pa;
   pa.m = c->m; pa.g = la.g; pa.f = c->f; pa.Af = c->dty0; pa.absmax = c->d_absmax;
@@ -2398,7 +2400,7 @@ static int sg_enqueue(hft_ctx *c, double dtl, int strang, int subcycles, int64_t
       la.m = c->m; la.g = gall; la.x = c->f; la.y = c->dty0; la.D = c->d_D;
       la.order = c->d_order; la.rstride = c->rstride; la.absmax = c->d_absmax; la.dn = status + 2;
       la.rec = c->d_prec;
-      if ((rc = launch(c, 2, k_laplacian, nblocks(gall), la, 0.0))) return rc;
+      if ((rc = launch(c, 2, k_laplacian, lap_blocks(gall), la, 0.0))) return rc;
       PtlArgs pa;
       pa.m = c->m; pa.g = gall; pa.f = c->f; pa.Af = c->dty0; pa.absmax = c->d_absmax;
       pa.dtmin = c->d_dtmin; pa.order = c->d_order; pa.coupled = p.coupled; pa.nlaunch = nr;
@@ -2607,7 +2609,7 @@ int hft_apply_diffusion_operator(hft_ctx *c, const double *x, double *y) {
   LapArgs la;
   la.m = c->m; la.g = tiles_for(c, c->nr); la.x = c->buf0; la.y = c->buf1; la.D = c->d_D;
   la.order = c->d_order_id; la.rstride = c->rstride; la.absmax = nullptr;
-  if ((rc = launch(c, 2, k_laplacian, nblocks(la.g), la, 2.0 * c->nr))) return rc;
+  if ((rc = launch(c, 2, k_laplacian, lap_blocks(la.g), la, 2.0 * c->nr))) return rc;
   if ((rc = get_field(c, c->buf1, dev))) return rc;
   dev_to_tp(c, dev, y);
   return 0;
