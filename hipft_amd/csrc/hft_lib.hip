@@ -217,6 +217,7 @@ struct hft_ctx {
   std::vector<int64_t> last_s, last_cycles;
   // analysis
   double *d_part = nullptr, *d_hist = nullptr;
+  double *h_hist = nullptr;  // pinned copy of d_hist (hft_analysis)
   double *d_da_t = nullptr, *d_sn_t = nullptr, *d_cs_t = nullptr, *d_tt = nullptr;
   double *d_da_p = nullptr, *d_cs_p = nullptr, *d_sn_p = nullptr;
   int *d_pcol = nullptr;
@@ -547,6 +548,7 @@ int setup_device(hft_ctx *c) {
       (rc = dalloc(&c->d_hist, (size_t)nr * 14)) || (rc = dalloc(&c->d_apar, (size_t)nr * 4)) ||
       (rc = dalloc(&c->d_ipar, (size_t)nr * 4)))
     return rc;
+  HIPCHK(hipHostMalloc((void **)&c->h_hist, (size_t)nr * 14 * sizeof(double)));
   return 0;
 }
 
@@ -1586,6 +1588,7 @@ int hft_destroy(hft_ctx *c) {
                   (void *)c->d_prec})
     if (q) (void)hipFree(q);
   if (c->h_plan) (void)hipHostFree(c->h_plan);
+  if (c->h_hist) (void)hipHostFree(c->h_hist);
   for (auto *vec : {&c->adj_fix, &c->adj_ck, &c->adj_w, &c->adj_st})
     for (double *q : *vec)
       if (q) (void)hipFree(q);
@@ -2911,8 +2914,12 @@ int hft_analysis(hft_ctx *c, const double *fval_pt, const double *fval_scale, do
     HIPCHK(hipEventRecord(e1, c->stream));
     c->pending.push_back({e0, e1, 3, 1.0 * nr, 1.0 * nr});
   }
-  HIPCHK(hipMemcpyAsync(out, c->d_hist, (size_t)nr * 14 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  // through pinned memory: a copy into the caller's pageable array would be
+  // staged by the runtime
+  HIPCHK(hipMemcpyAsync(c->h_hist, c->d_hist, (size_t)nr * 14 * sizeof(double),
+                        hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
+  memcpy(out, c->h_hist, (size_t)nr * 14 * sizeof(double));
   return 0;
 }
 
