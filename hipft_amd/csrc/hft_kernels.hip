@@ -2497,33 +2497,45 @@ __global__ __launch_bounds__(kBlock) void k_analysis_part(AnaArgs a) {
   }
 }
 
+// kNPart lanes per realization (launched with kNPart * nr threads in blocks
+// of 64): lane q combines partial q over the row blocks in block order, then
+// the realization's first lane forms the 14 history values.
 __global__ void k_analysis_final(const double *part, int nblk, int nr, int ntm, int npm,
                                  double *out) {
-  int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= nr) return;
+  static_assert(64 % kNPart == 0, "a realization's lanes lie in one wavefront");
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  const int r = g / kNPart, q = g % kNPart;
   const double huge = 1.7976931348623157e308, tiny = 2.2250738585072014e-308;
-  double acc[kNPart];
-  for (int q = 0; q < kNPart; q++) acc[q] = 0.0;
-  acc[0] = huge;
-  acc[1] = tiny;
-  acc[2] = huge;
-  for (int b = 0; b < nblk; b++) {
-    const double *p = part + ((size_t)r * nblk + b) * kNPart;
-    acc[0] = fmin(acc[0], p[0]);
-    acc[1] = fmax(acc[1], p[1]);
-    acc[2] = fmin(acc[2], p[2]);
-    for (int q = 3; q < kNPart; q++) acc[q] = acc[q] + p[q];
+  const int op = q == 0 || q == 2 ? 0 : (q == 1 ? 1 : 2);  // min, max, sum
+  double acc = op == 0 ? huge : (op == 1 ? tiny : 0.0);
+  auto comb = [&](double v) { acc = op == 0 ? fmin(acc, v) : (op == 1 ? fmax(acc, v) : acc + v); };
+  if (r < nr) {
+    const double *pr = part + (size_t)r * nblk * kNPart + q;
+    int b = 0;
+    for (; b + 8 <= nblk; b += 8) {
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) v[u] = pr[(size_t)(b + u) * kNPart];
+#pragma unroll
+      for (int u = 0; u < 8; u++) comb(v[u]);
+    }
+    for (; b < nblk; b++) comb(pr[(size_t)b * kNPart]);
   }
+  double accs[kNPart];
+  const int lane0 = (threadIdx.x & 63) & ~(kNPart - 1);
+#pragma unroll
+  for (int k = 0; k < kNPart; k++) accs[k] = __shfl(acc, lane0 + k, 64);
+  if (r >= nr || q != 0) return;
   const double rsun_cm2 = 4.84416e21, three_quarter = 0.75, pi_i = 0.3183098861837907;
   double *o = out + (size_t)r * 14;
-  o[0] = acc[0];
-  o[1] = acc[1];
-  o[2] = acc[2];
-  o[13] = (acc[4] == 0.0) ? sqrt(acc[3] / (double)(ntm * (npm - 1))) : sqrt(acc[3] / acc[4]);
-  for (int q = 0; q < 8; q++) o[3 + q] = rsun_cm2 * acc[5 + q];
-  o[12] = three_quarter * pi_i * acc[15];
-  double e1 = three_quarter * pi_i * acc[13];
-  double e2 = three_quarter * pi_i * acc[14];
+  o[0] = accs[0];
+  o[1] = accs[1];
+  o[2] = accs[2];
+  o[13] = (accs[4] == 0.0) ? sqrt(accs[3] / (double)(ntm * (npm - 1))) : sqrt(accs[3] / accs[4]);
+  for (int k = 0; k < 8; k++) o[3 + k] = rsun_cm2 * accs[5 + k];
+  o[12] = three_quarter * pi_i * accs[15];
+  double e1 = three_quarter * pi_i * accs[13];
+  double e2 = three_quarter * pi_i * accs[14];
   o[11] = sqrt(e1 * e1 + e2 * e2);
 }
 

@@ -2907,7 +2907,7 @@ int hft_analysis(hft_ctx *c, const double *fval_pt, const double *fval_scale, do
   }
   hipLaunchKernelGGL(k_analysis_part, dim3(c->ana_nblk, nr), dim3(kBlock), 0, c->stream, a);
   HIPCHK(hipGetLastError());
-  hipLaunchKernelGGL(k_analysis_final, dim3((nr + 63) / 64), dim3(64), 0, c->stream,
+  hipLaunchKernelGGL(k_analysis_final, dim3((nr * kNPart + 63) / 64), dim3(64), 0, c->stream,
                      (const double *)c->d_part, c->ana_nblk, nr, c->ntm, c->npm, c->d_hist);
   HIPCHK(hipGetLastError());
   if (c->timing) {
@@ -2951,7 +2951,7 @@ int hft_analysis_noalloc(hft_ctx *c, const double *f_dev, int64_t ld, double *ou
   // every partial is written before it is read: nothing to clear
   hipLaunchKernelGGL(k_analysis_part, dim3(c->ana_nblk, nr), dim3(kBlock), 0, c->stream, a);
   HIPCHK(hipGetLastError());
-  hipLaunchKernelGGL(k_analysis_final, dim3((nr + 63) / 64), dim3(64), 0, c->stream,
+  hipLaunchKernelGGL(k_analysis_final, dim3((nr * kNPart + 63) / 64), dim3(64), 0, c->stream,
                      (const double *)c->scratch_dev, c->ana_nblk, nr, c->ntm, c->npm, out_dev);
   HIPCHK(hipGetLastError());
   return 0;
