@@ -390,8 +390,10 @@ def test_independent_mode_matches_single_runs(gpu_available):
     assert len(set(s.tolist())) > 1  # stage counts differ per realization
 
 
-def test_analysis_matches_oracle(gpu_available):
-    o, g = pair(91, 181, nr=2)
+@pytest.mark.parametrize("nr", [2, 5])
+def test_analysis_matches_oracle(gpu_available, nr):
+    # nr = 5: the final reduction's 16 lanes per realization span two wavefronts
+    o, g = pair(91, 181, nr=nr)
     ho = o.analysis()
     hg = g.analysis()
     assert np.array_equal(hg[:, :3], ho[:, :3])
