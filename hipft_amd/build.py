@@ -22,10 +22,6 @@ HIPCC_FLAGS = [
     "-ffp-contract=off",
     # no packed FP32 instructions
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    # instruction scheduling that favours independent work in flight: the
-    # fused RKG2 march is bound by latency at two wavefronts per SIMD
-    # (1.5 % per step, same results)
-    "-mllvm", "-amdgpu-sched-strategy=max-ilp",
     "-Wno-unused-result",
 ]
 
