@@ -1619,8 +1619,8 @@ static constexpr int kLapRows = 4;
 __device__ inline int lap_groups(int ntm) { return (ntm + kLapRows - 1) / kLapRows; }
 
 // Eight wavefronts per SIMD (at most 64 VGPRs): more rows in flight per CU
-// than the five the unconstrained allocation (86 VGPRs) allows, 150 -> 132 us
-// per PTL pass at 64 realizations of 512x1024.
+// than the five the unconstrained allocation (over 80 VGPRs) allows,
+// 150 -> 132 us per PTL pass at 64 realizations of 512x1024.
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) void k_laplacian(LapArgs a) {
   const Mesh &m = a.m;
   const int ng = lap_groups(m.ntm);
