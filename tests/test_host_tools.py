@@ -4,12 +4,22 @@ import json
 import os
 import subprocess
 import sys
+import types
 
 import pytest
 
 from hipft_amd import tune
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+# The tuner's clock in these tests: the fake model advances it by its cost,
+# so the search does not depend on how long sleeps take on a busy host.
+_CLOCK = [0.0]
+
+
+@pytest.fixture(autouse=True)
+def _fake_clock(monkeypatch):
+    monkeypatch.setattr(tune, "time", types.SimpleNamespace(perf_counter=lambda: _CLOCK[0]))
 
 
 class FakeModel:
@@ -37,18 +47,15 @@ class FakeModel:
         return True
 
     def diffusion_step(self, dt, subcycles):
-        import time
         self.calls += 1
-        time.sleep(1e-4 * self.cost(self.cfg))
+        _CLOCK[0] += 1e-4 * self.cost(self.cfg)
 
     def advection_step(self, dt):
-        import time
-        time.sleep(1e-4 * (2.0 if self.cfg["advection_band"] == 0 else 1.0))
+        _CLOCK[0] += 1e-4 * (2.0 if self.cfg["advection_band"] == 0 else 1.0)
 
     def flux_transport_step(self, dt, strang, subcycles):
-        import time
         self.steps = getattr(self, "steps", 0) + 1
-        time.sleep(1e-4 * self.step_cost(self.cfg))
+        _CLOCK[0] += 1e-4 * self.step_cost(self.cfg)
 
     step_cost = staticmethod(lambda c: 1.0)
 
